@@ -1,0 +1,627 @@
+// Host runtime behind the C ABI (include/cr/cr.h): state, validation,
+// capacity management with grow-and-retry, stage launches and statistics.
+//
+// Mirrors FW::CudaRaster (src/cudaraster/CudaRaster.cpp): the same state
+// machine, the same validity checks, the same overflow-retry contract
+// (:277-339), per-stage events for getStats (:346-363).  Buffers are plain
+// device allocations owned by the context (the reference's lazily migrating
+// FW::Buffer is not needed: inputs are device pointers, surfaces are device
+// buffers).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "cr/cr.h"
+#include "cr_device.h"
+
+using namespace crhip;
+
+struct cr_surface {
+  int32_t width = 0, height = 0, format = 0, samples = 1, samplesLog2 = 0;
+  int32_t roundedW = 0, roundedH = 0, textureW = 0;
+  uint32_t* dev = nullptr;
+  size_t bytes = 0;
+  int device = 0;
+};
+
+namespace {
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t need) {
+    if (need <= bytes && ptr) return hipSuccess;
+    if (ptr) {
+      hipError_t e = hipFree(ptr);
+      if (e != hipSuccess) return e;
+      ptr = nullptr;
+      bytes = 0;
+    }
+    size_t alloc = std::max<size_t>(need, 256);
+    hipError_t e = hipMalloc(&ptr, alloc);
+    if (e == hipSuccess) bytes = alloc;
+    return e;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+thread_local std::string g_lastError;
+
+}  // namespace
+
+struct cr_context {
+  std::string err;
+  int device = 0;
+  cr_surface* color = nullptr;
+  cr_surface* depth = nullptr;
+  cr_surface* triid = nullptr;
+  bool deferredClear = false;
+  uint32_t clearColor = 0, clearDepth = 0;
+  bool pipeSet = false;
+  cr_pipe_desc pipe{};
+  const uint8_t* vbuf = nullptr;
+  int64_t vofs = 0, numVerts = 0;
+  const uint8_t* ibuf = nullptr;
+  int64_t iofs = 0;
+  int32_t numTris = 0;
+  cr_debug_params dbg{};
+
+  int64_t maxSubtris = 1, maxBinEntries = 1, maxTileEntries = 1;
+  DevBuf triSubtris, triHeader, triData, subtriTri;
+  DevBuf binChunkCount, binTotal, binStart, binList;
+  DevBuf tileStart, tileCount, tileList;
+  DevBuf counters;
+  Counters* hostCounters = nullptr;   // pinned
+  hipEvent_t ev[5] = {};
+  bool evRecorded = false;
+  hipStream_t pendingStream = nullptr;
+  bool pending = false;
+  cr_counters last{};
+};
+
+namespace {
+
+cr_status fail(cr_context* ctx, cr_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  g_lastError = buf;
+  return st;
+}
+
+#define CR_HIP(ctx, call)                                                                      \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return fail((ctx), CR_ERR_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+int popc8(int v) {
+  int c = 0;
+  for (int i = 0; i < 8; i++) c += (v >> i) & 1;
+  return c;
+}
+
+// Vec4f::toABGR (base/Math.cpp:41-48), the host-side quantiser used by
+// deferredClear.
+uint32_t hostChannel(float c) {
+  float cl = c;
+  if (!(cl >= 0.0f)) cl = 0.0f;
+  if (cl > 1.0f) cl = 1.0f;
+  uint64_t q = (uint64_t)((double)cl * 72057594037927936.0);
+  return (uint32_t)((((q * 255u) >> 55) + 1u) >> 1);
+}
+
+// encodeDepth (cudaraster/cuda/Util.hpp:290-298)
+uint32_t encodeDepth(uint32_t depth) {
+  double v = (double)depth;
+  v /= 65536.0 * 65536.0 - 1.0;
+  v = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+  v *= (double)(kDepthMax - kDepthMin);
+  v += (double)kDepthMin;
+  return (uint32_t)v;
+}
+
+struct Geometry {
+  int32_t vw, vh, wp, hp, wt, ht, nt, wb, hb, nb;
+};
+
+Geometry geometryOf(const cr_surface* s) {
+  Geometry g;
+  g.vw = s->width;
+  g.vh = s->height;
+  g.wp = s->roundedW;
+  g.hp = s->roundedH;
+  g.wt = g.wp >> kTileLog2;
+  g.ht = g.hp >> kTileLog2;
+  g.nt = g.wt * g.ht;
+  g.wb = (g.wt + (1 << kBinLog2) - 1) >> kBinLog2;
+  g.hb = (g.ht + (1 << kBinLog2) - 1) >> kBinLog2;
+  g.nb = g.wb * g.hb;
+  return g;
+}
+
+cr_status validateDraw(cr_context* ctx) {
+  if (!ctx->color) return fail(ctx, CR_ERR_STATE, "CudaRaster: Surfaces not set!");
+  if (!ctx->pipeSet) return fail(ctx, CR_ERR_STATE, "CudaRaster: Pixel pipe not set!");
+  if (!ctx->vbuf && ctx->numTris > 0) return fail(ctx, CR_ERR_STATE, "CudaRaster: Vertex buffer not set!");
+  if (!ctx->ibuf && ctx->numTris > 0) return fail(ctx, CR_ERR_STATE, "CudaRaster: Index buffer not set!");
+  if (ctx->pipe.samples_log2 != ctx->color->samplesLog2)
+    return fail(ctx, CR_ERR_STATE, "CudaRaster: Mismatch in multisampling between pixel pipe and surface!");
+  if (ctx->triid && (ctx->triid->width != ctx->color->width || ctx->triid->height != ctx->color->height ||
+                     ctx->triid->samples != ctx->color->samples))
+    return fail(ctx, CR_ERR_STATE, "triangle-id surface does not match the colour surface");
+  return CR_OK;
+}
+
+cr_status ensureBuffers(cr_context* ctx, const Geometry& g, int32_t numChunks) {
+  int64_t nt = ctx->numTris;
+  CR_HIP(ctx, ctx->triSubtris.ensure((size_t)std::max<int64_t>(nt, 1)));
+  CR_HIP(ctx, ctx->triHeader.ensure((size_t)ctx->maxSubtris * sizeof(Header)));
+  CR_HIP(ctx, ctx->triData.ensure((size_t)ctx->maxSubtris * sizeof(TriData)));
+  CR_HIP(ctx, ctx->subtriTri.ensure((size_t)std::max<int64_t>(ctx->maxSubtris - nt, 1) * 4));
+  CR_HIP(ctx, ctx->binChunkCount.ensure((size_t)g.nb * std::max(numChunks, 1) * 4));
+  CR_HIP(ctx, ctx->binTotal.ensure((size_t)kMaxBins * 4));
+  CR_HIP(ctx, ctx->binStart.ensure((size_t)kMaxBins * 4));
+  CR_HIP(ctx, ctx->binList.ensure((size_t)ctx->maxBinEntries * 4));
+  CR_HIP(ctx, ctx->tileStart.ensure((size_t)g.nt * 4));
+  CR_HIP(ctx, ctx->tileCount.ensure((size_t)g.nt * 4));
+  CR_HIP(ctx, ctx->tileList.ensure((size_t)ctx->maxTileEntries * 4));
+  return CR_OK;
+}
+
+Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks) {
+  Params P;
+  memset(&P, 0, sizeof(P));
+  P.numTris = ctx->numTris;
+  P.numVertices = (int32_t)std::min<int64_t>(ctx->numVerts, INT32_MAX);
+  P.vertexStride = ctx->pipe.vertex_struct_size;
+  P.vertices = ctx->vbuf ? ctx->vbuf + ctx->vofs : nullptr;
+  P.indices = ctx->ibuf ? reinterpret_cast<const int32_t*>(ctx->ibuf + ctx->iofs) : nullptr;
+  P.viewportWidth = g.vw;
+  P.viewportHeight = g.vh;
+  P.widthPixels = g.wp;
+  P.heightPixels = g.hp;
+  P.widthTiles = g.wt;
+  P.heightTiles = g.ht;
+  P.numTiles = g.nt;
+  P.widthBins = g.wb;
+  P.heightBins = g.hb;
+  P.numBins = g.nb;
+  P.samplesLog2 = ctx->pipe.samples_log2;
+  P.flags = ctx->pipe.render_mode_flags & (kFlagDepth | kFlagLerp);
+  P.blend = ctx->pipe.blend_shader;
+  P.shader = ctx->pipe.fragment_shader;
+  P.constColor = ctx->pipe.const_color;
+  P.deferredClear = ctx->deferredClear ? 1 : 0;
+  P.clearColor = ctx->clearColor;
+  P.clearDepth = ctx->clearDepth;
+  P.hiz = (ctx->dbg.disable_hiz || ctx->dbg.count_fragments) ? 0 : 1;
+  P.maxSubtris = (int32_t)ctx->maxSubtris;
+  P.triSubtris = ctx->triSubtris.as<uint8_t>();
+  P.triHeader = ctx->triHeader.as<Header>();
+  P.triData = ctx->triData.as<TriData>();
+  P.subtriTri = ctx->subtriTri.as<uint32_t>();
+  P.binChunk = kBinChunkTris;
+  P.numChunks = numChunks;
+  P.binChunkCount = ctx->binChunkCount.as<uint32_t>();
+  P.binTotal = ctx->binTotal.as<uint32_t>();
+  P.maxBinEntries = (int32_t)ctx->maxBinEntries;
+  P.binList = ctx->binList.as<uint32_t>();
+  P.tileStart = ctx->tileStart.as<uint32_t>();
+  P.tileCount = ctx->tileCount.as<uint32_t>();
+  P.maxTileEntries = (int32_t)ctx->maxTileEntries;
+  P.tileList = ctx->tileList.as<uint32_t>();
+  P.color = ctx->color->dev;
+  P.depth = ctx->depth->dev;
+  P.triid = ctx->triid ? ctx->triid->dev : nullptr;
+  P.counters = ctx->counters.as<Counters>();
+  return P;
+}
+
+cr_status launchAll(cr_context* ctx, const Params& P, hipStream_t stream) {
+  CR_HIP(ctx, hipEventRecord(ctx->ev[0], stream));
+  launchReset(P, stream);
+  launchSetup(P, stream);
+  CR_HIP(ctx, hipEventRecord(ctx->ev[1], stream));
+  launchBin(P, ctx->binStart.as<uint32_t>(), stream);
+  CR_HIP(ctx, hipEventRecord(ctx->ev[2], stream));
+  launchCoarse(P, ctx->binStart.as<uint32_t>(), stream);
+  CR_HIP(ctx, hipEventRecord(ctx->ev[3], stream));
+  launchFine(P, stream);
+  CR_HIP(ctx, hipEventRecord(ctx->ev[4], stream));
+  CR_HIP(ctx, hipGetLastError());
+  CR_HIP(ctx, hipMemcpyAsync(ctx->hostCounters, ctx->counters.ptr, sizeof(Counters), hipMemcpyDeviceToHost, stream));
+  ctx->evRecorded = true;
+  return CR_OK;
+}
+
+// Slack added when growing, as the reference (CudaRaster.cpp:239-241).
+constexpr int64_t kSubtrisSlack = 4096;
+constexpr int64_t kBinSlack = 256 * 512;
+constexpr int64_t kTileSlack = 4096 * 32;
+constexpr int64_t kMaxEntries = (int64_t)INT32_MAX - 1;
+constexpr int32_t kMaxTris = 1 << 28;
+
+void initialCapacities(cr_context* ctx, const Geometry& g) {
+  int64_t nt = ctx->numTris;
+  ctx->maxSubtris = std::max(ctx->maxSubtris, nt + kSubtrisSlack);
+  ctx->maxBinEntries = std::max(ctx->maxBinEntries, nt + (int64_t)g.nb * 64 + kBinSlack);
+  ctx->maxTileEntries = std::max(ctx->maxTileEntries, 2 * nt + (int64_t)g.nt + kTileSlack);
+}
+
+cr_status finishDraw(cr_context* ctx, bool* overflow) {
+  const Counters& c = *ctx->hostCounters;
+  ctx->last.num_tris = ctx->numTris;
+  ctx->last.num_subtris = c.numSubtris;
+  ctx->last.bin_entries = c.binEntries;
+  ctx->last.tile_entries = c.tileEntries;
+  ctx->last.fragments = (int64_t)c.fragments;
+  *overflow = c.overflow != 0;
+  if (!*overflow) return CR_OK;
+  if (c.overflow & 1u) ctx->maxSubtris = std::max<int64_t>(ctx->maxSubtris, (int64_t)c.numSubtris + kSubtrisSlack);
+  if (c.overflow & 2u) ctx->maxBinEntries = std::max<int64_t>(ctx->maxBinEntries, (int64_t)c.binEntries + kBinSlack);
+  if (c.overflow & 4u) ctx->maxTileEntries = std::max<int64_t>(ctx->maxTileEntries, (int64_t)c.tileEntries + kTileSlack);
+  if (ctx->maxSubtris > kMaxEntries || ctx->maxBinEntries > kMaxEntries || ctx->maxTileEntries > kMaxEntries)
+    return fail(ctx, CR_ERR_OVERFLOW, "CudaRaster: internal queue capacity exceeded (2^31 entries)");
+  return CR_OK;
+}
+
+cr_status drawImpl(cr_context* ctx, hipStream_t stream, bool sync) {
+  cr_status st = validateDraw(ctx);
+  if (st != CR_OK) return st;
+  CR_HIP(ctx, hipSetDevice(ctx->device));
+  if (ctx->pending) {
+    cr_status s2 = cr_draw_check(ctx);
+    if (s2 != CR_OK && s2 != CR_ERR_OVERFLOW) return s2;
+  }
+  Geometry g = geometryOf(ctx->color);
+  int32_t numChunks = (ctx->numTris + kBinChunkTris - 1) / kBinChunkTris;
+  initialCapacities(ctx, g);
+  ctx->last.retries = 0;
+  for (;;) {
+    st = ensureBuffers(ctx, g, numChunks);
+    if (st != CR_OK) return st;
+    Params P = makeParams(ctx, g, numChunks);
+    st = launchAll(ctx, P, stream);
+    if (st != CR_OK) return st;
+    if (!sync) {
+      ctx->pending = true;
+      ctx->pendingStream = stream;
+      ctx->deferredClear = false;
+      return CR_OK;
+    }
+    CR_HIP(ctx, hipStreamSynchronize(stream));
+    bool overflow = false;
+    st = finishDraw(ctx, &overflow);
+    if (st != CR_OK) return st;
+    if (!overflow) break;
+    ctx->last.retries++;
+  }
+  ctx->deferredClear = false;
+  return CR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t cr_abi_version(void) { return (1 << 16) | 0; }
+
+const char* cr_last_error(const cr_context* ctx) { return ctx ? ctx->err.c_str() : g_lastError.c_str(); }
+
+cr_status cr_context_create(cr_context** out) {
+  if (!out) return fail(nullptr, CR_ERR_INVALID, "cr_context_create: out is NULL");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(nullptr, CR_ERR_DEVICE, "CudaRaster: No HIP-capable devices found!");
+  cr_context* ctx = new (std::nothrow) cr_context();
+  if (!ctx) return fail(nullptr, CR_ERR_NOMEM, "out of host memory");
+  hipError_t e = hipGetDevice(&ctx->device);
+  if (e == hipSuccess) e = ctx->counters.ensure(sizeof(Counters));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hostCounters, sizeof(Counters), hipHostMallocDefault);
+  for (int i = 0; i < 5 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
+  if (e != hipSuccess) {
+    cr_context_destroy(ctx);
+    return fail(nullptr, CR_ERR_DEVICE, "cr_context_create: %s", hipGetErrorString(e));
+  }
+  memset(ctx->hostCounters, 0, sizeof(Counters));
+  *out = ctx;
+  return CR_OK;
+}
+
+cr_status cr_context_destroy(cr_context* ctx) {
+  if (!ctx) return CR_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->pending) (void)hipStreamSynchronize(ctx->pendingStream);
+  for (DevBuf* b : {&ctx->triSubtris, &ctx->triHeader, &ctx->triData, &ctx->subtriTri, &ctx->binChunkCount,
+                    &ctx->binTotal, &ctx->binStart, &ctx->binList, &ctx->tileStart, &ctx->tileCount,
+                    &ctx->tileList, &ctx->counters})
+    b->release();
+  if (ctx->hostCounters) (void)hipHostFree(ctx->hostCounters);
+  for (int i = 0; i < 5; i++)
+    if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+  delete ctx;
+  return CR_OK;
+}
+
+cr_status cr_surface_create(int32_t width, int32_t height, int32_t format, int32_t numSamples, cr_surface** out) {
+  if (!out) return fail(nullptr, CR_ERR_INVALID, "cr_surface_create: out is NULL");
+  *out = nullptr;
+  // CudaSurface.cpp:44-62
+  if (std::min(width, height) <= 0) return fail(nullptr, CR_ERR_INVALID, "CudaSurface: Size must be positive!");
+  if (std::max(width, height) > kMaxViewport)
+    return fail(nullptr, CR_ERR_INVALID, "CudaSurface: CR_MAXVIEWPORT_SIZE exceeded!");
+  if (format < 0 || format > CR_FORMAT_TRIANGLE_ID) return fail(nullptr, CR_ERR_INVALID, "CudaSurface: Invalid format!");
+  if (numSamples > 8) return fail(nullptr, CR_ERR_INVALID, "CudaSurface: numSamples cannot exceed 8!");
+  if (numSamples < 1 || popc8(numSamples) != 1)
+    return fail(nullptr, CR_ERR_INVALID, "CudaSurface: numSamples must be a power of two!");
+  cr_surface* s = new (std::nothrow) cr_surface();
+  if (!s) return fail(nullptr, CR_ERR_NOMEM, "out of host memory");
+  s->width = width;
+  s->height = height;
+  s->format = format;
+  s->samples = numSamples;
+  s->samplesLog2 = popc8(numSamples - 1);
+  s->roundedW = (width + 7) & ~7;
+  s->roundedH = (height + 7) & ~7;
+  s->textureW = s->roundedW * numSamples;
+  s->bytes = (size_t)s->textureW * s->roundedH * 4;
+  hipError_t e = hipGetDevice(&s->device);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->dev, s->bytes);
+  if (e == hipSuccess) e = hipMemset(s->dev, 0, s->bytes);
+  if (e != hipSuccess) {
+    if (s->dev) (void)hipFree(s->dev);
+    delete s;
+    return fail(nullptr, CR_ERR_DEVICE, "cr_surface_create: %s", hipGetErrorString(e));
+  }
+  *out = s;
+  return CR_OK;
+}
+
+cr_status cr_surface_destroy(cr_surface* s) {
+  if (!s) return CR_OK;
+  if (s->dev) {
+    (void)hipSetDevice(s->device);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(s->dev);
+  }
+  delete s;
+  return CR_OK;
+}
+
+cr_status cr_surface_get_info(const cr_surface* s, int32_t* width, int32_t* height, int32_t* rw, int32_t* rh,
+                              int32_t* tw, int32_t* ns, int32_t* fmt) {
+  if (!s) return fail(nullptr, CR_ERR_INVALID, "cr_surface_get_info: surface is NULL");
+  if (width) *width = s->width;
+  if (height) *height = s->height;
+  if (rw) *rw = s->roundedW;
+  if (rh) *rh = s->roundedH;
+  if (tw) *tw = s->textureW;
+  if (ns) *ns = s->samples;
+  if (fmt) *fmt = s->format;
+  return CR_OK;
+}
+
+cr_status cr_surface_get_device_ptr(const cr_surface* s, void** p, size_t* bytes) {
+  if (!s) return fail(nullptr, CR_ERR_INVALID, "cr_surface_get_device_ptr: surface is NULL");
+  if (p) *p = s->dev;
+  if (bytes) *bytes = s->bytes;
+  return CR_OK;
+}
+
+cr_status cr_surface_download(const cr_surface* s, void* host, size_t bytes) {
+  if (!s || !host) return fail(nullptr, CR_ERR_INVALID, "cr_surface_download: NULL argument");
+  if (bytes != s->bytes) return fail(nullptr, CR_ERR_INVALID, "cr_surface_download: %zu bytes, surface holds %zu", bytes, s->bytes);
+  if (hipSetDevice(s->device) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(host, s->dev, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(nullptr, CR_ERR_DEVICE, "cr_surface_download: copy failed");
+  return CR_OK;
+}
+
+cr_status cr_surface_upload(cr_surface* s, const void* host, size_t bytes) {
+  if (!s || !host) return fail(nullptr, CR_ERR_INVALID, "cr_surface_upload: NULL argument");
+  if (bytes != s->bytes) return fail(nullptr, CR_ERR_INVALID, "cr_surface_upload: %zu bytes, surface holds %zu", bytes, s->bytes);
+  if (hipSetDevice(s->device) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(s->dev, host, bytes, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(nullptr, CR_ERR_DEVICE, "cr_surface_upload: copy failed");
+  return CR_OK;
+}
+
+cr_status cr_set_surfaces(cr_context* ctx, cr_surface* color, cr_surface* depth) {
+  if (!ctx) return fail(nullptr, CR_ERR_INVALID, "cr_set_surfaces: context is NULL");
+  ctx->color = nullptr;
+  ctx->depth = nullptr;
+  if (!color && !depth) return CR_OK;
+  // CudaRaster.cpp:142-159
+  if (!color) return fail(ctx, CR_ERR_INVALID, "CudaRaster: No color buffer specified!");
+  if (!depth) return fail(ctx, CR_ERR_INVALID, "CudaRaster: No depth buffer specified!");
+  if (color->format != CR_FORMAT_RGBA8) return fail(ctx, CR_ERR_INVALID, "CudaRaster: Unsupported color buffer format!");
+  if (depth->format != CR_FORMAT_DEPTH32) return fail(ctx, CR_ERR_INVALID, "CudaRaster: Unsupported depth buffer format!");
+  if (color->width != depth->width || color->height != depth->height)
+    return fail(ctx, CR_ERR_INVALID, "CudaRaster: Mismatch in size between surfaces!");
+  if (color->samples != depth->samples)
+    return fail(ctx, CR_ERR_INVALID, "CudaRaster: Mismatch in multisampling between surfaces!");
+  ctx->color = color;
+  ctx->depth = depth;
+  return CR_OK;
+}
+
+cr_status cr_set_triangle_id_surface(cr_context* ctx, cr_surface* tri) {
+  if (!ctx) return fail(nullptr, CR_ERR_INVALID, "cr_set_triangle_id_surface: context is NULL");
+  if (tri && tri->format != CR_FORMAT_TRIANGLE_ID)
+    return fail(ctx, CR_ERR_INVALID, "triangle-id surface must have CR_FORMAT_TRIANGLE_ID");
+  ctx->triid = tri;
+  return CR_OK;
+}
+
+cr_status cr_deferred_clear(cr_context* ctx, const float rgba[4], float depth) {
+  if (!ctx || !rgba) return fail(ctx, CR_ERR_INVALID, "cr_deferred_clear: NULL argument");
+  // CudaRaster.cpp:174-179
+  ctx->deferredClear = true;
+  ctx->clearColor = hostChannel(rgba[0]) | (hostChannel(rgba[1]) << 8) | (hostChannel(rgba[2]) << 16) |
+                    (hostChannel(rgba[3]) << 24);
+  double d = (double)depth * 4294967296.0;
+  uint64_t q;
+  if (!(d > 0.0)) q = 0;
+  else if (d >= 18446744073709551616.0) q = UINT64_MAX;
+  else q = (uint64_t)d;
+  ctx->clearDepth = encodeDepth((uint32_t)std::min<uint64_t>(q, 0xFFFFFFFFull));
+  return CR_OK;
+}
+
+cr_status cr_set_pixel_pipe(cr_context* ctx, const cr_pipe_desc* p) {
+  if (!ctx) return fail(nullptr, CR_ERR_INVALID, "cr_set_pixel_pipe: context is NULL");
+  if (!p) {
+    ctx->pipeSet = false;
+    return CR_OK;
+  }
+  if (p->samples_log2 < 0 || p->samples_log2 > 3) return fail(ctx, CR_ERR_INVALID, "pipe: samples_log2 must be 0..3");
+  if (p->vertex_struct_size < 16 || (p->vertex_struct_size & 15))
+    return fail(ctx, CR_ERR_INVALID, "pipe: vertex_struct_size must be a positive multiple of 16 (Vec4f records)");
+  if (p->fragment_shader != CR_SHADER_GOURAUD && p->fragment_shader != CR_SHADER_CONSTANT)
+    return fail(ctx, CR_ERR_INVALID, "pipe: unknown fragment shader %d", p->fragment_shader);
+  if (p->fragment_shader == CR_SHADER_GOURAUD && p->vertex_struct_size < 32)
+    return fail(ctx, CR_ERR_INVALID, "pipe: GouraudShader needs a colour varying (vertex_struct_size >= 32)");
+  if (p->blend_shader < CR_BLEND_REPLACE || p->blend_shader > CR_BLEND_DEPTH_ONLY)
+    return fail(ctx, CR_ERR_INVALID, "pipe: unknown blend shader %d", p->blend_shader);
+  if (p->render_mode_flags & CR_RENDER_QUADS)
+    return fail(ctx, CR_ERR_UNSUPPORTED, "pipe: RenderModeFlag_EnableQuads is not supported");
+  if (p->render_mode_flags & ~(uint32_t)(CR_RENDER_DEPTH | CR_RENDER_LERP | CR_RENDER_QUADS))
+    return fail(ctx, CR_ERR_INVALID, "pipe: unknown render mode flags 0x%x", p->render_mode_flags);
+  ctx->pipe = *p;
+  ctx->pipeSet = true;
+  return CR_OK;
+}
+
+cr_status cr_pipe_lookup(const char* name, int32_t blend, int32_t samplesLog2, uint32_t flags, cr_pipe_desc* out) {
+  if (!name || !out) return fail(nullptr, CR_ERR_INVALID, "cr_pipe_lookup: NULL argument");
+  cr_pipe_desc d{};
+  d.blend_shader = blend;
+  d.samples_log2 = samplesLog2;
+  d.render_mode_flags = flags;
+  if (strcmp(name, "PixelPipe_passthrough") == 0) {
+    d.vertex_struct_size = 16;
+    d.fragment_shader = CR_SHADER_CONSTANT;
+    d.const_color = 0xFF0000FFu;   // toABGR(Vec4f(1,0,0,1))
+  } else if (strcmp(name, "PixelPipe_gouraud") == 0) {
+    d.vertex_struct_size = 32;
+    d.fragment_shader = CR_SHADER_GOURAUD;
+  } else {
+    return fail(nullptr, CR_ERR_INVALID, "CudaRaster: Invalid pixel pipe!");
+  }
+  *out = d;
+  return CR_OK;
+}
+
+cr_status cr_set_vertex_buffer(cr_context* ctx, const void* p, int64_t ofs, int64_t numVertices) {
+  if (!ctx) return fail(nullptr, CR_ERR_INVALID, "cr_set_vertex_buffer: context is NULL");
+  if (ofs < 0 || (ofs & 15) || numVertices < 0)
+    return fail(ctx, CR_ERR_INVALID, "cr_set_vertex_buffer: offset must be a non-negative multiple of 16");
+  ctx->vbuf = static_cast<const uint8_t*>(p);
+  ctx->vofs = ofs;
+  ctx->numVerts = numVertices;
+  return CR_OK;
+}
+
+cr_status cr_set_index_buffer(cr_context* ctx, const void* p, int64_t ofs, int32_t numTris) {
+  if (!ctx) return fail(nullptr, CR_ERR_INVALID, "cr_set_index_buffer: context is NULL");
+  if (ofs < 0 || (ofs & 3) || numTris < 0)
+    return fail(ctx, CR_ERR_INVALID, "cr_set_index_buffer: bad offset or triangle count");
+  if (numTris > kMaxTris) return fail(ctx, CR_ERR_INVALID, "cr_set_index_buffer: more than 2^28 triangles");
+  ctx->ibuf = static_cast<const uint8_t*>(p);
+  ctx->iofs = ofs;
+  ctx->numTris = numTris;
+  return CR_OK;
+}
+
+cr_status cr_draw_triangles(cr_context* ctx, void* stream) {
+  if (!ctx) return fail(nullptr, CR_ERR_INVALID, "cr_draw_triangles: context is NULL");
+  return drawImpl(ctx, static_cast<hipStream_t>(stream), true);
+}
+
+cr_status cr_draw_triangles_async(cr_context* ctx, void* stream) {
+  if (!ctx) return fail(nullptr, CR_ERR_INVALID, "cr_draw_triangles_async: context is NULL");
+  return drawImpl(ctx, static_cast<hipStream_t>(stream), false);
+}
+
+cr_status cr_draw_check(cr_context* ctx) {
+  if (!ctx) return fail(nullptr, CR_ERR_INVALID, "cr_draw_check: context is NULL");
+  if (!ctx->pending) return CR_OK;
+  CR_HIP(ctx, hipStreamSynchronize(ctx->pendingStream));
+  ctx->pending = false;
+  bool overflow = false;
+  cr_status st = finishDraw(ctx, &overflow);
+  if (st != CR_OK) return st;
+  if (overflow) return fail(ctx, CR_ERR_OVERFLOW, "draw exceeded the reserved capacities; capacities grown, re-run it");
+  return CR_OK;
+}
+
+cr_status cr_reserve(cr_context* ctx, int64_t maxSubtris, int64_t maxBin, int64_t maxTile) {
+  if (!ctx) return fail(nullptr, CR_ERR_INVALID, "cr_reserve: context is NULL");
+  if (maxSubtris > kMaxEntries || maxBin > kMaxEntries || maxTile > kMaxEntries)
+    return fail(ctx, CR_ERR_INVALID, "cr_reserve: capacity above 2^31 entries");
+  ctx->maxSubtris = std::max(ctx->maxSubtris, maxSubtris);
+  ctx->maxBinEntries = std::max(ctx->maxBinEntries, maxBin);
+  ctx->maxTileEntries = std::max(ctx->maxTileEntries, maxTile);
+  return CR_OK;
+}
+
+cr_status cr_get_stats(cr_context* ctx, cr_stats* out) {
+  if (!ctx || !out) return fail(ctx, CR_ERR_INVALID, "cr_get_stats: NULL argument");
+  memset(out, 0, sizeof(*out));
+  if (!ctx->evRecorded) return CR_OK;
+  CR_HIP(ctx, hipEventSynchronize(ctx->ev[4]));
+  float ms[4];
+  for (int i = 0; i < 4; i++) CR_HIP(ctx, hipEventElapsedTime(&ms[i], ctx->ev[i], ctx->ev[i + 1]));
+  out->setup_time = ms[0] * 1e-3f;
+  out->bin_time = ms[1] * 1e-3f;
+  out->coarse_time = ms[2] * 1e-3f;
+  out->fine_time = ms[3] * 1e-3f;
+  return CR_OK;
+}
+
+cr_status cr_get_counters(cr_context* ctx, cr_counters* out) {
+  if (!ctx || !out) return fail(ctx, CR_ERR_INVALID, "cr_get_counters: NULL argument");
+  *out = ctx->last;
+  return CR_OK;
+}
+
+cr_status cr_get_profiling_info(cr_context* ctx, char* buf, size_t len) {
+  if (!ctx || !buf || len == 0) return fail(ctx, CR_ERR_INVALID, "cr_get_profiling_info: NULL argument");
+  cr_stats s;
+  cr_status st = cr_get_stats(ctx, &s);
+  if (st != CR_OK) return st;
+  const cr_counters& c = ctx->last;
+  double total = (double)s.setup_time + s.bin_time + s.coarse_time + s.fine_time;
+  snprintf(buf, len,
+           "TriangleSetup:  %8.3f ms\nBinRaster:      %8.3f ms\nCoarseRaster:   %8.3f ms\nFineRaster:     %8.3f ms\n"
+           "Total:          %8.3f ms\n\nTriangles       %lld\nSubtriangles    %lld\nBin entries     %lld\n"
+           "Tile entries    %lld\nFragments       %lld%s\nRetries         %lld\n",
+           s.setup_time * 1e3, s.bin_time * 1e3, s.coarse_time * 1e3, s.fine_time * 1e3, total * 1e3,
+           (long long)c.num_tris, (long long)c.num_subtris, (long long)c.bin_entries, (long long)c.tile_entries,
+           (long long)c.fragments, (ctx->dbg.disable_hiz || ctx->dbg.count_fragments) ? "" : " (after early-Z)",
+           (long long)c.retries);
+  return CR_OK;
+}
+
+cr_status cr_set_debug_params(cr_context* ctx, const cr_debug_params* p) {
+  if (!ctx || !p) return fail(ctx, CR_ERR_INVALID, "cr_set_debug_params: NULL argument");
+  ctx->dbg = *p;
+  return CR_OK;
+}
+
+}  // extern "C"

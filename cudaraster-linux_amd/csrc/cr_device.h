@@ -1,0 +1,213 @@
+// Device-side data layouts and rasterization rules for the gfx950 pipeline.
+//
+// The rules are the reference's (cudaraster/cuda/TriangleSetup.inl,
+// Util.hpp, Util.inl, FineRaster.inl); the layouts of the per-triangle
+// records keep CRTriangleHeader / CRTriangleData (PrivateDefs.hpp:26-62) so
+// that setup output can be compared field by field.  Everything else --
+// queues, lists, counters -- is this library's own design.
+//
+// Float arithmetic: the library is compiled with -ffp-contract=off, IEEE
+// division and denormals preserved.  Fused multiply-adds appear only as
+// explicit fmaf() at the places where nvcc's default contraction fuses the
+// reference's `a*b + c` expressions (left product fused in `a*b + c*d`).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace crhip {
+
+// Constants.hpp:21-84
+constexpr int kSubpixelLog2 = 4;
+constexpr int kTileLog2 = 3;                  // 8x8 pixels per tile
+constexpr int kBinLog2 = 4;                   // 16x16 tiles per bin
+constexpr int kMaxViewport = 2048;
+constexpr int kMaxBinsLog2 = 4;               // <= 16x16 bins
+constexpr int kMaxBins = 1 << (2 * kMaxBinsLog2);
+constexpr int kBinTiles = 1 << (2 * kBinLog2);
+constexpr uint32_t kDepthMin = 2200u << 3;
+constexpr uint32_t kDepthMax = 0xFFFFFFFFu - (2200u << 3);
+constexpr int kBaryMax = (1 << (30 - kSubpixelLog2)) - 1;
+__host__ __device__ constexpr uint32_t lerpError(int s) { return 2200u << s; }
+
+enum : uint32_t { kFlagDepth = 1, kFlagLerp = 2 };
+enum : int32_t { kBlendReplace = 0, kBlendSrcOver = 1, kBlendAdditive = 2, kBlendDepthOnly = 3 };
+enum : int32_t { kShaderGouraud = 0, kShaderConstant = 1 };
+
+struct Header {          // CRTriangleHeader
+  int16_t v0x, v0y, v1x, v1y, v2x, v2y;
+  uint32_t misc;         // 1 subtri: zmin[31:12] | flips; >= 2: first slot
+};
+struct TriData {         // CRTriangleData
+  uint32_t zx, zy, zb, zslope;
+  int32_t wx, wy, wb;
+  int32_t ux, uy, ub;
+  int32_t vx, vy, vb;
+  uint32_t vi0, vi1, vi2;
+};
+static_assert(sizeof(Header) == 16, "header layout");
+static_assert(sizeof(TriData) == 64, "data layout");
+
+// Device counters; one 256-byte block per context.
+struct Counters {
+  uint32_t numSubtris;     // slot allocator for clipped subtriangles (starts at numTris)
+  uint32_t tileEntries;    // tile-list entry allocator
+  uint32_t overflow;       // bit 0 subtris, bit 1 bin entries, bit 2 tile entries
+  uint32_t binEntries;     // total bin-list entries (written by the scan)
+  uint32_t binScanDone;    // blocks of the bin scan that have finished
+  uint32_t pad0[3];
+  uint64_t fragments;      // (subtriangle, pixel) pairs with >= 1 covered sample
+  uint64_t pad[27];
+};
+static_assert(sizeof(Counters) == 256, "counter block");
+
+struct Params {
+  // Input.
+  int32_t numTris;
+  int32_t numVertices;
+  int32_t vertexStride;            // bytes
+  const uint8_t* vertices;         // already offset
+  const int32_t* indices;          // int3 per triangle, already offset
+  // Geometry of the surfaces.
+  int32_t viewportWidth, viewportHeight;
+  int32_t widthPixels, heightPixels;   // rounded to tiles
+  int32_t widthTiles, heightTiles, numTiles;
+  int32_t widthBins, heightBins, numBins;
+  // Pipe.
+  int32_t samplesLog2;
+  uint32_t flags;
+  int32_t blend;
+  int32_t shader;
+  uint32_t constColor;
+  int32_t deferredClear;
+  uint32_t clearColor, clearDepth;
+  int32_t hiz;                     // 1: tile-level early depth rejection
+  // Setup output.
+  int32_t maxSubtris;
+  uint8_t* triSubtris;
+  Header* triHeader;
+  TriData* triData;
+  uint32_t* subtriTri;             // source triangle of slots >= numTris
+  // Bin lists (bin-major, each bin contiguous and in submission order).
+  int32_t binChunk;                // triangles per bin chunk
+  int32_t numChunks;
+  uint32_t* binChunkCount;         // [numBins][numChunks]: counts, then offsets
+  uint32_t* binTotal;              // [numBins]
+  int32_t maxBinEntries;
+  uint32_t* binList;
+  // Tile lists (each tile contiguous and in submission order).
+  uint32_t* tileStart;             // [numTiles]
+  uint32_t* tileCount;             // [numTiles]
+  int32_t maxTileEntries;
+  uint32_t* tileList;
+  // Surfaces: (widthPixels * N) x heightPixels texels, row 0 at the bottom.
+  uint32_t* color;
+  uint32_t* depth;
+  uint32_t* triid;                 // optional
+  Counters* counters;
+};
+
+// ---------------------------------------------------------------------------
+// Exact conversions (PTX cvt semantics restated).
+// ---------------------------------------------------------------------------
+
+// cvt.rni.sat.s32.f32: nearest-even, saturate, NaN -> 0.
+__device__ __forceinline__ int32_t cvtRniSatS32(float a) {
+  if (a != a) return 0;
+  float r = __builtin_rintf(a);
+  if (r >= 2147483648.0f) return INT32_MAX;
+  if (r < -2147483648.0f) return INT32_MIN;
+  return (int32_t)r;
+}
+
+// cvt.rni.sat.u32.f32
+__device__ __forceinline__ uint32_t cvtRniSatU32(float a) {
+  if (a != a) return 0u;
+  float r = __builtin_rintf(a);
+  if (r >= 4294967296.0f) return 0xFFFFFFFFu;
+  if (r <= 0.0f) return 0u;
+  return (uint32_t)r;
+}
+
+// (U32)float in the reference's device code: cvt.rzi.u32.f32 (saturating).
+__device__ __forceinline__ uint32_t cvtRziSatU32(float a) {
+  if (a != a || a <= 0.0f) return 0u;
+  if (a >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)a;
+}
+
+__device__ __forceinline__ int32_t asr32(int32_t a, int s) { return a >> s; }   // arithmetic on AMDGPU
+__device__ __forceinline__ int64_t asr64(int64_t a, int s) { return a >> s; }
+__device__ __forceinline__ int32_t wsub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+__device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+__device__ __forceinline__ int32_t wmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+__device__ __forceinline__ int64_t wmul64(int64_t a, int64_t b) { return (int64_t)((uint64_t)a * (uint64_t)b); }
+__device__ __forceinline__ uint32_t wabs32(int32_t a) { return a < 0 ? 0u - (uint32_t)a : (uint32_t)a; }
+
+// toABGR (Util.inl:30-38): per channel floor(255*c + 1/2) of the exact
+// product, saturated to [0,255], NaN -> 0.  Evaluated exactly in F64.
+__device__ __forceinline__ uint32_t abgrChannel(float c) {
+  double x = fma((double)c, 255.0, 0.5);      // exact: 24x8-bit product + 1/2
+  if (!(x >= 0.0)) return 0u;                 // negative or NaN
+  if (x >= 255.0) return 255u;
+  return (uint32_t)x;                         // truncation == floor for x >= 0
+}
+__device__ __forceinline__ uint32_t toABGR(float r, float g, float b, float a) {
+  return abgrChannel(r) | (abgrChannel(g) << 8) | (abgrChannel(b) << 16) | (abgrChannel(a) << 24);
+}
+
+// blendABGR / blendABGRClamp with the stock factors (Util.inl:42-60,
+// PixelPipe.inl:73-85).
+__device__ __forceinline__ uint32_t blendPixel(int32_t mode, uint32_t src, uint32_t dst) {
+  if (mode == kBlendReplace) return src;
+  if (mode == kBlendDepthOnly) return dst;
+  uint32_t out = 0;
+  uint32_t a = src >> 24;
+#pragma unroll
+  for (int ch = 0; ch < 4; ch++) {
+    uint32_t s = (src >> (8 * ch)) & 0xFFu;
+    uint32_t d = (dst >> (8 * ch)) & 0xFFu;
+    uint32_t x;
+    if (mode == kBlendSrcOver) {
+      x = s * a + d * (255u - a);
+    } else {
+      x = min(s * 255u + d * 255u, 255u * 255u);
+    }
+    out |= ((x * 0x010101u + 0x800000u) >> 24) << (8 * ch);
+  }
+  return out;
+}
+
+// c_msaaPatterns (Util.hpp:27-36), packed 4 bits per sample.
+__host__ __device__ constexpr uint32_t msaaPatternPacked(int s) {
+  return s == 0 ? 0x0u : s == 1 ? 0x10u : s == 2 ? 0x2031u : 0x51360427u;
+}
+__host__ __device__ constexpr int msaaPattern(int s, int i) {
+  return (int)((msaaPatternPacked(s) >> (4 * i)) & 0xFu);
+}
+
+// selectMSAACentroid (Util.hpp:182-203)
+__host__ __device__ inline int selectCentroid(int s, uint32_t mask) {
+  int n = 1 << s;
+  if (mask == 0 || mask == (1u << n) - 1u) return -1;
+  int best = -1, bestDist = 0x7FFFFFFF;
+  for (int i = 0; i < n; i++) {
+    if (mask & (1u << i)) {
+      int dx = msaaPattern(s, i) * 2 + 1 - n;
+      int dy = i * 2 + 1 - n;
+      int dist = dx * dx + dy * dy;
+      if (dist < bestDist) { best = i; bestDist = dist; }
+    }
+  }
+  return best;
+}
+
+// Host-side launchers, one per stage file.
+void launchSetup(const Params& P, hipStream_t stream);
+void launchReset(const Params& P, hipStream_t stream);
+void launchBin(const Params& P, uint32_t* binStart, hipStream_t stream);
+void launchCoarse(const Params& P, const uint32_t* binStart, hipStream_t stream);
+void launchFine(const Params& P, hipStream_t stream);
+constexpr int kBinChunkTris = 2048;
+
+}  // namespace crhip

@@ -1,0 +1,378 @@
+"""Python front end of libcr.so (the C ABI in include/cr/cr.h).
+
+Mirrors the reference's host classes FW::CudaRaster and FW::CudaSurface
+(src/cudaraster/CudaRaster.hpp:143-168, CudaSurface.hpp:60-71) with the same
+method names, so tests read like uses of the reference API.  Errors the
+reference turns into fail() raise CrError here.
+
+Device memory for vertex and index buffers comes from torch (plumbing
+only); torch is imported before libcr.so is loaded so that one HIP runtime
+serves both.  Everything the rasterizer computes runs in libcr.so's HIP
+kernels; there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import scenes  # noqa: F401  (re-exported workloads)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libcr.so")
+
+CR_OK, CR_ERR_INVALID, CR_ERR_STATE, CR_ERR_DEVICE, CR_ERR_OVERFLOW, CR_ERR_UNSUPPORTED, CR_ERR_NOMEM = range(7)
+FORMAT_RGBA8, FORMAT_DEPTH32, FORMAT_TRIANGLE_ID = 0, 1, 2
+RENDER_DEPTH, RENDER_LERP, RENDER_QUADS = 1, 2, 4
+SHADER_GOURAUD, SHADER_CONSTANT = 0, 1
+BLEND_REPLACE, BLEND_SRC_OVER, BLEND_ADDITIVE, BLEND_DEPTH_ONLY = 0, 1, 2, 3
+
+# The exported C symbols (include/cr/cr.h).
+EXPORTS = [
+    "cr_context_create", "cr_context_destroy", "cr_last_error",
+    "cr_surface_create", "cr_surface_destroy", "cr_surface_get_info", "cr_surface_get_device_ptr",
+    "cr_surface_download", "cr_surface_upload",
+    "cr_set_surfaces", "cr_set_triangle_id_surface", "cr_deferred_clear", "cr_set_pixel_pipe",
+    "cr_pipe_lookup", "cr_set_vertex_buffer", "cr_set_index_buffer",
+    "cr_draw_triangles", "cr_draw_triangles_async", "cr_draw_check", "cr_reserve",
+    "cr_get_stats", "cr_get_counters", "cr_get_profiling_info", "cr_set_debug_params", "cr_abi_version",
+]
+
+
+class CrError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+class PipeDesc(ctypes.Structure):
+    _fields_ = [("vertex_struct_size", ctypes.c_int32), ("fragment_shader", ctypes.c_int32),
+                ("blend_shader", ctypes.c_int32), ("samples_log2", ctypes.c_int32),
+                ("render_mode_flags", ctypes.c_uint32), ("const_color", ctypes.c_uint32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("setupTime", ctypes.c_float), ("binTime", ctypes.c_float),
+                ("coarseTime", ctypes.c_float), ("fineTime", ctypes.c_float)]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("num_tris", "num_subtris", "bin_entries", "tile_entries",
+                                              "fragments", "retries")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class DebugParams(ctypes.Structure):
+    _fields_ = [("disable_hiz", ctypes.c_int32), ("count_fragments", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib_path() -> str:
+    return LIB_PATH
+
+
+def load(path: str | None = None):
+    """Load libcr.so.  Raises if the library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"libcr.so not built ({p}); run __graft_entry__.build() or "
+                           "python cudaraster-linux_amd/build.py")
+    try:  # one HIP runtime per process: let torch's runtime be the one libcr binds to
+        import torch  # noqa: F401
+    except Exception:  # pragma: no cover - torch is part of the environment
+        pass
+    L = ctypes.CDLL(p)
+    vp, i32, u32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64
+    sig = {
+        "cr_context_create": [ctypes.POINTER(vp)],
+        "cr_context_destroy": [vp],
+        "cr_surface_create": [i32, i32, i32, i32, ctypes.POINTER(vp)],
+        "cr_surface_destroy": [vp],
+        "cr_surface_get_info": [vp] + [ctypes.POINTER(i32)] * 7,
+        "cr_surface_get_device_ptr": [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t)],
+        "cr_surface_download": [vp, vp, ctypes.c_size_t],
+        "cr_surface_upload": [vp, vp, ctypes.c_size_t],
+        "cr_set_surfaces": [vp, vp, vp],
+        "cr_set_triangle_id_surface": [vp, vp],
+        "cr_deferred_clear": [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_float],
+        "cr_set_pixel_pipe": [vp, ctypes.POINTER(PipeDesc)],
+        "cr_pipe_lookup": [ctypes.c_char_p, i32, i32, u32, ctypes.POINTER(PipeDesc)],
+        "cr_set_vertex_buffer": [vp, vp, i64, i64],
+        "cr_set_index_buffer": [vp, vp, i64, i32],
+        "cr_draw_triangles": [vp, vp],
+        "cr_draw_triangles_async": [vp, vp],
+        "cr_draw_check": [vp],
+        "cr_reserve": [vp, i64, i64, i64],
+        "cr_get_stats": [vp, ctypes.POINTER(Stats)],
+        "cr_get_counters": [vp, ctypes.POINTER(Counters)],
+        "cr_get_profiling_info": [vp, ctypes.c_char_p, ctypes.c_size_t],
+        "cr_set_debug_params": [vp, ctypes.POINTER(DebugParams)],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.cr_last_error.argtypes = [vp]
+    L.cr_last_error.restype = ctypes.c_char_p
+    L.cr_abi_version.argtypes = []
+    L.cr_abi_version.restype = ctypes.c_int32
+    _lib = L
+    return L
+
+
+def _check(status, ctx=None):
+    if status != CR_OK:
+        msg = load().cr_last_error(ctx)
+        raise CrError(status, msg.decode() if msg else "")
+
+
+def pipe_lookup(name, blend=BLEND_REPLACE, samples_log2=0, flags=RENDER_DEPTH | RENDER_LERP) -> PipeDesc:
+    d = PipeDesc()
+    _check(load().cr_pipe_lookup(name.encode(), blend, samples_log2, flags, ctypes.byref(d)))
+    return d
+
+
+def pipe_desc(vertex_struct_size=32, fragment_shader=SHADER_GOURAUD, blend_shader=BLEND_REPLACE,
+              samples_log2=0, render_mode_flags=RENDER_DEPTH | RENDER_LERP, const_color=0xFF0000FF) -> PipeDesc:
+    return PipeDesc(vertex_struct_size, fragment_shader, blend_shader, samples_log2, render_mode_flags,
+                    const_color & 0xFFFFFFFF)
+
+
+class CudaSurface:
+    """FW::CudaSurface (CudaSurface.hpp:36-78): RGBA8 / DEPTH32 / triangle-id
+    render target with the tile-replicated MSAA layout."""
+
+    def __init__(self, size, fmt=FORMAT_RGBA8, num_samples=1):
+        L = load()
+        h = ctypes.c_void_p()
+        _check(L.cr_surface_create(int(size[0]), int(size[1]), int(fmt), int(num_samples), ctypes.byref(h)))
+        self._h = h
+        vals = [ctypes.c_int32() for _ in range(7)]
+        _check(L.cr_surface_get_info(h, *(ctypes.byref(v) for v in vals)))
+        (self.width, self.height, self.rounded_width, self.rounded_height, self.texture_width,
+         self.num_samples, self.format) = (v.value for v in vals)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def getSize(self):
+        return (self.width, self.height)
+
+    def getRoundedSize(self):
+        return (self.rounded_width, self.rounded_height)
+
+    def getTextureSize(self):
+        return (self.texture_width, self.rounded_height)
+
+    def getNumSamples(self):
+        return self.num_samples
+
+    def getSamplesLog2(self):
+        return int(self.num_samples - 1).bit_count()
+
+    @property
+    def shape(self):
+        return (self.rounded_height, self.texture_width)
+
+    def device_ptr(self):
+        p = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        _check(load().cr_surface_get_device_ptr(self._h, ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.uint32)
+        _check(load().cr_surface_download(self._h, out.ctypes.data, out.nbytes))
+        return out
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.uint32)
+        if a.shape != self.shape:
+            raise ValueError(f"surface expects {self.shape}, got {a.shape}")
+        _check(load().cr_surface_upload(self._h, a.ctypes.data, a.nbytes))
+
+    def close(self):
+        if self._h:
+            load().cr_surface_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ptr(buf):
+    if buf is None:
+        return None
+    if isinstance(buf, int):
+        return buf
+    if hasattr(buf, "data_ptr"):
+        if not buf.is_cuda:
+            raise ValueError("vertex/index buffers must be device tensors")
+        return buf.data_ptr()
+    raise TypeError("expected a device tensor or a raw device pointer")
+
+
+class CudaRaster:
+    """FW::CudaRaster (CudaRaster.hpp:42-190) over the C ABI."""
+
+    def __init__(self):
+        self._ctx = None
+        self._keep = {}
+
+    def init(self):
+        h = ctypes.c_void_p()
+        _check(load().cr_context_create(ctypes.byref(h)))
+        self._ctx = h
+        return self
+
+    @property
+    def ctx(self):
+        if self._ctx is None:
+            self.init()
+        return self._ctx
+
+    def setSurfaces(self, color, depth):
+        self._keep["color"], self._keep["depth"] = color, depth
+        _check(load().cr_set_surfaces(self.ctx, color.handle if color else None,
+                                      depth.handle if depth else None), self.ctx)
+
+    def setTriangleIdSurface(self, surf):
+        self._keep["triid"] = surf
+        _check(load().cr_set_triangle_id_surface(self.ctx, surf.handle if surf else None), self.ctx)
+
+    def deferredClear(self, color=(0.0, 0.0, 0.0, 0.0), depth=1.0):
+        c = (ctypes.c_float * 4)(*[float(x) for x in color])
+        _check(load().cr_deferred_clear(self.ctx, c, float(depth)), self.ctx)
+
+    def setPixelPipe(self, pipe):
+        if isinstance(pipe, str):
+            pipe = pipe_lookup(pipe)
+        _check(load().cr_set_pixel_pipe(self.ctx, ctypes.byref(pipe) if pipe is not None else None), self.ctx)
+        self._stride = pipe.vertex_struct_size if pipe is not None else None
+
+    def setVertexBuffer(self, buf, ofs=0, num_vertices=None):
+        """num_vertices defaults to the whole tensor divided by the pipe's
+        vertex struct size (set the pipe first)."""
+        self._keep["vb"] = buf
+        if num_vertices is None:
+            if buf is None:
+                num_vertices = 0
+            else:
+                stride = getattr(self, "_stride", None)
+                if not stride:
+                    raise CrError(CR_ERR_STATE, "set the pixel pipe before sizing the vertex buffer")
+                num_vertices = (buf.numel() * buf.element_size() - int(ofs)) // stride
+        _check(load().cr_set_vertex_buffer(self.ctx, _ptr(buf), int(ofs), int(num_vertices)), self.ctx)
+
+    def setIndexBuffer(self, buf, ofs, num_tris):
+        self._keep["ib"] = buf
+        _check(load().cr_set_index_buffer(self.ctx, _ptr(buf), int(ofs), int(num_tris)), self.ctx)
+
+    def drawTriangles(self, stream=None):
+        s = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
+        _check(load().cr_draw_triangles(self.ctx, s), self.ctx)
+
+    def drawTrianglesAsync(self, stream=None):
+        s = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
+        _check(load().cr_draw_triangles_async(self.ctx, s), self.ctx)
+
+    def drawCheck(self):
+        return load().cr_draw_check(self.ctx)
+
+    def reserve(self, max_subtris, max_bin_entries, max_tile_entries):
+        _check(load().cr_reserve(self.ctx, int(max_subtris), int(max_bin_entries), int(max_tile_entries)), self.ctx)
+
+    def getStats(self) -> dict:
+        s = Stats()
+        _check(load().cr_get_stats(self.ctx, ctypes.byref(s)), self.ctx)
+        return {"setupTime": s.setupTime, "binTime": s.binTime, "coarseTime": s.coarseTime,
+                "fineTime": s.fineTime}
+
+    def getCounters(self) -> dict:
+        c = Counters()
+        _check(load().cr_get_counters(self.ctx, ctypes.byref(c)), self.ctx)
+        return c.as_dict()
+
+    def getProfilingInfo(self) -> str:
+        buf = ctypes.create_string_buffer(4096)
+        _check(load().cr_get_profiling_info(self.ctx, buf, 4096), self.ctx)
+        return buf.value.decode()
+
+    def setDebugParams(self, disable_hiz=False, count_fragments=False):
+        p = DebugParams(1 if disable_hiz else 0, 1 if count_fragments else 0)
+        _check(load().cr_set_debug_params(self.ctx, ctypes.byref(p)), self.ctx)
+
+    def close(self):
+        if self._ctx is not None:
+            load().cr_context_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_buffers(scene, device="cuda"):
+    """Upload a scene's vertex and index buffers as device tensors."""
+    import torch
+    v = torch.from_numpy(np.ascontiguousarray(scene.vertices, dtype=np.float32)).to(device)
+    i = torch.from_numpy(np.ascontiguousarray(scene.indices, dtype=np.int32)).to(device)
+    return v, i
+
+
+def scene_pipe(scene) -> PipeDesc:
+    return pipe_desc(scene.stride, scene.shader, scene.blend, scene.samples_log2, scene.flags,
+                     scene.const_color)
+
+
+class SceneRenderer:
+    """Sets up a CudaRaster for a scene the way test/SceneCR.cpp:149-195 and
+    :243-317 do (surfaces, pipe, deferred clear, buffers, draw)."""
+
+    def __init__(self, scene, triangle_ids=True, device="cuda", debug=None):
+        self.scene = scene
+        n = scene.num_samples
+        self.color = CudaSurface((scene.width, scene.height), FORMAT_RGBA8, n)
+        self.depth = CudaSurface((scene.width, scene.height), FORMAT_DEPTH32, n)
+        self.triid = CudaSurface((scene.width, scene.height), FORMAT_TRIANGLE_ID, n) if triangle_ids else None
+        self.cr = CudaRaster().init()
+        self.cr.setSurfaces(self.color, self.depth)
+        if self.triid is not None:
+            self.cr.setTriangleIdSurface(self.triid)
+        self.cr.setPixelPipe(scene_pipe(scene))
+        if debug:
+            self.cr.setDebugParams(**debug)
+        self.vbuf, self.ibuf = device_buffers(scene, device)
+        self.cr.setVertexBuffer(self.vbuf, 0, scene.vertices.shape[0])
+        self.cr.setIndexBuffer(self.ibuf, 0, scene.num_tris)
+
+    def draw(self, clear=None, stream=None):
+        sc = self.scene
+        if clear if clear is not None else sc.deferred_clear:
+            self.cr.deferredClear(sc.clear_color, sc.clear_depth)
+        self.cr.drawTriangles(stream)
+
+    def read(self):
+        out = {"color": self.color.download(), "depth": self.depth.download()}
+        out["triid"] = self.triid.download() if self.triid is not None else None
+        return out
+
+    def close(self):
+        self.cr.close()
+        for s in (self.color, self.depth, self.triid):
+            if s is not None:
+                s.close()

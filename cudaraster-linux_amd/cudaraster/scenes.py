@@ -1,0 +1,272 @@
+"""Seeded synthetic workloads for the rasterization path.
+
+Every scene is a clip-space vertex buffer (records of ``stride`` bytes, all
+float4: clipPos first, varyings after, as ShadedVertexBase / GouraudVertex
+require -- cudaraster/cuda/PixelPipe.hpp:41-47, 130-133), an ``int3`` index
+buffer (cudaraster/cuda/TriangleSetup.inl:225, 246) and the pipe state it is
+drawn with.  The same buffers feed the GPU path and the CPU oracle.
+
+Workloads:
+  cube()          12-triangle cube of test/App.cpp:208-239 seen through the
+                  demo camera (eye (0,2,4) -> origin, 60 deg, 0.1..100;
+                  test/App.cpp:83-84, 97-102; test/engine/Camera.cpp:57-110)
+  random_tris()   independent triangles: centres uniform over the viewport,
+                  screen area log-uniform, uniform orientation and winding,
+                  view depth uniform per triangle (+-10 % per vertex)
+  skewed()        many sub-pixel triangles plus a set of near-full-screen
+                  triangles at random depths, interleaved by seed
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+# RenderModeFlag_* (cudaraster/cuda/PixelPipe.hpp:30-35)
+FLAG_DEPTH = 1
+FLAG_LERP = 2
+
+BLEND_REPLACE, BLEND_SRCOVER, BLEND_ADDITIVE, BLEND_DEPTHONLY = 0, 1, 2, 3
+SHADER_GOURAUD, SHADER_CONSTANT = 0, 1
+
+ZNEAR, ZFAR = 0.1, 100.0
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    width: int
+    height: int
+    samples_log2: int
+    flags: int
+    blend: int
+    shader: int
+    vertices: np.ndarray          # float32 [V, stride/4]
+    indices: np.ndarray           # int32 [T, 3]
+    clear_color: tuple = (0.2, 0.4, 0.8, 1.0)   # test/SceneCR.cpp:287
+    clear_depth: float = 1.0
+    deferred_clear: bool = True
+    const_color: int = 0xFF0000FF
+
+    @property
+    def stride(self) -> int:
+        return int(self.vertices.shape[1]) * 4
+
+    @property
+    def num_tris(self) -> int:
+        return int(self.indices.shape[0])
+
+    @property
+    def num_samples(self) -> int:
+        return 1 << self.samples_log2
+
+    @property
+    def surface_shape(self):
+        """(rows, texels per row) of a surface: ceil8(H) x ceil8(W)*N."""
+        wr = (self.width + 7) & ~7
+        hr = (self.height + 7) & ~7
+        return hr, wr * self.num_samples
+
+    def input_bytes(self) -> int:
+        return int(self.vertices.nbytes + self.indices.nbytes)
+
+
+# ---------------------------------------------------------------------------
+# Camera (glm 0.9.3 conventions: perspective() takes fovy in degrees,
+# thirdparty/glm/gtc/matrix_transform.inl:223-244)
+# ---------------------------------------------------------------------------
+
+def perspective(fovy_deg, aspect, znear, zfar):
+    rng = math.tan(math.radians(fovy_deg / 2.0)) * znear
+    left, right = -rng * aspect, rng * aspect
+    bottom, top = -rng, rng
+    m = np.zeros((4, 4), dtype=np.float64)
+    m[0, 0] = 2.0 * znear / (right - left)
+    m[1, 1] = 2.0 * znear / (top - bottom)
+    m[2, 2] = -(zfar + znear) / (zfar - znear)
+    m[3, 2] = -1.0
+    m[2, 3] = -(2.0 * zfar * znear) / (zfar - znear)
+    return m
+
+
+def look_at(eye, center, up):
+    eye, center, up = (np.asarray(v, dtype=np.float64) for v in (eye, center, up))
+    f = center - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    m = np.eye(4)
+    m[0, :3], m[1, :3], m[2, :3] = s, u, -f
+    m[0, 3], m[1, 3], m[2, 3] = -s @ eye, -u @ eye, f @ eye
+    return m
+
+
+def demo_camera(width, height):
+    """View-projection of the demo (Camera::setViewParams / setProjectionParams)."""
+    eye, target = np.array([0.0, 2.0, 4.0]), np.array([0.0, 0.0, 0.0])
+    d = target - eye
+    d /= np.linalg.norm(d)
+    up = np.array([0.0, 1.0, 0.0])
+    left = np.cross(up, d)
+    left /= np.linalg.norm(left)
+    up = np.cross(d, left)
+    up /= np.linalg.norm(up)
+    return perspective(60.0, width / height, ZNEAR, ZFAR) @ look_at(eye, target, up)
+
+
+CUBE_POSITIONS = np.array([
+    [-1, -1, -1], [-1, -1, +1], [-1, +1, -1], [-1, +1, +1],
+    [+1, -1, -1], [+1, -1, +1], [+1, +1, -1], [+1, +1, +1]], dtype=np.float64)
+
+CUBE_INDICES = np.array([
+    [7, 3, 1], [7, 1, 5], [7, 5, 6], [6, 5, 4], [6, 4, 2], [2, 4, 0],
+    [2, 0, 3], [3, 0, 1], [3, 7, 6], [3, 6, 2], [5, 1, 0], [5, 0, 4]], dtype=np.int32)
+
+
+def cube(width=1024, height=768, samples_log2=0, seed=0, mvp=None) -> Scene:
+    """The demo cube: 8 vertices, 12 triangles, per-vertex float3 colours
+    (seeded, alpha padded to 1), Gouraud, Replace, Depth|Lerp."""
+    rng = np.random.default_rng(seed)
+    m = demo_camera(width, height) if mvp is None else np.asarray(mvp, dtype=np.float64)
+    pos = np.concatenate([CUBE_POSITIONS, np.ones((8, 1))], axis=1)
+    clip = (pos @ m.T).astype(np.float32)
+    col = np.concatenate([rng.random((8, 3)), np.ones((8, 1))], axis=1).astype(np.float32)
+    verts = np.concatenate([clip, col], axis=1).astype(np.float32)
+    return Scene("cube", width, height, samples_log2, FLAG_DEPTH | FLAG_LERP,
+                 BLEND_REPLACE, SHADER_GOURAUD, np.ascontiguousarray(verts), CUBE_INDICES.copy())
+
+
+def _proj_z(w):
+    """Clip z for eye depth w (= -z_eye) under the demo projection."""
+    a = (ZFAR + ZNEAR) / (ZFAR - ZNEAR)
+    b = 2.0 * ZFAR * ZNEAR / (ZFAR - ZNEAR)
+    return a * w - b
+
+
+def _triangle_shapes(rng, n, area):
+    """Random triangle shapes of the given screen areas (px^2), centred at 0,
+    uniform orientation and random winding."""
+    ang0 = rng.random(n) * 2.0 * math.pi
+    jit = (rng.random((n, 3)) - 0.5) * (math.pi / 3.0)
+    ang = ang0[:, None] + np.array([0.0, 2.0 * math.pi / 3.0, 4.0 * math.pi / 3.0])[None, :] + jit
+    rad = 0.6 + 0.4 * rng.random((n, 3))
+    x = rad * np.cos(ang)
+    y = rad * np.sin(ang)
+    flip = rng.random(n) < 0.5
+    x[flip] = x[flip][:, ::-1]
+    y[flip] = y[flip][:, ::-1]
+    a0 = 0.5 * np.abs((x[:, 1] - x[:, 0]) * (y[:, 2] - y[:, 0]) - (x[:, 2] - x[:, 0]) * (y[:, 1] - y[:, 0]))
+    scale = np.sqrt(area / np.maximum(a0, 1e-12))
+    x = x - x.mean(axis=1, keepdims=True)
+    y = y - y.mean(axis=1, keepdims=True)
+    return x * scale[:, None], y * scale[:, None]
+
+
+def _to_clip(px, py, w, width, height):
+    """Pixel positions (y up, origin bottom-left) at eye depth w -> clip."""
+    nx = px / width * 2.0 - 1.0
+    ny = py / height * 2.0 - 1.0
+    return np.stack([nx * w, ny * w, _proj_z(w), w], axis=-1)
+
+
+def random_tris(num_tris, width=2048, height=2048, samples_log2=2, seed=1,
+                area_range=(0.5, 256.0), depth_range=(ZNEAR, ZFAR), alpha_range=None,
+                flags=FLAG_DEPTH | FLAG_LERP, blend=BLEND_REPLACE, name="random") -> Scene:
+    """Independent triangles (3 vertices each): centre uniform over the
+    viewport, area log-uniform in area_range, uniform orientation (about half
+    back-facing), eye depth uniform in depth_range per triangle with +-10 %
+    per-vertex variation, float4 colours uniform in [0,1] (alpha uniform in
+    alpha_range when given)."""
+    rng = np.random.default_rng(seed)
+    n = int(num_tris)
+    cx = rng.random(n) * width
+    cy = rng.random(n) * height
+    la, lb = math.log(area_range[0]), math.log(area_range[1])
+    area = np.exp(la + (lb - la) * rng.random(n))
+    sx, sy = _triangle_shapes(rng, n, area)
+    wt = depth_range[0] + (depth_range[1] - depth_range[0]) * rng.random(n)
+    wv = np.clip(wt[:, None] * (0.9 + 0.2 * rng.random((n, 3))), depth_range[0], depth_range[1])
+    clip = _to_clip(cx[:, None] + sx, cy[:, None] + sy, wv, width, height)   # [n,3,4]
+    col = rng.random((n, 3, 4))
+    if alpha_range is not None:
+        col[..., 3] = alpha_range[0] + (alpha_range[1] - alpha_range[0]) * rng.random((n, 3))
+    verts = np.concatenate([clip, col], axis=-1).reshape(n * 3, 8).astype(np.float32)
+    idx = np.arange(n * 3, dtype=np.int32).reshape(n, 3)
+    return Scene(name, width, height, samples_log2, flags, blend, SHADER_GOURAUD,
+                 np.ascontiguousarray(verts), idx)
+
+
+def skewed(num_small=4_000_000, num_big=1024, width=2048, height=2048, samples_log2=3,
+           seed=2, area_range=(0.05, 1.0), alpha_range=(0.2, 0.8)) -> Scene:
+    """Sub-pixel triangles uniform over the screen plus near-full-screen
+    triangles at random depths, interleaved at seeded positions; SrcOver."""
+    rng = np.random.default_rng(seed)
+    small = random_tris(num_small, width, height, samples_log2, seed=seed + 1000,
+                        area_range=area_range, alpha_range=alpha_range)
+    nb = int(num_big)
+    # Big triangles: one vertex near each of three screen regions, spanning
+    # roughly the whole viewport (some exceed the guard band and are clipped).
+    u = rng.random((nb, 3, 2))
+    base = np.array([[-0.15, -0.15], [1.15, -0.10], [0.5, 1.2]])
+    jitter = (u - 0.5) * 0.5
+    pts = (base[None] + jitter) * np.array([width, height])
+    # Force counter-clockwise winding so every big triangle is front-facing.
+    px, py = pts[..., 0], pts[..., 1]
+    area2 = (px[:, 1] - px[:, 0]) * (py[:, 2] - py[:, 0]) - (px[:, 2] - px[:, 0]) * (py[:, 1] - py[:, 0])
+    cw = area2 < 0
+    px[cw] = px[cw][:, ::-1]
+    py[cw] = py[cw][:, ::-1]
+    wt = ZNEAR + (ZFAR - ZNEAR) * rng.random(nb)
+    wv = np.clip(wt[:, None] * (0.9 + 0.2 * rng.random((nb, 3))), ZNEAR, ZFAR)
+    clip = _to_clip(px, py, wv, width, height)
+    col = rng.random((nb, 3, 4))
+    col[..., 3] = alpha_range[0] + (alpha_range[1] - alpha_range[0]) * rng.random((nb, 3))
+    big = np.concatenate([clip, col], axis=-1).reshape(nb * 3, 8).astype(np.float32)
+
+    total = small.num_tris + nb
+    is_big = np.zeros(total, dtype=bool)
+    is_big[rng.choice(total, size=nb, replace=False)] = True
+    verts = np.empty((total * 3, 8), dtype=np.float32)
+    vb = np.repeat(is_big, 3)
+    verts[vb] = big
+    verts[~vb] = small.vertices
+    idx = np.arange(total * 3, dtype=np.int32).reshape(total, 3)
+    return Scene("skewed", width, height, samples_log2, FLAG_DEPTH | FLAG_LERP,
+                 BLEND_SRCOVER, SHADER_GOURAUD, verts, idx)
+
+
+def clip_mix(num_tris, width=256, height=256, samples_log2=2, seed=3) -> Scene:
+    """Clip-space triangles that straddle the frustum: vertices behind the
+    near plane, beyond the far plane and far outside the guard band, so the
+    setup's clip path (frustum clip + fan) and frustum cull are exercised."""
+    rng = np.random.default_rng(seed)
+    n = int(num_tris)
+    w = 0.05 + 2.0 * rng.random((n, 3))
+    xy = (rng.random((n, 3, 2)) * 2.0 - 1.0) * w[..., None] * (1.0 + 6.0 * rng.random((n, 1, 1)))
+    zndc = rng.random((n, 3)) * 2.6 - 1.3
+    pos = np.concatenate([xy, (zndc * w)[..., None], w[..., None]], axis=-1)
+    col = rng.random((n, 3, 4))
+    col[..., 3] = 0.2 + 0.6 * rng.random((n, 3))
+    verts = np.concatenate([pos, col], axis=-1).reshape(n * 3, 8).astype(np.float32)
+    idx = np.arange(n * 3, dtype=np.int32).reshape(n, 3)
+    return Scene("clip_mix", width, height, samples_log2, FLAG_DEPTH | FLAG_LERP, BLEND_SRCOVER,
+                 SHADER_GOURAUD, verts, idx)
+
+
+def config(i: int, scale: float = 1.0) -> Scene:
+    """The plain-method workloads, optionally scaled down (triangle counts and
+    resolution) for oracle-sized parity runs."""
+    if i == 0:
+        return cube()
+    if i == 1:
+        n = max(1, int(1_000_000 * scale))
+        res = max(64, int(round(2048 * math.sqrt(scale) / 8)) * 8) if scale < 1.0 else 2048
+        return random_tris(n, res, res, samples_log2=2, seed=1, name="random_1m")
+    if i == 2:
+        ns = max(1, int(4_000_000 * scale))
+        nb = max(1, int(1024 * scale)) if scale < 1.0 else 1024
+        res = max(64, int(round(2048 * math.sqrt(scale) / 8)) * 8) if scale < 1.0 else 2048
+        return skewed(ns, nb, res, res)
+    raise ValueError(f"no plain-method workload {i}")

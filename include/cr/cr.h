@@ -1,0 +1,198 @@
+/*
+ * cr.h -- C ABI of the MI355X (gfx950) rasterizer, libcr.so.
+ *
+ * Drop-in boundary for the reference's host API, FW::CudaRaster
+ * (src/cudaraster/CudaRaster.hpp:143-168) and FW::CudaSurface
+ * (src/cudaraster/CudaSurface.hpp:60-71).  Plain pointers and sizes only.
+ * Each entry point names the reference member it replaces.
+ *
+ * Errors: the reference calls fail() and exits (base/Defs.hpp:107-117).
+ * Here every function returns a cr_status; cr_last_error() gives the text of
+ * the last failure on that context (or of the last context-less failure when
+ * ctx is NULL).  Misuse the reference rejects with fail() is rejected with
+ * the same condition and CR_ERR_INVALID / CR_ERR_STATE.
+ *
+ * Threading: a context is single-threaded, as the reference (one host
+ * thread, shared module globals).  Different contexts are independent.
+ *
+ * Memory: vertex and index buffers are caller-owned device pointers
+ * (hipMalloc'ed or torch tensors).  Surfaces are device buffers owned by the
+ * library, laid out exactly as the reference's CUDA arrays
+ * (CudaSurface.cpp:86-91): (ceil8(W) * samples) x ceil8(H) 32-bit texels,
+ * row 0 at the bottom, an 8x8 tile replicated horizontally once per sample:
+ * texel (tileX*8*N + sample*8 + (x & 7), y).
+ */
+#ifndef CR_CR_H
+#define CR_CR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cr_status {
+    CR_OK = 0,
+    CR_ERR_INVALID = 1,      /* bad argument */
+    CR_ERR_STATE = 2,        /* call order / missing state */
+    CR_ERR_DEVICE = 3,       /* HIP runtime error */
+    CR_ERR_OVERFLOW = 4,     /* internal capacity exceeded beyond the limits */
+    CR_ERR_UNSUPPORTED = 5,
+    CR_ERR_NOMEM = 6
+} cr_status;
+
+typedef struct cr_context cr_context;
+typedef struct cr_surface cr_surface;
+
+/* CudaSurface::Format (CudaSurface.hpp:39-45) plus a 32-bit triangle-id
+ * target (per-sample index of the last triangle written, 0xFFFFFFFF where
+ * none was). */
+typedef enum cr_format {
+    CR_FORMAT_RGBA8 = 0,
+    CR_FORMAT_DEPTH32 = 1,
+    CR_FORMAT_TRIANGLE_ID = 2
+} cr_format;
+
+/* RenderModeFlag_* (cudaraster/cuda/PixelPipe.hpp:30-35). */
+enum {
+    CR_RENDER_DEPTH = 1 << 0,
+    CR_RENDER_LERP = 1 << 1,
+    CR_RENDER_QUADS = 1 << 2     /* accepted only as 0: numerical derivatives are not built */
+};
+
+/* Stock fragment shaders.  GOURAUD = GouraudShader (PixelPipe.inl:65-69):
+ * toABGR of varying 0 interpolated at the centroid.  CONSTANT writes
+ * pipe.const_color (FragmentShader_passthrough, test/shader/PassThrough.cu:44-54,
+ * is CONSTANT with 0xFF0000FF). */
+typedef enum cr_fragment_shader {
+    CR_SHADER_GOURAUD = 0,
+    CR_SHADER_CONSTANT = 1
+} cr_fragment_shader;
+
+/* Stock blend shaders (PixelPipe.hpp:145-175, PixelPipe.inl:73-85). */
+typedef enum cr_blend_shader {
+    CR_BLEND_REPLACE = 0,
+    CR_BLEND_SRC_OVER = 1,
+    CR_BLEND_ADDITIVE = 2,
+    CR_BLEND_DEPTH_ONLY = 3
+} cr_blend_shader;
+
+/* What CR_DEFINE_PIXEL_PIPE fixes at compile time in the reference
+ * (PixelPipe.inl:241-275): vertex struct size, shaders, samples, flags.
+ * All variants are compiled ahead of time for gfx950. */
+typedef struct cr_pipe_desc {
+    int32_t vertex_struct_size;  /* bytes, multiple of 16, >= 16 (>= 32 for GOURAUD) */
+    int32_t fragment_shader;     /* cr_fragment_shader */
+    int32_t blend_shader;        /* cr_blend_shader */
+    int32_t samples_log2;        /* 0..3 */
+    uint32_t render_mode_flags;  /* CR_RENDER_* */
+    uint32_t const_color;        /* ABGR8888 for CR_SHADER_CONSTANT */
+} cr_pipe_desc;
+
+/* CudaRaster::Stats (CudaRaster.hpp:45-51): seconds per stage of the last
+ * draw, from HIP events.  bin_time covers the bin count/scan/scatter
+ * kernels. */
+typedef struct cr_stats {
+    float setup_time;
+    float bin_time;
+    float coarse_time;
+    float fine_time;
+} cr_stats;
+
+/* Counters of the last draw (the reference's ProfilingMode_Counters subset
+ * that the pipeline keeps without slowing down). */
+typedef struct cr_counters {
+    int64_t num_tris;
+    int64_t num_subtris;         /* slots used (numTris + clipped extra) */
+    int64_t bin_entries;
+    int64_t tile_entries;
+    int64_t fragments;           /* (subtriangle, pixel) pairs with >= 1 covered sample;
+                                    exact only with hierarchical Z off */
+    int64_t retries;             /* capacity grow-and-retry rounds */
+} cr_counters;
+
+/* CudaRaster::DebugParams (CudaRaster.hpp:53-67) has per-stage host
+ * emulation; this library has one GPU implementation per stage and keeps
+ * the knobs that change execution but never the image. */
+typedef struct cr_debug_params {
+    int32_t disable_hiz;         /* 1: no tile-level early depth rejection */
+    int32_t count_fragments;     /* 1: exact fragment count (implies disable_hiz) */
+} cr_debug_params;
+
+/* ---- context (CudaRaster ctor + init, CudaRaster.cpp:98-128) ---- */
+cr_status cr_context_create(cr_context** out);
+cr_status cr_context_destroy(cr_context* ctx);
+const char* cr_last_error(const cr_context* ctx);
+
+/* ---- surfaces (CudaSurface ctor, CudaSurface.cpp:39-110) ----
+ * Size 1..2048 per axis, samples 1, 2, 4 or 8. */
+cr_status cr_surface_create(int32_t width, int32_t height, int32_t format, int32_t num_samples,
+                            cr_surface** out);
+cr_status cr_surface_destroy(cr_surface* surf);
+/* getSize / getRoundedSize / getTextureSize / getNumSamples. */
+cr_status cr_surface_get_info(const cr_surface* surf, int32_t* width, int32_t* height,
+                              int32_t* rounded_width, int32_t* rounded_height,
+                              int32_t* texture_width, int32_t* num_samples, int32_t* format);
+/* Device pointer of the texels (replaces getCudaArray). */
+cr_status cr_surface_get_device_ptr(const cr_surface* surf, void** dev_ptr, size_t* bytes);
+/* Synchronous copies of the whole texture (texture_width x rounded_height u32). */
+cr_status cr_surface_download(const cr_surface* surf, void* host, size_t bytes);
+cr_status cr_surface_upload(cr_surface* surf, const void* host, size_t bytes);
+
+/* ---- pipeline state ---- */
+/* CudaRaster::setSurfaces (CudaRaster.cpp:132-170).  Both NULL unsets. */
+cr_status cr_set_surfaces(cr_context* ctx, cr_surface* color, cr_surface* depth);
+/* Optional triangle-id target (CR_FORMAT_TRIANGLE_ID, same size/samples);
+ * NULL disables it.  No reference counterpart (per-sample triangle index
+ * of the last written fragment). */
+cr_status cr_set_triangle_id_surface(cr_context* ctx, cr_surface* tri_id);
+/* CudaRaster::deferredClear (CudaRaster.cpp:174-179): rgba as Vec4f, depth
+ * in [0,1]. Applies to the next draw. */
+cr_status cr_deferred_clear(cr_context* ctx, const float rgba[4], float depth);
+/* CudaRaster::setPixelPipe (CudaRaster.cpp:183-216). */
+cr_status cr_set_pixel_pipe(cr_context* ctx, const cr_pipe_desc* pipe);
+/* Stock pipe by name, as the demo's CR_DEFINE_PIXEL_PIPE names:
+ * "PixelPipe_passthrough" (constant red, 16-byte vertices) and
+ * "PixelPipe_gouraud" (GouraudVertex, GouraudShader); blend, samples and
+ * flags as the demo's -D defines (test/SceneCR.cpp:165-177). */
+cr_status cr_pipe_lookup(const char* name, int32_t blend_shader, int32_t samples_log2,
+                         uint32_t render_mode_flags, cr_pipe_desc* out);
+/* CudaRaster::setVertexBuffer (CudaRaster.cpp:220-224): device pointer +
+ * byte offset; num_vertices bounds the indices (out-of-range triangles are
+ * dropped). */
+cr_status cr_set_vertex_buffer(cr_context* ctx, const void* dev_ptr, int64_t ofs, int64_t num_vertices);
+/* CudaRaster::setIndexBuffer (CudaRaster.cpp:228-233): int3 per triangle. */
+cr_status cr_set_index_buffer(cr_context* ctx, const void* dev_ptr, int64_t ofs, int32_t num_tris);
+
+/* CudaRaster::drawTriangles (CudaRaster.cpp:237-342).  Launches the four
+ * stages on `stream` (a hipStream_t; NULL = default stream), reads back the
+ * overflow counters, grows the internal buffers and re-runs when needed, and
+ * returns once the frame is complete -- the reference's synchronous
+ * semantics. */
+cr_status cr_draw_triangles(cr_context* ctx, void* stream);
+/* Asynchronous variant (no counterpart in the reference): launches with the
+ * current capacities and returns without waiting.  cr_draw_check() waits
+ * for that draw and reports CR_ERR_OVERFLOW if it needs a re-run (the
+ * surfaces are then unchanged from before the draw when deferred clear was
+ * off, or cleared when it was on).  cr_reserve() pre-sizes the capacities. */
+cr_status cr_draw_triangles_async(cr_context* ctx, void* stream);
+cr_status cr_draw_check(cr_context* ctx);
+cr_status cr_reserve(cr_context* ctx, int64_t max_subtris, int64_t max_bin_entries,
+                     int64_t max_tile_entries);
+
+/* CudaRaster::getStats (CudaRaster.cpp:346-363). */
+cr_status cr_get_stats(cr_context* ctx, cr_stats* out);
+cr_status cr_get_counters(cr_context* ctx, cr_counters* out);
+/* CudaRaster::getProfilingInfo (CudaRaster.cpp:367-497): text report. */
+cr_status cr_get_profiling_info(cr_context* ctx, char* buf, size_t len);
+/* CudaRaster::setDebugParams (CudaRaster.cpp:501-504). */
+cr_status cr_set_debug_params(cr_context* ctx, const cr_debug_params* p);
+
+/* Version of the ABI (major << 16 | minor). */
+int32_t cr_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,146 @@
+/*
+ * CPU oracle for the four-stage rasterization path (test infrastructure).
+ *
+ * This library is the checker, never the product: only tests/, the smoke
+ * entry point and the CPU-baseline leg of bench.py load it.  It restates the
+ * reference's rasterization rules in plain C (C11 + pthreads):
+ *
+ *   triangle setup   cudaraster/cuda/TriangleSetup.inl:19-417
+ *   plane equations  cudaraster/cuda/Util.inl:65-91 (GPU variant, not the
+ *                    host emulation's F64 variant at CudaRaster.cpp:669-688)
+ *   clipping         cudaraster/cuda/Util.hpp:221-286
+ *   coverage         cudaraster/cuda/Util.inl:95-134, 340-357 (closed forms)
+ *   MSAA / centroid  cudaraster/cuda/Util.hpp:27-36, 182-203
+ *   depth encoding   cudaraster/cuda/Util.hpp:290-310, CudaRaster.cpp:174-179
+ *   shading, blend   cudaraster/cuda/FineRaster.inl:21-119, Util.inl:30-60,
+ *                    PixelPipe.inl:52-85
+ *   ordering / ROP   sequential per-sample semantics of the host emulation,
+ *                    CudaRaster.cpp:1184-1403 (triangle order, LESS, blend)
+ *   surface layout   CudaSurface.cpp:86-91, FineRaster.inl:909, 1088-1100
+ *
+ * Rendering is "sort-nothing": every subtriangle is rasterized in submission
+ * order over the samples of its bounding box.  The multi-threaded mode splits
+ * the surface into horizontal bands; each band is owned by one thread and
+ * walks the same ordered subtriangle list, so its output is bit-identical to
+ * the single-threaded run.
+ */
+#ifndef CR_ORACLE_H
+#define CR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CRO_FLAG_DEPTH = 1,   /* RenderModeFlag_EnableDepth (PixelPipe.hpp:32) */
+    CRO_FLAG_LERP  = 2,   /* RenderModeFlag_EnableLerp  (PixelPipe.hpp:33) */
+};
+
+enum {
+    CRO_BLEND_REPLACE   = 0,  /* PixelPipe.hpp:145-150 */
+    CRO_BLEND_SRCOVER   = 1,  /* PixelPipe.hpp:154-158, PixelPipe.inl:73-77 */
+    CRO_BLEND_ADDITIVE  = 2,  /* PixelPipe.hpp:162-166, PixelPipe.inl:81-85 */
+    CRO_BLEND_DEPTHONLY = 3,  /* PixelPipe.hpp:170-175 */
+};
+
+enum {
+    CRO_SHADER_GOURAUD  = 0,  /* PixelPipe.inl:65-69: toABGR(lerp(varying 0)) */
+    CRO_SHADER_CONSTANT = 1,  /* test/shader/PassThrough.cu:44-54 style */
+};
+
+typedef struct {
+    int32_t  width;           /* viewport width in pixels (1..2048)  */
+    int32_t  height;          /* viewport height in pixels (1..2048) */
+    int32_t  samplesLog2;     /* 0..3 */
+    uint32_t renderModeFlags; /* CRO_FLAG_* */
+    int32_t  blend;           /* CRO_BLEND_* */
+    int32_t  shader;          /* CRO_SHADER_* */
+    uint32_t constColor;      /* ABGR8888 for CRO_SHADER_CONSTANT */
+    int32_t  vertexStride;    /* bytes per vertex record, multiple of 16 */
+    int32_t  deferredClear;   /* 1: clear before drawing */
+    uint32_t clearColor;      /* ABGR8888 */
+    uint32_t clearDepth;      /* encoded depth */
+} cro_pipe;
+
+/* CRTriangleHeader (PrivateDefs.hpp:26-36). */
+typedef struct {
+    int16_t  v0x, v0y, v1x, v1y, v2x, v2y;
+    uint32_t misc;
+} cro_header;
+
+/* CRTriangleData (PrivateDefs.hpp:40-62). */
+typedef struct {
+    uint32_t zx, zy, zb, zslope;
+    int32_t  wx, wy, wb;
+    int32_t  ux, uy, ub;
+    int32_t  vx, vy, vb;
+    uint32_t vi0, vi1, vi2;
+} cro_data;
+
+typedef struct {
+    int64_t numTris;
+    int64_t numSubtris;        /* subtriangles emitted (visible) */
+    int64_t culledFrustum;     /* TriangleSetup.inl:262-281 */
+    int64_t culledBackface;    /* prepareTriangle == 1, fast path */
+    int64_t culledBetween;     /* prepareTriangle == 2, fast path */
+    int64_t clipped;           /* took the clip path */
+    int64_t fragments;         /* (subtriangle, pixel) pairs with >= 1 covered sample */
+} cro_stats;
+
+/* ---- single rules (exported for the pinning tests) ---- */
+uint32_t cro_to_abgr(const float c[4]);            /* Util.inl:30-38 (device)   */
+uint32_t cro_to_abgr_host(const float c[4]);       /* Math.cpp:41-48 (host)     */
+uint32_t cro_blend(int32_t mode, uint32_t src, uint32_t dst);
+uint32_t cro_encode_depth(uint32_t depth);         /* Util.hpp:290-298 */
+uint32_t cro_decode_depth(uint32_t depth);         /* Util.hpp:302-310 */
+uint32_t cro_clear_depth(float depth);             /* CudaRaster.cpp:178 */
+int32_t  cro_msaa_pattern(int32_t samplesLog2, int32_t i);
+int32_t  cro_select_centroid(int32_t samplesLog2, uint32_t mask);
+void     cro_setup_pleq(const float values[3], int32_t v0x, int32_t v0y,
+                        int32_t d1x, int32_t d1y, int32_t d2x, int32_t d2y,
+                        float areaRcp, int32_t samplesLog2, uint32_t out[3]);
+int32_t  cro_clip_triangle(float bary[18], const float v0[4], const float v1[4],
+                           const float v2[4]);
+/* Per-sample coverage of a header triangle; sample given in absolute
+ * subpixel coordinates (pixel i spans [16i, 16i+16)). */
+int32_t  cro_cover_point(const cro_header* h, int32_t width, int32_t height,
+                         int64_t sx, int64_t sy);
+/* Sample coverage mask of pixel (px,py) for a header triangle. */
+uint32_t cro_cover_pixel(const cro_header* h, int32_t width, int32_t height,
+                         int32_t samplesLog2, int32_t px, int32_t py);
+
+/* Set up one triangle.  Writes up to 7 subtriangles (fan order) and returns
+ * their number; *cull receives 0 visible, 1 frustum, 2 backface, 3 between
+ * samples, 4 clipped-to-nothing, 5 clip path. */
+int32_t  cro_setup_triangle(const cro_pipe* pipe, const float v0[4], const float v1[4],
+                            const float v2[4], const int32_t vidx[3],
+                            cro_header hdr[7], cro_data data[7], int32_t* cull);
+
+/* Whole-buffer setup: slot t holds triangle t when it has one subtriangle;
+ * triangles with >= 2 subtriangles get consecutive slots starting at
+ * numTris + (prefix over earlier clipped triangles) and hdr[t].misc = base.
+ * subtriTri[slot] = source triangle index.  Returns total slots used
+ * (numTris + clipped extra) or -1 if maxSlots is too small. */
+int64_t  cro_setup(const cro_pipe* pipe, const void* vertices, int64_t numVertices,
+                   const int32_t* indices, int64_t numTris,
+                   uint8_t* triSubtris, cro_header* hdr, cro_data* data,
+                   uint32_t* subtriTri, int64_t maxSlots, int32_t nthreads);
+
+/* Full draw: setup + in-order rasterization into surfaces laid out as
+ * (ceil8(W) * N) x ceil8(H) texels (CudaSurface.cpp:86-91).  color/depth
+ * must hold the previous contents when deferredClear == 0.  triid may be
+ * NULL; when given it receives the source triangle index of the last
+ * fragment that passed for each sample, 0xFFFFFFFF where none did (and is
+ * cleared to 0xFFFFFFFF by a deferred clear).  Returns 0 or -1 on bad
+ * arguments. */
+int32_t  cro_render(const cro_pipe* pipe, const void* vertices, int64_t numVertices,
+                    const int32_t* indices, int64_t numTris,
+                    uint32_t* color, uint32_t* depth, uint32_t* triid,
+                    int32_t nthreads, cro_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,66 @@
+"""C ABI: libcr.so loads on a machine without a GPU and exports every entry
+point include/cr/cr.h declares; argument checks that precede any device
+call behave as the reference's fail() conditions (CudaSurface.cpp:44-62,
+CudaRaster.cpp:183-216)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(REPO, "include", "cr", "cr.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cr_[a-z_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import cudaraster
+    return cudaraster.load()
+
+
+def test_library_exports_header(lib):
+    import cudaraster
+    fns = header_functions()
+    assert len(fns) >= 20
+    for f in fns:
+        assert hasattr(lib, f), f"libcr.so does not export {f}"
+    assert sorted(cudaraster.EXPORTS) == fns
+
+
+def test_abi_version(lib):
+    assert lib.cr_abi_version() >> 16 == 1
+
+
+@pytest.mark.parametrize("w,h,fmt,ns", [(0, 10, 0, 1), (10, -1, 0, 1), (2049, 10, 0, 1), (10, 10, 7, 1),
+                                        (10, 10, 0, 16), (10, 10, 0, 3), (10, 10, 0, 0)])
+def test_surface_argument_checks(lib, w, h, fmt, ns):
+    import cudaraster
+    out = ctypes.c_void_p()
+    st = lib.cr_surface_create(w, h, fmt, ns, ctypes.byref(out))
+    assert st == cudaraster.CR_ERR_INVALID
+    assert out.value is None
+    assert lib.cr_last_error(None)
+
+
+def test_pipe_lookup(lib):
+    import cudaraster
+    d = cudaraster.pipe_lookup("PixelPipe_passthrough", cudaraster.BLEND_SRC_OVER, 2, 3)
+    assert (d.vertex_struct_size, d.fragment_shader, d.const_color) == (16, cudaraster.SHADER_CONSTANT, 0xFF0000FF)
+    assert (d.blend_shader, d.samples_log2, d.render_mode_flags) == (1, 2, 3)
+    d = cudaraster.pipe_lookup("PixelPipe_gouraud")
+    assert (d.vertex_struct_size, d.fragment_shader) == (32, cudaraster.SHADER_GOURAUD)
+    with pytest.raises(cudaraster.CrError):
+        cudaraster.pipe_lookup("PixelPipe_nonexistent")
+
+
+def test_null_arguments(lib):
+    import cudaraster
+    assert lib.cr_draw_triangles(None, None) == cudaraster.CR_ERR_INVALID
+    assert lib.cr_set_surfaces(None, None, None) == cudaraster.CR_ERR_INVALID
+    assert lib.cr_context_create(None) == cudaraster.CR_ERR_INVALID
+    assert lib.cr_context_destroy(None) == cudaraster.CR_OK
