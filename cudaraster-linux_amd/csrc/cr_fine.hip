@@ -4,18 +4,40 @@
 // (cudaraster/cuda/FineRaster.inl:499-752, 855-1126): for every triangle of
 // the tile, in submission order, exact sample coverage (top-left rule), LESS
 // depth test, the fragment shader at the pixel centre / coverage centroid,
-// and the blend ROP.  The wave64 design differs from the reference's
-// fragment-distribution warp: each lane owns one pixel and keeps all of its
-// samples (colour, depth, triangle id) in registers for the whole tile, so
-// fragments of the same pixel are applied in order by construction -- no
-// intra-warp conflict resolution, no read-modify-write of the surface per
-// sample, one coalesced read and one coalesced write of the tile per draw.
-// The triangle is wave-uniform (scalar loads); coverage is three integer edge
-// tests per sample in 32-bit arithmetic (see coverage bounds below); the
-// 64-bit ballot of per-pixel coverage gives empty-triangle rejection.
+// and the blend ROP.
+//
+// Wave64 design (not the reference's fragment-distribution warp):
+//  * each lane owns one pixel and keeps all its samples (depth, colour or
+//    visibility, triangle id) in registers for the whole tile, so fragments
+//    of one pixel are applied in order by construction -- no intra-wave
+//    conflict resolution and exactly one coalesced read/write of the tile;
+//  * the tile's triangle list is consumed in chunks of 64: lane j sets up
+//    triangle j (tile-relative edge thresholds per sample, depth plane per
+//    sample, hierarchical-Z test), survivors are compacted into an LDS
+//    record array, and the wave then walks the records with broadcast LDS
+//    reads -- per (triangle, tile) the lanes only do integer compares, one
+//    depth add per sample and register selects;
+//  * for Replace and DepthOnly blending the colour written to a sample is
+//    the shaded colour of the last fragment that passed there, so shading is
+//    deferred: the loop records (subtriangle slot, centroid) per sample and
+//    the tile is shaded once per (pixel, distinct triangle) at the end.
+//    SrcOver and Additive blend in submission order inside the loop.
 #include "cr_device.h"
 
 namespace crhip {
+
+template <int S>
+struct Rec {
+  static constexpr int N = 1 << S;
+  static constexpr int R = 0;                  // 3*N coverage thresholds, R[k*N+i]
+  static constexpr int D = 3 * N;              // dx0 dy0 dx1 dy1 dx2 dy2
+  static constexpr int Z = D + 6;              // zx, zy, Zs[0..N-1]
+  static constexpr int SLOT = Z + 2 + N;       // subtriangle slot
+  static constexpr int TRI = SLOT + 1;         // source triangle index
+  static constexpr int SH = TRI + 1;           // in-order shading: W/U/V planes (9) + colours (12)
+  static constexpr int SIZE_VIS = (SH + 3) & ~3;
+  static constexpr int SIZE_FULL = (SH + 21 + 3) & ~3;
+};
 
 __device__ __forceinline__ uint32_t waveMax(uint32_t v) {
 #pragma unroll
@@ -23,12 +45,49 @@ __device__ __forceinline__ uint32_t waveMax(uint32_t v) {
   return v;
 }
 
-template <int S>
-__device__ __forceinline__ int centroidSample(uint32_t mask) {
-  return selectCentroid(S, mask);
+__device__ __forceinline__ uint64_t laneMaskLt64() {
+  const uint32_t lane = threadIdx.x & 63;
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
 template <int S>
+__device__ __forceinline__ int centroidCode(uint32_t cover) {
+  const int c = selectCentroid(S, cover);
+  return c < 0 ? 15 : c;
+}
+
+// Perspective-correct barycentrics at the centroid (computeBarys,
+// FineRaster.inl:21-44, 93-114) and GouraudShader + toABGR
+// (PixelPipe.inl:52-69).
+template <int S>
+__device__ __forceinline__ uint32_t shadeGouraud(bool lerpOn, int code, int px, int py, const uint4& t1,
+                                                 const uint4& t2, const uint4& t3, const float4& c0,
+                                                 const float4& c1, const float4& c2) {
+  constexpr int N = 1 << S;
+  float b0, b1, b2;
+  if (lerpOn) {
+    const int cx = (code == 15) ? N : 2 * msaaPattern(S, code) + 1;
+    const int cy = (code == 15) ? N : 2 * code + 1;
+    const int32_t sx = px * (2 * N) + cx, sy = py * (2 * N) + cy;
+    const int32_t W = wadd(wadd(wmul((int32_t)t1.x, sx), wmul((int32_t)t1.y, sy)), (int32_t)t1.z);
+    const int32_t U = wadd(wadd(wmul((int32_t)t1.w, sx), wmul((int32_t)t2.x, sy)), (int32_t)t2.y);
+    const int32_t V = wadd(wadd(wmul((int32_t)t2.z, sx), wmul((int32_t)t2.w, sy)), (int32_t)t3.x);
+    const float w = 1.0f / (float)W;
+    const float u = w * (float)U;
+    const float v = w * (float)V;
+    b0 = 1.0f - u - v; b1 = u; b2 = v;
+  } else {
+    b0 = 0.0f; b1 = 0.0f; b2 = 1.0f;                  // FineRaster.inl:72-81
+  }
+  return toABGR(fmaf(c2.x, b2, fmaf(c0.x, b0, c1.x * b1)), fmaf(c2.y, b2, fmaf(c0.y, b0, c1.y * b1)),
+                fmaf(c2.z, b2, fmaf(c0.z, b0, c1.z * b1)), fmaf(c2.w, b2, fmaf(c0.w, b0, c1.w * b1)));
+}
+
+__device__ __forceinline__ float4 varying0(const uint8_t* verts, uint32_t vi, int stride) {
+  return *reinterpret_cast<const float4*>(verts + (size_t)vi * stride + 16);
+}
+
+template <int S, bool VIS>
 __global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __restrict__ hdr,
                                                  const uint4* __restrict__ tdata,
                                                  const uint32_t* __restrict__ tileList,
@@ -37,10 +96,14 @@ __global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __
                                                  uint32_t* __restrict__ color, uint32_t* __restrict__ depth,
                                                  uint32_t* __restrict__ triid) {
   constexpr int N = 1 << S;
+  using L = Rec<S>;
+  constexpr int RS = VIS ? L::SIZE_VIS : L::SIZE_FULL;
+  __shared__ __attribute__((aligned(16))) uint32_t rec[64 * RS];
+
   if (P.counters->overflow) return;
-  // XCD-aware order: consecutive workgroups go round-robin to the 8 XCDs;
-  // give each XCD one contiguous band of tiles so that triangles shared by
-  // neighbouring tiles stay in one L2.
+  // XCD-aware order: workgroups go round-robin to the 8 XCDs; each XCD gets
+  // one contiguous band of tiles so triangles shared by neighbouring tiles
+  // stay in one L2.
   const int per = (P.numTiles + 7) >> 3;
   const int tile = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
   if (tile >= P.numTiles) return;
@@ -56,17 +119,18 @@ __global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __
   const bool depthOn = (P.flags & kFlagDepth) != 0;
   const bool lerpOn = (P.flags & kFlagLerp) != 0;
   const bool idOn = triid != nullptr;
+  const bool gouraud = P.shader == kShaderGouraud;
 
-  uint32_t col[N], dep[N], tid[N];
+  uint32_t col[N], dep[N], aux[N];   // aux: VIS -> (slot << 4 | centroid) or ~0; else triangle id
   if (P.deferredClear) {
 #pragma unroll
-    for (int i = 0; i < N; i++) { col[i] = P.clearColor; dep[i] = P.clearDepth; tid[i] = 0xFFFFFFFFu; }
+    for (int i = 0; i < N; i++) { col[i] = P.clearColor; dep[i] = P.clearDepth; aux[i] = 0xFFFFFFFFu; }
   } else {
 #pragma unroll
     for (int i = 0; i < N; i++) {
       col[i] = color[texel + i * 8];
       dep[i] = depth[texel + i * 8];
-      tid[i] = idOn ? triid[texel + i * 8] : 0xFFFFFFFFu;
+      aux[i] = VIS ? 0xFFFFFFFFu : (idOn ? triid[texel + i * 8] : 0xFFFFFFFFu);
     }
   }
 
@@ -74,140 +138,224 @@ __global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __
   const int32_t Tx = (tx << (kTileLog2 + kSubpixelLog2)) - (P.viewportWidth << (kSubpixelLog2 - 1));
   const int32_t Ty = (ty << (kTileLog2 + kSubpixelLog2)) - (P.viewportHeight << (kSubpixelLog2 - 1));
   const int32_t lx16 = lx << kSubpixelLog2, ly16 = ly << kSubpixelLog2;
+  const uint32_t lxN = (uint32_t)(lx * N), lyN = (uint32_t)(ly * N);
+  const uint32_t tileSx = (uint32_t)(tx * 8 * N), tileSy = (uint32_t)(ty * 8 * N);
 
-  uint32_t tileZMax = 0;
-  bool zdirty = true;
   uint64_t frags = 0;
   const uint32_t start = P.tileStart[tile];
+  const bool hiz = depthOn && P.hiz;
 
   for (uint32_t c = 0; c < cnt; c += 64) {
-    const uint32_t n = min(64u, cnt - c);
-    const uint32_t mySlot = (lane < (int)n) ? tileList[start + c + lane] : 0u;
-    for (uint32_t j = 0; j < n; j++) {
-      const uint32_t slot = __builtin_amdgcn_readlane(mySlot, j);
-      if (slot >= (uint32_t)P.maxSubtris) continue;
-      const uint4 h = hdr[slot];
-
-      // Hierarchical Z: conservative tile-level rejection (FineRaster.inl:229-240).
-      if (depthOn && P.hiz) {
-        if (zdirty) {
-          uint32_t m = dep[0];
+    // ---- lane j sets up triangle j of this chunk --------------------------
+    uint32_t tileZMax = 0xFFFFFFFFu;
+    if (hiz) {
+      uint32_t mx = dep[0];
 #pragma unroll
-          for (int i = 1; i < N; i++) m = max(m, dep[i]);
-          tileZMax = waveMax(m);
-          zdirty = false;
-        }
-        if ((h.w & 0xFFFFF000u) >= tileZMax) continue;
+      for (int i = 1; i < N; i++) mx = max(mx, dep[i]);
+      tileZMax = waveMax(mx);
+    }
+    const uint32_t n = min(64u, cnt - c);
+    bool keep = false;
+    uint32_t slot = 0;
+    uint4 h = make_uint4(0, 0, 0, 0), z = make_uint4(0, 0, 0, 0);
+    if (lane < (int)n) {
+      slot = tileList[start + c + lane];
+      if (slot < (uint32_t)P.maxSubtris) {
+        h = hdr[slot];
+        keep = !hiz || ((h.w & 0xFFFFF000u) < tileZMax);
+        if (keep && depthOn) z = tdata[(size_t)slot * 4];
       }
-
-      // Edge functions in tile-relative subpixels.  |a|,|d| <= 2^16; the
-      // constant term is clamped to +-2^30, the lane term |L| < 2^24 and the
-      // sample term |o| < 2^21, so C - L - o keeps the exact sign in 32 bits.
+    }
+    const uint64_t keepMask = __ballot(keep);
+    const uint32_t m = (uint32_t)__popcll(keepMask);
+    if (keep) {
+      uint32_t* r = rec + (uint32_t)__popcll(keepMask & laneMaskLt64()) * RS;
       int32_t ax[3], ay[3];
       ax[0] = (int32_t)(int16_t)(h.x & 0xFFFFu) - Tx; ay[0] = (int32_t)(int16_t)(h.x >> 16) - Ty;
       ax[1] = (int32_t)(int16_t)(h.y & 0xFFFFu) - Tx; ay[1] = (int32_t)(int16_t)(h.y >> 16) - Ty;
       ax[2] = (int32_t)(int16_t)(h.z & 0xFFFFu) - Tx; ay[2] = (int32_t)(int16_t)(h.z >> 16) - Ty;
-      int32_t T[3], dxk[3], dyk[3];
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         const int kn = (k + 1) % 3;
         const int32_t dx = ax[kn] - ax[k], dy = ay[kn] - ay[k];
+        // Edge function at the tile origin, a x d, minus 1 on exclusive
+        // edges (Util.inl:100).  |a|,|d| <= 2^16.  Clamping to +-2^30
+        // keeps the sign of a x d - s x d exact for every in-tile sample
+        // (|s x d| < 2^25), and every threshold below fits in 32 bits.
         int64_t cc = (int64_t)ax[k] * dy - (int64_t)ay[k] * dx;
-        if (dy > 0 || (dy == 0 && dx <= 0)) cc -= 1;      // exclusive edge (Util.inl:100)
+        if (dy > 0 || (dy == 0 && dx <= 0)) cc -= 1;
         cc = cc > (1ll << 30) ? (1ll << 30) : (cc < -(1ll << 30) ? -(1ll << 30) : cc);
-        T[k] = (int32_t)cc - (lx16 * dy - ly16 * dx);
-        dxk[k] = dx;
-        dyk[k] = dy;
+        const int32_t ck = (int32_t)cc;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          const int32_t ox = (2 * msaaPattern(S, i) + 1) << (3 - S);
+          const int32_t oy = (2 * i + 1) << (3 - S);
+          r[L::R + k * N + i] = (uint32_t)(ck - (ox * dy - oy * dx));
+        }
+        r[L::D + 2 * k] = (uint32_t)dx;
+        r[L::D + 2 * k + 1] = (uint32_t)dy;
       }
+      r[L::Z] = z.x;
+      r[L::Z + 1] = z.y;
+#pragma unroll
+      for (int i = 0; i < N; i++)
+        r[L::Z + 2 + i] = z.x * (tileSx + (uint32_t)msaaPattern(S, i)) + z.y * (tileSy + (uint32_t)i) + z.z;
+      r[L::SLOT] = slot;
+      r[L::TRI] = (slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris];
+      if (!VIS && gouraud) {
+        const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
+                    t3 = tdata[(size_t)slot * 4 + 3];
+        const float4 c0 = varying0(verts, t3.y, P.vertexStride), c1 = varying0(verts, t3.z, P.vertexStride),
+                     c2 = varying0(verts, t3.w, P.vertexStride);
+        uint32_t* q = r + L::SH;
+        q[0] = t1.x; q[1] = t1.y; q[2] = t1.z; q[3] = t1.w; q[4] = t2.x; q[5] = t2.y; q[6] = t2.z;
+        q[7] = t2.w; q[8] = t3.x;
+        q[9] = __float_as_uint(c0.x); q[10] = __float_as_uint(c0.y); q[11] = __float_as_uint(c0.z);
+        q[12] = __float_as_uint(c0.w); q[13] = __float_as_uint(c1.x); q[14] = __float_as_uint(c1.y);
+        q[15] = __float_as_uint(c1.z); q[16] = __float_as_uint(c1.w); q[17] = __float_as_uint(c2.x);
+        q[18] = __float_as_uint(c2.y); q[19] = __float_as_uint(c2.z); q[20] = __float_as_uint(c2.w);
+      }
+    }
+    __syncthreads();
+
+    // ---- walk the surviving triangles in submission order ----------------
+    for (uint32_t j = 0; j < m; j++) {
+      const uint32_t* r = rec + j * RS;
+      int32_t Lk[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) Lk[k] = lx16 * (int32_t)r[L::D + 2 * k + 1] - ly16 * (int32_t)r[L::D + 2 * k];
       uint32_t cover = 0;
 #pragma unroll
       for (int i = 0; i < N; i++) {
-        const int32_t ox = (2 * msaaPattern(S, i) + 1) << (3 - S);
-        const int32_t oy = (2 * i + 1) << (3 - S);
-        bool in = true;
-#pragma unroll
-        for (int k = 0; k < 3; k++) in = in && (T[k] >= ox * dyk[k] - oy * dxk[k]);
+        const bool in = (Lk[0] <= (int32_t)r[L::R + i]) & (Lk[1] <= (int32_t)r[L::R + N + i]) &
+                        (Lk[2] <= (int32_t)r[L::R + 2 * N + i]);
         cover |= in ? (1u << i) : 0u;
       }
       const uint64_t covBallot = __ballot(cover != 0);
       if (covBallot == 0) continue;
       frags += (uint64_t)__popcll(covBallot);
 
-      const uint4* td = tdata + (size_t)slot * 4;
       uint32_t write = cover;
       uint32_t dz[N];
       if (depthOn) {
-        const uint4 z = td[0];
-        const uint32_t pl = z.x * (uint32_t)(px * N) + z.y * (uint32_t)(py * N) + z.z;
+        const uint32_t pl = r[L::Z] * lxN + r[L::Z + 1] * lyN;
 #pragma unroll
         for (int i = 0; i < N; i++) {
-          dz[i] = pl + z.x * (uint32_t)msaaPattern(S, i) + z.y * (uint32_t)i;
+          dz[i] = r[L::Z + 2 + i] + pl;
           if (dz[i] >= dep[i]) write &= ~(1u << i);        // LESS (FineRaster.inl:697)
         }
       }
       if (__ballot(write != 0) == 0) continue;
 
-      const uint32_t src_tri = (slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris];
-      uint32_t src = P.constColor;
-      if (write != 0 && P.shader == kShaderGouraud) {
-        const uint4 t3 = td[3];
-        float b0, b1, b2;
-        if (lerpOn) {
-          // computeBarys at the centroid (FineRaster.inl:21-44, 93-114).
-          const uint4 t1 = td[1], t2 = td[2];
-          const int ctr = centroidSample<S>(cover);
-          const int cx = (ctr < 0) ? N : 2 * msaaPattern(S, ctr) + 1;
-          const int cy = (ctr < 0) ? N : 2 * ctr + 1;
-          const int32_t sx = px * (2 * N) + cx, sy = py * (2 * N) + cy;
-          const int32_t W = wadd(wadd(wmul((int32_t)t1.x, sx), wmul((int32_t)t1.y, sy)), (int32_t)t1.z);
-          const int32_t U = wadd(wadd(wmul((int32_t)t1.w, sx), wmul((int32_t)t2.x, sy)), (int32_t)t2.y);
-          const int32_t V = wadd(wadd(wmul((int32_t)t2.z, sx), wmul((int32_t)t2.w, sy)), (int32_t)t3.x);
-          const float w = 1.0f / (float)W;
-          const float u = w * (float)U;
-          const float v = w * (float)V;
-          b0 = 1.0f - u - v; b1 = u; b2 = v;
-        } else {
-          b0 = 0.0f; b1 = 0.0f; b2 = 1.0f;                  // FineRaster.inl:72-81
+      if (VIS) {
+        const uint32_t v = (r[L::SLOT] << 4) | (uint32_t)centroidCode<S>(cover);
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          if (write & (1u << i)) {
+            if (depthOn) dep[i] = dz[i];
+            aux[i] = v;
+          }
         }
-        const float4 c0 = *reinterpret_cast<const float4*>(verts + (size_t)t3.y * P.vertexStride + 16);
-        const float4 c1 = *reinterpret_cast<const float4*>(verts + (size_t)t3.z * P.vertexStride + 16);
-        const float4 c2 = *reinterpret_cast<const float4*>(verts + (size_t)t3.w * P.vertexStride + 16);
-        // interpolateVarying (PixelPipe.inl:52-59) + toABGR (PixelPipe.inl:68).
-        src = toABGR(fmaf(c2.x, b2, fmaf(c0.x, b0, c1.x * b1)), fmaf(c2.y, b2, fmaf(c0.y, b0, c1.y * b1)),
-                     fmaf(c2.z, b2, fmaf(c0.z, b0, c1.z * b1)), fmaf(c2.w, b2, fmaf(c0.w, b0, c1.w * b1)));
+      } else {
+        uint32_t src = P.constColor;
+        if (write != 0 && gouraud) {
+          const uint32_t* q = r + L::SH;
+          const uint4 t1 = make_uint4(q[0], q[1], q[2], q[3]);
+          const uint4 t2 = make_uint4(q[4], q[5], q[6], q[7]);
+          const uint4 t3 = make_uint4(q[8], 0, 0, 0);
+          const float4 c0 = make_float4(__uint_as_float(q[9]), __uint_as_float(q[10]), __uint_as_float(q[11]),
+                                        __uint_as_float(q[12]));
+          const float4 c1 = make_float4(__uint_as_float(q[13]), __uint_as_float(q[14]), __uint_as_float(q[15]),
+                                        __uint_as_float(q[16]));
+          const float4 c2 = make_float4(__uint_as_float(q[17]), __uint_as_float(q[18]), __uint_as_float(q[19]),
+                                        __uint_as_float(q[20]));
+          src = shadeGouraud<S>(lerpOn, centroidCode<S>(cover), px, py, t1, t2, t3, c0, c1, c2);
+        }
+        const uint32_t tri = r[L::TRI];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          if (write & (1u << i)) {
+            if (depthOn) dep[i] = dz[i];
+            col[i] = blendPixel(P.blend, src, col[i]);
+            aux[i] = tri;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- deferred shading: once per (pixel, distinct winning triangle) -------
+  uint32_t tid[N];
+  if (VIS) {
+    uint32_t pending = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) pending |= (aux[i] != 0xFFFFFFFFu) ? (1u << i) : 0u;
+    while (pending) {
+      const int first = __ffs(pending) - 1;
+      uint32_t v = aux[0];
+#pragma unroll
+      for (int i = 1; i < N; i++) v = (i == first) ? aux[i] : v;
+      const uint32_t slot = v >> 4;
+      const uint32_t tri = (slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris];
+      uint32_t src = P.constColor;
+      if (gouraud) {
+        const uint4* td = tdata + (size_t)slot * 4;
+        const uint4 t1 = td[1], t2 = td[2], t3 = td[3];
+        src = shadeGouraud<S>(lerpOn, (int)(v & 15u), px, py, t1, t2, t3, varying0(verts, t3.y, P.vertexStride),
+                              varying0(verts, t3.z, P.vertexStride), varying0(verts, t3.w, P.vertexStride));
       }
 #pragma unroll
       for (int i = 0; i < N; i++) {
-        if (write & (1u << i)) {
-          if (depthOn) dep[i] = dz[i];
-          col[i] = blendPixel(P.blend, src, col[i]);
-          tid[i] = src_tri;
+        if ((pending & (1u << i)) && aux[i] == v) {
+          if (P.blend != kBlendDepthOnly) col[i] = src;
+          tid[i] = tri;
+          pending &= ~(1u << i);
         }
       }
-      if (depthOn) zdirty = true;
     }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      if (aux[i] == 0xFFFFFFFFu)
+        tid[i] = P.deferredClear ? 0xFFFFFFFFu : (idOn ? triid[texel + i * 8] : 0xFFFFFFFFu);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; i++) tid[i] = aux[i];
   }
 
+  const bool storeColor = P.blend != kBlendDepthOnly || P.deferredClear;
 #pragma unroll
   for (int i = 0; i < N; i++) {
-    color[texel + i * 8] = col[i];
+    if (storeColor) color[texel + i * 8] = col[i];
     depth[texel + i * 8] = dep[i];
     if (idOn) triid[texel + i * 8] = tid[i];
   }
   if (lane == 0 && frags) atomicAdd((unsigned long long*)&P.counters->fragments, (unsigned long long)frags);
 }
 
-// Host-side launcher (one instantiation per sample count).
-void launchFine(const Params& P, hipStream_t stream) {
+template <int S, bool VIS>
+static void launchFineT(const Params& P, hipStream_t stream) {
   const int per = (P.numTiles + 7) >> 3;
-  dim3 grid(per * 8), block(64);
   const uint4* hdr = reinterpret_cast<const uint4*>(P.triHeader);
   const uint4* td = reinterpret_cast<const uint4*>(P.triData);
-  switch (P.samplesLog2) {
-    case 0: hipLaunchKernelGGL(fineKernel<0>, grid, block, 0, stream, P, hdr, td, P.tileList, P.vertices, P.subtriTri, P.color, P.depth, P.triid); break;
-    case 1: hipLaunchKernelGGL(fineKernel<1>, grid, block, 0, stream, P, hdr, td, P.tileList, P.vertices, P.subtriTri, P.color, P.depth, P.triid); break;
-    case 2: hipLaunchKernelGGL(fineKernel<2>, grid, block, 0, stream, P, hdr, td, P.tileList, P.vertices, P.subtriTri, P.color, P.depth, P.triid); break;
-    default: hipLaunchKernelGGL(fineKernel<3>, grid, block, 0, stream, P, hdr, td, P.tileList, P.vertices, P.subtriTri, P.color, P.depth, P.triid); break;
+  hipLaunchKernelGGL((fineKernel<S, VIS>), dim3(per * 8), dim3(64), 0, stream, P, hdr, td, P.tileList, P.vertices,
+                     P.subtriTri, P.color, P.depth, P.triid);
+}
+
+// Host-side launcher.  Deferred shading needs the visibility word
+// (slot << 4 | centroid) to hold every slot: slots < 2^28.
+void launchFine(const Params& P, hipStream_t stream) {
+  const bool vis = (P.blend == kBlendReplace || P.blend == kBlendDepthOnly) && P.maxSubtris < (1 << 28);
+  switch (P.samplesLog2 * 2 + (vis ? 1 : 0)) {
+    case 0: launchFineT<0, false>(P, stream); break;
+    case 1: launchFineT<0, true>(P, stream); break;
+    case 2: launchFineT<1, false>(P, stream); break;
+    case 3: launchFineT<1, true>(P, stream); break;
+    case 4: launchFineT<2, false>(P, stream); break;
+    case 5: launchFineT<2, true>(P, stream); break;
+    case 6: launchFineT<3, false>(P, stream); break;
+    default: launchFineT<3, true>(P, stream); break;
   }
 }
 
