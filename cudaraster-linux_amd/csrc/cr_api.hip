@@ -79,7 +79,7 @@ struct cr_context {
   int64_t maxSubtris = 1, maxBinEntries = 1, maxTileEntries = 1;
   DevBuf triSubtris, triHeader, triData, subtriTri;
   DevBuf binChunkCount, binTotal, binStart, binList;
-  DevBuf tileStart, tileCount, tileList;
+  DevBuf tileStart, tileCount, tileList, tileFrags;
   DevBuf counters;
   Counters* hostCounters = nullptr;   // pinned
   hipEvent_t ev[5] = {};
@@ -179,6 +179,7 @@ cr_status ensureBuffers(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   CR_HIP(ctx, ctx->tileStart.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileCount.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileList.ensure((size_t)ctx->maxTileEntries * 4));
+  if (ctx->dbg.count_fragments) CR_HIP(ctx, ctx->tileFrags.ensure((size_t)g.nt * 4));
   return CR_OK;
 }
 
@@ -209,6 +210,7 @@ Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   P.clearColor = ctx->clearColor;
   P.clearDepth = ctx->clearDepth;
   P.hiz = (ctx->dbg.disable_hiz || ctx->dbg.count_fragments) ? 0 : 1;
+  P.tileFrags = ctx->dbg.count_fragments ? ctx->tileFrags.as<uint32_t>() : nullptr;
   P.maxSubtris = (int32_t)ctx->maxSubtris;
   P.triSubtris = ctx->triSubtris.as<uint8_t>();
   P.triHeader = ctx->triHeader.as<Header>();
@@ -242,6 +244,7 @@ cr_status launchAll(cr_context* ctx, const Params& P, hipStream_t stream) {
   CR_HIP(ctx, hipEventRecord(ctx->ev[3], stream));
   launchFine(P, stream);
   CR_HIP(ctx, hipEventRecord(ctx->ev[4], stream));
+  launchFragmentSum(P, stream);
   CR_HIP(ctx, hipGetLastError());
   CR_HIP(ctx, hipMemcpyAsync(ctx->hostCounters, ctx->counters.ptr, sizeof(Counters), hipMemcpyDeviceToHost, stream));
   ctx->evRecorded = true;
@@ -349,7 +352,7 @@ cr_status cr_context_destroy(cr_context* ctx) {
   if (ctx->pending) (void)hipStreamSynchronize(ctx->pendingStream);
   for (DevBuf* b : {&ctx->triSubtris, &ctx->triHeader, &ctx->triData, &ctx->subtriTri, &ctx->binChunkCount,
                     &ctx->binTotal, &ctx->binStart, &ctx->binList, &ctx->tileStart, &ctx->tileCount,
-                    &ctx->tileList, &ctx->counters})
+                    &ctx->tileList, &ctx->tileFrags, &ctx->counters})
     b->release();
   if (ctx->hostCounters) (void)hipHostFree(ctx->hostCounters);
   for (int i = 0; i < 5; i++)
@@ -613,7 +616,7 @@ cr_status cr_get_profiling_info(cr_context* ctx, char* buf, size_t len) {
            "Tile entries    %lld\nFragments       %lld%s\nRetries         %lld\n",
            s.setup_time * 1e3, s.bin_time * 1e3, s.coarse_time * 1e3, s.fine_time * 1e3, total * 1e3,
            (long long)c.num_tris, (long long)c.num_subtris, (long long)c.bin_entries, (long long)c.tile_entries,
-           (long long)c.fragments, (ctx->dbg.disable_hiz || ctx->dbg.count_fragments) ? "" : " (after early-Z)",
+           (long long)c.fragments, ctx->dbg.count_fragments ? "" : " (not counted: set count_fragments)",
            (long long)c.retries);
   return CR_OK;
 }

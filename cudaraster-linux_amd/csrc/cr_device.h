@@ -82,6 +82,7 @@ struct Params {
   int32_t deferredClear;
   uint32_t clearColor, clearDepth;
   int32_t hiz;                     // 1: tile-level early depth rejection
+  uint32_t* tileFrags;             // per-tile fragment counts (counting draws only), else null
   // Setup output.
   int32_t maxSubtris;
   uint8_t* triSubtris;
@@ -208,6 +209,7 @@ void launchReset(const Params& P, hipStream_t stream);
 void launchBin(const Params& P, uint32_t* binStart, hipStream_t stream);
 void launchCoarse(const Params& P, const uint32_t* binStart, hipStream_t stream);
 void launchFine(const Params& P, hipStream_t stream);
+void launchFragmentSum(const Params& P, hipStream_t stream);
 constexpr int kBinChunkTris = 2048;
 
 }  // namespace crhip

@@ -50,11 +50,30 @@ __device__ __forceinline__ uint64_t laneMaskLt64() {
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Sample order by distance to the pixel centre, lowest index first on ties:
+// the centroid of a coverage mask (selectMSAACentroid, Util.hpp:182-203) is
+// its first covered sample in this order, or the pixel centre (code 15) when
+// none or all samples are covered.  The fine stage evaluates it as a
+// compile-time priority chain instead of the reference's search loop.
 template <int S>
-__device__ __forceinline__ int centroidCode(uint32_t cover) {
-  const int c = selectCentroid(S, cover);
-  return c < 0 ? 15 : c;
-}
+struct CentroidOrder {
+  int order[8];
+  constexpr CentroidOrder() : order() {
+    const int n = 1 << S;
+    bool used[8] = {false, false, false, false, false, false, false, false};
+    for (int k = 0; k < n; k++) {
+      int best = -1, bestDist = 1 << 30;
+      for (int i = 0; i < n; i++) {
+        if (used[i]) continue;
+        const int dx = msaaPattern(S, i) * 2 + 1 - n, dy = i * 2 + 1 - n;
+        const int d = dx * dx + dy * dy;
+        if (d < bestDist) { best = i; bestDist = d; }
+      }
+      used[best] = true;
+      order[k] = best;
+    }
+  }
+};
 
 // Perspective-correct barycentrics at the centroid (computeBarys,
 // FineRaster.inl:21-44, 93-114) and GouraudShader + toABGR
@@ -87,30 +106,51 @@ __device__ __forceinline__ float4 varying0(const uint8_t* verts, uint32_t vi, in
   return *reinterpret_cast<const float4*>(verts + (size_t)vi * stride + 16);
 }
 
+constexpr int kFineWaves = 4;     // tiles (waves) per workgroup
+constexpr int kChunk = 32;        // triangles set up per chunk
+
+// LDS ordering inside one wave: every lane's record stores must land before
+// any lane reads them; a wavefront-scope fence plus the wave barrier orders
+// the LDS operations of the wave (no cross-wave synchronisation is needed:
+// each wave owns its tile and its LDS slice).
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int S, bool VIS>
-__global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __restrict__ hdr,
-                                                 const uint4* __restrict__ tdata,
-                                                 const uint32_t* __restrict__ tileList,
-                                                 const uint8_t* __restrict__ verts,
-                                                 const uint32_t* __restrict__ subtriTri,
-                                                 uint32_t* __restrict__ color, uint32_t* __restrict__ depth,
-                                                 uint32_t* __restrict__ triid) {
+__global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P, const uint4* __restrict__ hdr,
+                                                              const uint4* __restrict__ tdata,
+                                                              const uint32_t* __restrict__ tileList,
+                                                              const uint8_t* __restrict__ verts,
+                                                              const uint32_t* __restrict__ subtriTri,
+                                                              uint32_t* __restrict__ color,
+                                                              uint32_t* __restrict__ depth,
+                                                              uint32_t* __restrict__ triid) {
   constexpr int N = 1 << S;
   using L = Rec<S>;
   constexpr int RS = VIS ? L::SIZE_VIS : L::SIZE_FULL;
-  __shared__ __attribute__((aligned(16))) uint32_t rec[64 * RS];
+  __shared__ __attribute__((aligned(16))) uint32_t recAll[kFineWaves * kChunk * RS];
 
   if (P.counters->overflow) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  uint32_t* rec = recAll + wave * (kChunk * RS);
   // XCD-aware order: workgroups go round-robin to the 8 XCDs; each XCD gets
   // one contiguous band of tiles so triangles shared by neighbouring tiles
   // stay in one L2.
-  const int per = (P.numTiles + 7) >> 3;
-  const int tile = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-  if (tile >= P.numTiles) return;
+  const int groups = (P.numTiles + kFineWaves - 1) / kFineWaves;
+  const int per = (groups + 7) >> 3;
+  const int group = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  const int tile = group * kFineWaves + wave;
+  if (group >= groups || tile >= P.numTiles) return;
   const uint32_t cnt = P.tileCount[tile];
-  if (cnt == 0 && !P.deferredClear) return;
+  if (cnt == 0 && !P.deferredClear) {
+    if (P.tileFrags != nullptr && lane == 0) P.tileFrags[tile] = 0;
+    return;
+  }
 
-  const int lane = threadIdx.x;
   const int lx = lane & 7, ly = lane >> 3;
   const int tx = tile % P.widthTiles, ty = tile / P.widthTiles;
   const int px = (tx << kTileLog2) + lx, py = (ty << kTileLog2) + ly;
@@ -140,13 +180,14 @@ __global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __
   const int32_t lx16 = lx << kSubpixelLog2, ly16 = ly << kSubpixelLog2;
   const uint32_t lxN = (uint32_t)(lx * N), lyN = (uint32_t)(ly * N);
   const uint32_t tileSx = (uint32_t)(tx * 8 * N), tileSy = (uint32_t)(ty * 8 * N);
+  constexpr CentroidOrder<S> co;
 
   uint64_t frags = 0;
   const uint32_t start = P.tileStart[tile];
   const bool hiz = depthOn && P.hiz;
 
-  for (uint32_t c = 0; c < cnt; c += 64) {
-    // ---- lane j sets up triangle j of this chunk --------------------------
+  for (uint32_t c = 0; c < cnt; c += kChunk) {
+    // ---- lane j < kChunk sets up triangle j of this chunk ------------------
     uint32_t tileZMax = 0xFFFFFFFFu;
     if (hiz) {
       uint32_t mx = dep[0];
@@ -154,7 +195,7 @@ __global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __
       for (int i = 1; i < N; i++) mx = max(mx, dep[i]);
       tileZMax = waveMax(mx);
     }
-    const uint32_t n = min(64u, cnt - c);
+    const uint32_t n = min((uint32_t)kChunk, cnt - c);
     bool keep = false;
     uint32_t slot = 0;
     uint4 h = make_uint4(0, 0, 0, 0), z = make_uint4(0, 0, 0, 0);
@@ -162,8 +203,8 @@ __global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __
       slot = tileList[start + c + lane];
       if (slot < (uint32_t)P.maxSubtris) {
         h = hdr[slot];
+        if (depthOn) z = tdata[(size_t)slot * 4];
         keep = !hiz || ((h.w & 0xFFFFF000u) < tileZMax);
-        if (keep && depthOn) z = tdata[(size_t)slot * 4];
       }
     }
     const uint64_t keepMask = __ballot(keep);
@@ -201,7 +242,7 @@ __global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __
       for (int i = 0; i < N; i++)
         r[L::Z + 2 + i] = z.x * (tileSx + (uint32_t)msaaPattern(S, i)) + z.y * (tileSy + (uint32_t)i) + z.z;
       r[L::SLOT] = slot;
-      r[L::TRI] = (slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris];
+      if (!VIS) r[L::TRI] = (slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris];
       if (!VIS && gouraud) {
         const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
                     t3 = tdata[(size_t)slot * 4 + 3];
@@ -216,49 +257,66 @@ __global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __
         q[18] = __float_as_uint(c2.y); q[19] = __float_as_uint(c2.z); q[20] = __float_as_uint(c2.w);
       }
     }
-    __syncthreads();
+    waveSync();
 
     // ---- walk the surviving triangles in submission order ----------------
     for (uint32_t j = 0; j < m; j++) {
       const uint32_t* r = rec + j * RS;
+      const uint32_t* cur = r;
+      // |lx16|,|ly16| < 2^7 and |d| < 2^17: 24-bit multiplies are exact.
       int32_t Lk[3];
 #pragma unroll
-      for (int k = 0; k < 3; k++) Lk[k] = lx16 * (int32_t)r[L::D + 2 * k + 1] - ly16 * (int32_t)r[L::D + 2 * k];
-      uint32_t cover = 0;
+      for (int k = 0; k < 3; k++)
+        Lk[k] = __mul24(lx16, (int32_t)cur[L::D + 2 * k + 1]) - __mul24(ly16, (int32_t)cur[L::D + 2 * k]);
+      bool cov[N];
+      bool any = false;
 #pragma unroll
       for (int i = 0; i < N; i++) {
-        const bool in = (Lk[0] <= (int32_t)r[L::R + i]) & (Lk[1] <= (int32_t)r[L::R + N + i]) &
-                        (Lk[2] <= (int32_t)r[L::R + 2 * N + i]);
-        cover |= in ? (1u << i) : 0u;
+        cov[i] = (Lk[0] <= (int32_t)cur[L::R + i]) & (Lk[1] <= (int32_t)cur[L::R + N + i]) &
+                 (Lk[2] <= (int32_t)cur[L::R + 2 * N + i]);
+        any = any | cov[i];
       }
-      const uint64_t covBallot = __ballot(cover != 0);
+      const uint64_t covBallot = __ballot(any);
       if (covBallot == 0) continue;
       frags += (uint64_t)__popcll(covBallot);
 
-      uint32_t write = cover;
+      bool wr[N];
       uint32_t dz[N];
+      bool anyWr = false;
       if (depthOn) {
         const uint32_t pl = r[L::Z] * lxN + r[L::Z + 1] * lyN;
 #pragma unroll
         for (int i = 0; i < N; i++) {
           dz[i] = r[L::Z + 2 + i] + pl;
-          if (dz[i] >= dep[i]) write &= ~(1u << i);        // LESS (FineRaster.inl:697)
+          wr[i] = cov[i] & (dz[i] < dep[i]);               // LESS (FineRaster.inl:697)
+          anyWr = anyWr | wr[i];
         }
+        if (__ballot(anyWr) == 0) continue;
+      } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) { wr[i] = cov[i]; dz[i] = 0; }
+        anyWr = any;
       }
-      if (__ballot(write != 0) == 0) continue;
+      // Centroid (selectMSAACentroid): first covered sample in distance
+      // order, the pixel centre (15) when none or all are covered.
+      int code = 15;
+      bool all = true;
+#pragma unroll
+      for (int k = N - 1; k >= 0; k--) code = cov[co.order[k]] ? co.order[k] : code;
+#pragma unroll
+      for (int i = 0; i < N; i++) all = all & cov[i];
+      if (all) code = 15;
 
       if (VIS) {
-        const uint32_t v = (r[L::SLOT] << 4) | (uint32_t)centroidCode<S>(cover);
+        const uint32_t v = (r[L::SLOT] << 4) | (uint32_t)code;
 #pragma unroll
         for (int i = 0; i < N; i++) {
-          if (write & (1u << i)) {
-            if (depthOn) dep[i] = dz[i];
-            aux[i] = v;
-          }
+          if (depthOn) dep[i] = wr[i] ? dz[i] : dep[i];
+          aux[i] = wr[i] ? v : aux[i];
         }
       } else {
         uint32_t src = P.constColor;
-        if (write != 0 && gouraud) {
+        if (anyWr && gouraud) {
           const uint32_t* q = r + L::SH;
           const uint4 t1 = make_uint4(q[0], q[1], q[2], q[3]);
           const uint4 t2 = make_uint4(q[4], q[5], q[6], q[7]);
@@ -269,12 +327,12 @@ __global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __
                                         __uint_as_float(q[16]));
           const float4 c2 = make_float4(__uint_as_float(q[17]), __uint_as_float(q[18]), __uint_as_float(q[19]),
                                         __uint_as_float(q[20]));
-          src = shadeGouraud<S>(lerpOn, centroidCode<S>(cover), px, py, t1, t2, t3, c0, c1, c2);
+          src = shadeGouraud<S>(lerpOn, code, px, py, t1, t2, t3, c0, c1, c2);
         }
         const uint32_t tri = r[L::TRI];
 #pragma unroll
         for (int i = 0; i < N; i++) {
-          if (write & (1u << i)) {
+          if (wr[i]) {
             if (depthOn) dep[i] = dz[i];
             col[i] = blendPixel(P.blend, src, col[i]);
             aux[i] = tri;
@@ -282,7 +340,7 @@ __global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __
         }
       }
     }
-    __syncthreads();
+    waveSync();
   }
 
   // ---- deferred shading: once per (pixel, distinct winning triangle) -------
@@ -331,16 +389,19 @@ __global__ void __launch_bounds__(64) fineKernel(const Params P, const uint4* __
     depth[texel + i * 8] = dep[i];
     if (idOn) triid[texel + i * 8] = tid[i];
   }
-  if (lane == 0 && frags) atomicAdd((unsigned long long*)&P.counters->fragments, (unsigned long long)frags);
+  // Fragment statistics only on counting draws: one plain store per tile
+  // (a device-wide atomic per tile serialises across the XCDs).
+  if (P.tileFrags != nullptr && lane == 0) P.tileFrags[tile] = (uint32_t)frags;
 }
 
 template <int S, bool VIS>
 static void launchFineT(const Params& P, hipStream_t stream) {
-  const int per = (P.numTiles + 7) >> 3;
+  const int groups = (P.numTiles + kFineWaves - 1) / kFineWaves;
+  const int per = (groups + 7) >> 3;
   const uint4* hdr = reinterpret_cast<const uint4*>(P.triHeader);
   const uint4* td = reinterpret_cast<const uint4*>(P.triData);
-  hipLaunchKernelGGL((fineKernel<S, VIS>), dim3(per * 8), dim3(64), 0, stream, P, hdr, td, P.tileList, P.vertices,
-                     P.subtriTri, P.color, P.depth, P.triid);
+  hipLaunchKernelGGL((fineKernel<S, VIS>), dim3(per * 8), dim3(64 * kFineWaves), 0, stream, P, hdr, td, P.tileList,
+                     P.vertices, P.subtriTri, P.color, P.depth, P.triid);
 }
 
 // Host-side launcher.  Deferred shading needs the visibility word
@@ -357,6 +418,27 @@ void launchFine(const Params& P, hipStream_t stream) {
     case 6: launchFineT<3, false>(P, stream); break;
     default: launchFineT<3, true>(P, stream); break;
   }
+}
+
+__global__ void __launch_bounds__(1024) fragmentSumKernel(const uint32_t* tileFrags, int numTiles, Counters* c) {
+  __shared__ unsigned long long part[16];
+  unsigned long long acc = 0;
+  for (int i = threadIdx.x; i < numTiles; i += 1024) acc += tileFrags[i];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < 16; w++) t += part[w];
+    c->fragments = t;
+  }
+}
+
+void launchFragmentSum(const Params& P, hipStream_t stream) {
+  if (P.tileFrags == nullptr) return;
+  hipLaunchKernelGGL(fragmentSumKernel, dim3(1), dim3(1024), 0, stream, (const uint32_t*)P.tileFrags, P.numTiles,
+                     P.counters);
 }
 
 }  // namespace crhip

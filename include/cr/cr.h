@@ -108,7 +108,7 @@ typedef struct cr_counters {
     int64_t bin_entries;
     int64_t tile_entries;
     int64_t fragments;           /* (subtriangle, pixel) pairs with >= 1 covered sample;
-                                    exact only with hierarchical Z off */
+                                    counted only by draws with count_fragments set, else 0 */
     int64_t retries;             /* capacity grow-and-retry rounds */
 } cr_counters;
 
@@ -117,7 +117,7 @@ typedef struct cr_counters {
  * the knobs that change execution but never the image. */
 typedef struct cr_debug_params {
     int32_t disable_hiz;         /* 1: no tile-level early depth rejection */
-    int32_t count_fragments;     /* 1: exact fragment count (implies disable_hiz) */
+    int32_t count_fragments;     /* 1: count fragments exactly (implies disable_hiz; slower) */
 } cr_debug_params;
 
 /* ---- context (CudaRaster ctor + init, CudaRaster.cpp:98-128) ---- */
