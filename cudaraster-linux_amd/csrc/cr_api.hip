@@ -78,7 +78,7 @@ struct cr_context {
 
   int64_t maxSubtris = 1, maxBinEntries = 1, maxTileEntries = 1;
   DevBuf triSubtris, triHeader, triData, subtriTri;
-  DevBuf binChunkCount, binTotal, binStart, binList;
+  DevBuf chunkBase, binCnt, binOff, binList;
   DevBuf tileStart, tileCount, tileList, tileFrags;
   DevBuf counters;
   Counters* hostCounters = nullptr;   // pinned
@@ -172,9 +172,9 @@ cr_status ensureBuffers(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   CR_HIP(ctx, ctx->triHeader.ensure((size_t)ctx->maxSubtris * sizeof(Header)));
   CR_HIP(ctx, ctx->triData.ensure((size_t)ctx->maxSubtris * sizeof(TriData)));
   CR_HIP(ctx, ctx->subtriTri.ensure((size_t)std::max<int64_t>(ctx->maxSubtris - nt, 1) * 4));
-  CR_HIP(ctx, ctx->binChunkCount.ensure((size_t)g.nb * std::max(numChunks, 1) * 4));
-  CR_HIP(ctx, ctx->binTotal.ensure((size_t)kMaxBins * 4));
-  CR_HIP(ctx, ctx->binStart.ensure((size_t)kMaxBins * 4));
+  CR_HIP(ctx, ctx->chunkBase.ensure((size_t)std::max(numChunks, 1) * 4));
+  CR_HIP(ctx, ctx->binCnt.ensure((size_t)g.nb * std::max(numChunks, 1) * 4));
+  CR_HIP(ctx, ctx->binOff.ensure((size_t)g.nb * std::max(numChunks, 1) * 4));
   CR_HIP(ctx, ctx->binList.ensure((size_t)ctx->maxBinEntries * 4));
   CR_HIP(ctx, ctx->tileStart.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileCount.ensure((size_t)g.nt * 4));
@@ -216,10 +216,10 @@ Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   P.triHeader = ctx->triHeader.as<Header>();
   P.triData = ctx->triData.as<TriData>();
   P.subtriTri = ctx->subtriTri.as<uint32_t>();
-  P.binChunk = kBinChunkTris;
   P.numChunks = numChunks;
-  P.binChunkCount = ctx->binChunkCount.as<uint32_t>();
-  P.binTotal = ctx->binTotal.as<uint32_t>();
+  P.chunkBase = ctx->chunkBase.as<uint32_t>();
+  P.binCnt = ctx->binCnt.as<uint32_t>();
+  P.binOff = ctx->binOff.as<uint32_t>();
   P.maxBinEntries = (int32_t)ctx->maxBinEntries;
   P.binList = ctx->binList.as<uint32_t>();
   P.tileStart = ctx->tileStart.as<uint32_t>();
@@ -236,11 +236,10 @@ Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks) {
 cr_status launchAll(cr_context* ctx, const Params& P, hipStream_t stream) {
   CR_HIP(ctx, hipEventRecord(ctx->ev[0], stream));
   launchReset(P, stream);
-  launchSetup(P, stream);
+  launchSetup(P, stream);                       // setup and binning are one kernel
   CR_HIP(ctx, hipEventRecord(ctx->ev[1], stream));
-  launchBin(P, ctx->binStart.as<uint32_t>(), stream);
   CR_HIP(ctx, hipEventRecord(ctx->ev[2], stream));
-  launchCoarse(P, ctx->binStart.as<uint32_t>(), stream);
+  launchCoarse(P, stream);
   CR_HIP(ctx, hipEventRecord(ctx->ev[3], stream));
   launchFine(P, stream);
   CR_HIP(ctx, hipEventRecord(ctx->ev[4], stream));
@@ -291,7 +290,7 @@ cr_status drawImpl(cr_context* ctx, hipStream_t stream, bool sync) {
     if (s2 != CR_OK && s2 != CR_ERR_OVERFLOW) return s2;
   }
   Geometry g = geometryOf(ctx->color);
-  int32_t numChunks = (ctx->numTris + kBinChunkTris - 1) / kBinChunkTris;
+  int32_t numChunks = (ctx->numTris + kSetupChunkTris - 1) / kSetupChunkTris;
   initialCapacities(ctx, g);
   ctx->last.retries = 0;
   for (;;) {
@@ -350,8 +349,8 @@ cr_status cr_context_destroy(cr_context* ctx) {
   if (!ctx) return CR_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->pending) (void)hipStreamSynchronize(ctx->pendingStream);
-  for (DevBuf* b : {&ctx->triSubtris, &ctx->triHeader, &ctx->triData, &ctx->subtriTri, &ctx->binChunkCount,
-                    &ctx->binTotal, &ctx->binStart, &ctx->binList, &ctx->tileStart, &ctx->tileCount,
+  for (DevBuf* b : {&ctx->triSubtris, &ctx->triHeader, &ctx->triData, &ctx->subtriTri, &ctx->chunkBase,
+                    &ctx->binCnt, &ctx->binOff, &ctx->binList, &ctx->tileStart, &ctx->tileCount,
                     &ctx->tileList, &ctx->tileFrags, &ctx->counters})
     b->release();
   if (ctx->hostCounters) (void)hipHostFree(ctx->hostCounters);

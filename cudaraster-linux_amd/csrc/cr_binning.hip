@@ -1,74 +1,68 @@
-// Stages 2 and 3: order-preserving bin and tile queues.
+// Stage 3: coarse raster -- per-bin, order-preserving tile queues.
 //
-// The reference streams triangles through 16 persistent bin blocks into
-// linked 512-entry segments per (bin, stream) and merges the 16 streams per
-// bin in the coarse stage (cudaraster/cuda/BinRaster.inl:19-469,
-// CoarseRaster.inl:28-809).  Here both levels are counting sorts that keep
-// submission order by construction and produce one contiguous list per bin
-// and per tile:
+// The reference merges 16 bin streams by lowest triangle index and emits
+// 32-entry linked tile segments (cudaraster/cuda/CoarseRaster.inl:28-809).
+// Here one block of 16 waves owns a bin and builds one contiguous list per
+// tile in submission order, in two passes over the bin's entries:
 //
-//   binCount    chunk of kBinChunk triangles per block: per-bin emit counts
-//   binScan     per bin, exclusive scan of the chunk counts; the last block
-//               to finish also computes the bin starts (prefix over bins)
-//   binScatter  chunk per block: subtriangles of each 256-triangle round are
-//               compacted in LDS (wave prefix sums), then every bin's entries
-//               are ranked with per-wave 64-bit emit masks + popcounts and
-//               written in (triangle, subtriangle) order
-//   coarse      one block (16 waves) per bin: count per tile, allocate the
-//               bin's tile lists with one atomic, then emit in rounds of 1024
-//               entries with the same emit-mask ranking
+//   pass 0  count the entries of every tile of the bin;
+//   (alloc) exclusive scan over the 256 tiles, one atomic reserves the bin's
+//           share of the tile queue;
+//   pass 1  emit: per-wave 64-bit emit masks per tile, prefix over waves,
+//           rank = popcount of the lower lanes -- entries land in order.
 //
-// Binning is conservative (bounding boxes, plus a tile-level edge test in
-// the coarse stage); the fine stage computes exact coverage, so neither
-// level can change the image.
+// The bin's entries are the concatenation, in chunk order, of the segments
+// the fused setup/bin stage wrote per chunk of triangles; a segment table is
+// scanned into LDS and every thread finds its entry by binary search.
+//
+// Tile coverage is conservative: bounding box clipped to the bin and the
+// surface, then an exact-sign edge test of each tile's closed square.  Per
+// wave, the test runs per lane when every triangle's box is at most 2x2
+// tiles or the boxes are scattered, and cooperatively (one ballot per tile
+// of the wave's union box) when the boxes overlap -- the large-triangle
+// case.  Binning never changes the image; the fine stage is exact.
 #include "cr_device.h"
 
 namespace crhip {
 
-constexpr int kBinChunk = kBinChunkTris;   // triangles per bin chunk
-constexpr int kBinThreads = 256;
 constexpr int kCoarseThreads = 1024;
 constexpr int kCoarseWaves = kCoarseThreads / 64;
+constexpr int kSegBatch = 2048;        // chunks per segment-table batch
 
-// Slot of subtriangle s of triangle t (BinRaster.inl:148-157 encodes the
-// same choice as t*8+s).
-__device__ __forceinline__ uint32_t subtriSlot(const Params& P, uint32_t t, int k, int s) {
-  return (k == 1) ? t : P.triHeader[t].misc + (uint32_t)s;
-}
-
-struct Box { int32_t x0, y0, x1, y1; };   // inclusive, empty when x0 > x1
-
-// Absolute subpixel bounding box of a header triangle.
-__device__ __forceinline__ void headerBounds(const Params& P, const Header& h, int32_t& minx, int32_t& miny,
-                                             int32_t& maxx, int32_t& maxy) {
-  int32_t ox = P.viewportWidth << (kSubpixelLog2 - 1), oy = P.viewportHeight << (kSubpixelLog2 - 1);
-  minx = min(min((int32_t)h.v0x, (int32_t)h.v1x), (int32_t)h.v2x) + ox;
-  maxx = max(max((int32_t)h.v0x, (int32_t)h.v1x), (int32_t)h.v2x) + ox;
-  miny = min(min((int32_t)h.v0y, (int32_t)h.v1y), (int32_t)h.v2y) + oy;
-  maxy = max(max((int32_t)h.v0y, (int32_t)h.v1y), (int32_t)h.v2y) + oy;
-}
-
-// Bins (128x128 px) touched by the triangle's bounding box.
-__device__ __forceinline__ Box binBox(const Params& P, const Header& h) {
-  int32_t minx, miny, maxx, maxy;
-  headerBounds(P, h, minx, miny, maxx, maxy);
-  Box b;
-  const int sh = kSubpixelLog2 + kTileLog2 + kBinLog2;
-  if (maxx < 0 || maxy < 0 || minx >= (P.widthPixels << kSubpixelLog2) || miny >= (P.heightPixels << kSubpixelLog2)) {
-    b.x0 = 1; b.x1 = 0; b.y0 = 1; b.y1 = 0;
-    return b;
-  }
-  b.x0 = max(minx >> sh, 0); b.x1 = min(maxx >> sh, P.widthBins - 1);
-  b.y0 = max(miny >> sh, 0); b.y1 = min(maxy >> sh, P.heightBins - 1);
-  return b;
-}
-
-__device__ __forceinline__ uint64_t laneMaskLt() {
-  uint32_t lane = threadIdx.x & 63;
+__device__ __forceinline__ uint64_t lanesBelow() {
+  const uint32_t lane = threadIdx.x & 63;
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// ---------------------------------------------------------------------------
+template <int NT>
+__device__ __forceinline__ uint32_t blockScanC(uint32_t v, uint32_t* sums, uint32_t& total) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sums[wave] = x;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const uint32_t w = (threadIdx.x < NW) ? sums[threadIdx.x] : 0u;
+    uint32_t sc = w;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(sc, o, 64);
+      if ((int)threadIdx.x >= o) sc += y;
+    }
+    if (threadIdx.x < NW) sums[threadIdx.x] = sc - w;
+    if (threadIdx.x == NW - 1) sums[NW] = sc;
+  }
+  __syncthreads();
+  total = sums[NW];
+  const uint32_t r = sums[wave] + x - v;
+  __syncthreads();
+  return r;
+}
 
 __global__ void __launch_bounds__(64) resetCountersKernel(Counters* c, uint32_t numTris) {
   uint32_t* w = reinterpret_cast<uint32_t*>(c);
@@ -77,213 +71,104 @@ __global__ void __launch_bounds__(64) resetCountersKernel(Counters* c, uint32_t 
   if (threadIdx.x == 0) c->numSubtris = numTris;
 }
 
-__global__ void __launch_bounds__(kBinThreads) binCountKernel(const Params P) {
-  __shared__ uint32_t cnt[kMaxBins];
-  if (P.counters->overflow & 1u) return;   // clipped slots may lie past the buffer
-  for (int i = threadIdx.x; i < P.numBins; i += kBinThreads) cnt[i] = 0;
-  __syncthreads();
-  int chunk = blockIdx.x;
-  for (int r = 0; r < kBinChunk; r += kBinThreads) {
-    int t = chunk * kBinChunk + r + threadIdx.x;
-    if (t >= P.numTris) break;
-    int k = P.triSubtris[t];
-    for (int s = 0; s < k; s++) {
-      Header h = P.triHeader[subtriSlot(P, t, k, s)];
-      Box b = binBox(P, h);
-      for (int y = b.y0; y <= b.y1; y++)
-        for (int x = b.x0; x <= b.x1; x++) atomicAdd(&cnt[x + y * P.widthBins], 1u);
-    }
+// Per-lane tile test data of one triangle inside bin (bx, by).
+struct TileTest {
+  int32_t x0, y0, x1, y1;   // bin-local tile box (inclusive); empty when x0 > x1
+  int32_t G[3];             // clamped edge thresholds at the box origin
+  int32_t sx[3], sy[3];     // threshold change per tile step in x / y
+  __device__ __forceinline__ bool hit(int x, int y) const {
+    const int32_t rx = x - x0, ry = y - y0;
+    return (rx * sx[0] + ry * sy[0] <= G[0]) & (rx * sx[1] + ry * sy[1] <= G[1]) &
+           (rx * sx[2] + ry * sy[2] <= G[2]);
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < P.numBins; i += kBinThreads)
-    P.binChunkCount[(size_t)i * P.numChunks + chunk] = cnt[i];
-}
-
-// Block-wide exclusive scan of one value per thread (1024 threads).
-template <int NT>
-__device__ __forceinline__ uint32_t blockExclusiveScan(uint32_t v, uint32_t* warpSums, uint32_t& total) {
-  constexpr int NW = NT / 64;
-  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) warpSums[wave] = x;
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    uint32_t w = (threadIdx.x < NW) ? warpSums[threadIdx.x] : 0u;
-    uint32_t s = w;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      uint32_t y = __shfl_up(s, o, 64);
-      if ((int)threadIdx.x >= o) s += y;
-    }
-    if (threadIdx.x < NW) warpSums[threadIdx.x] = s - w;
-    if (threadIdx.x == NW - 1) warpSums[NW] = s;
-  }
-  __syncthreads();
-  total = warpSums[NW];
-  uint32_t r = warpSums[wave] + x - v;
-  __syncthreads();
-  return r;
-}
-
-__global__ void __launch_bounds__(1024) binScanKernel(const Params P, uint32_t* binStart) {
-  __shared__ uint32_t sums[17];
-  __shared__ uint32_t isLast;
-  int b = blockIdx.x;
-  uint32_t* row = P.binChunkCount + (size_t)b * P.numChunks;
-  uint32_t carry = 0;
-  for (int base = 0; base < P.numChunks; base += 1024) {
-    int i = base + threadIdx.x;
-    uint32_t v = (i < P.numChunks) ? row[i] : 0u;
-    uint32_t total;
-    uint32_t ex = blockExclusiveScan<1024>(v, sums, total);
-    if (i < P.numChunks) row[i] = carry + ex;
-    carry += total;
-  }
-  if (threadIdx.x == 0) {
-    P.binTotal[b] = carry;
-    __threadfence();
-    isLast = (atomicAdd(&P.counters->binScanDone, 1u) == (uint32_t)P.numBins - 1) ? 1u : 0u;
-  }
-  __syncthreads();
-  if (!isLast) return;
-  __threadfence();
-  // Last block: bin starts (prefix over bins) and the grand total.
-  uint32_t v = (threadIdx.x < (unsigned)P.numBins) ? __atomic_load_n(&P.binTotal[threadIdx.x], __ATOMIC_RELAXED) : 0u;
-  uint32_t total;
-  uint32_t ex = blockExclusiveScan<1024>(v, sums, total);
-  if (threadIdx.x < (unsigned)P.numBins) binStart[threadIdx.x] = ex;
-  if (threadIdx.x == 0) {
-    P.counters->binEntries = total;
-    if (total > (uint32_t)P.maxBinEntries) atomicOr(&P.counters->overflow, 2u);
-  }
-}
-
-__global__ void __launch_bounds__(kBinThreads) binScatterKernel(const Params P, const uint32_t* binStart) {
-  constexpr int NW = kBinThreads / 64;
-  __shared__ uint32_t outBase[kMaxBins];
-  __shared__ uint32_t run[kMaxBins];
-  __shared__ uint64_t emitMask[NW][kMaxBins];
-  __shared__ uint32_t emitPre[NW][kMaxBins];
-  __shared__ uint32_t list[kBinThreads * 7];
-  __shared__ uint32_t sums[NW + 1];
-
-  if (P.counters->overflow & 3u) return;
-  int chunk = blockIdx.x;
-  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < P.numBins; i += kBinThreads) {
-    outBase[i] = binStart[i] + P.binChunkCount[(size_t)i * P.numChunks + chunk];
-    run[i] = 0;
-  }
-  __syncthreads();
-
-  for (int r = 0; r < kBinChunk; r += kBinThreads) {
-    int t0 = chunk * kBinChunk + r;
-    if (t0 >= P.numTris) break;
-    int t = t0 + threadIdx.x;
-    int k = (t < P.numTris) ? P.triSubtris[t] : 0;
-    uint32_t total;
-    uint32_t pos = blockExclusiveScan<kBinThreads>((uint32_t)k, sums, total);
-    for (int s = 0; s < k; s++) list[pos + s] = subtriSlot(P, t, k, s);
-    __syncthreads();
-
-    for (uint32_t sub = 0; sub < total; sub += kBinThreads) {
-      for (int i = threadIdx.x; i < NW * P.numBins; i += kBinThreads)
-        emitMask[i / P.numBins][i % P.numBins] = 0ull;
-      __syncthreads();
-      uint32_t e = sub + threadIdx.x;
-      bool valid = e < total;
-      uint32_t slot = valid ? list[e] : 0u;
-      Box b = {1, 1, 0, 0};
-      if (valid) {
-        b = binBox(P, P.triHeader[slot]);
-        uint64_t bit = 1ull << lane;
-        for (int y = b.y0; y <= b.y1; y++)
-          for (int x = b.x0; x <= b.x1; x++) atomicOr((unsigned long long*)&emitMask[wave][x + y * P.widthBins], bit);
-      }
-      __syncthreads();
-      for (int i = threadIdx.x; i < P.numBins; i += kBinThreads) {
-        uint32_t acc = run[i];
-#pragma unroll
-        for (int w = 0; w < NW; w++) {
-          emitPre[w][i] = acc;
-          acc += (uint32_t)__popcll(emitMask[w][i]);
-        }
-        run[i] = acc;
-      }
-      __syncthreads();
-      if (valid) {
-        uint64_t lt = laneMaskLt();
-        for (int y = b.y0; y <= b.y1; y++)
-          for (int x = b.x0; x <= b.x1; x++) {
-            int bin = x + y * P.widthBins;
-            uint32_t p = outBase[bin] + emitPre[wave][bin] + (uint32_t)__popcll(emitMask[wave][bin] & lt);
-            if (p < (uint32_t)P.maxBinEntries) P.binList[p] = slot;
-          }
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Coarse stage
-// ---------------------------------------------------------------------------
-
-// Tiles of bin (bx, by) touched by a triangle: bounding box clipped to the
-// bin and to the surface, then a conservative edge test per tile against the
-// closed tile square (no sample lies on a tile border, so a tile whose
-// square is strictly outside one edge holds no covered sample).
-struct TriTiles {
-  int32_t x0, y0, x1, y1;         // bin-local tile box (inclusive)
-  int32_t ax[3], ay[3], dx[3], dy[3];
-  int64_t c[3];                   // a_k x d_k (absolute subpixels)
 };
 
-__device__ __forceinline__ TriTiles triTiles(const Params& P, const Header& h, int bx, int by) {
-  TriTiles r;
-  int32_t ox = P.viewportWidth << (kSubpixelLog2 - 1), oy = P.viewportHeight << (kSubpixelLog2 - 1);
-  r.ax[0] = h.v0x + ox; r.ay[0] = h.v0y + oy;
-  r.ax[1] = h.v1x + ox; r.ay[1] = h.v1y + oy;
-  r.ax[2] = h.v2x + ox; r.ay[2] = h.v2y + oy;
-  int32_t minx = min(min(r.ax[0], r.ax[1]), r.ax[2]), maxx = max(max(r.ax[0], r.ax[1]), r.ax[2]);
-  int32_t miny = min(min(r.ay[0], r.ay[1]), r.ay[2]), maxy = max(max(r.ay[0], r.ay[1]), r.ay[2]);
+// Tile (x, y) of the bin holds a covered sample only if its closed square
+// reaches the inner side of all three edges: min over the square of
+// p x d <= a x d.  With the corner that minimises p x d, p x d at tile
+// (x, y) = p x d at the box origin + rel, |rel| < 2^29 (<= 15 tile steps of
+// 128 subpixels times |d| < 2^17), so the threshold c - p0 x d clamped to
+// +-2^30 decides every tile exactly in 32 bits.
+__device__ __forceinline__ TileTest tileTestOf(const Params& P, uint4 h, int bx, int by) {
+  TileTest r;
+  const int32_t ox = P.viewportWidth << (kSubpixelLog2 - 1), oy = P.viewportHeight << (kSubpixelLog2 - 1);
+  int32_t ax[3], ay[3];
+  ax[0] = (int32_t)(int16_t)(h.x & 0xFFFFu) + ox; ay[0] = (int32_t)(int16_t)(h.x >> 16) + oy;
+  ax[1] = (int32_t)(int16_t)(h.y & 0xFFFFu) + ox; ay[1] = (int32_t)(int16_t)(h.y >> 16) + oy;
+  ax[2] = (int32_t)(int16_t)(h.z & 0xFFFFu) + ox; ay[2] = (int32_t)(int16_t)(h.z >> 16) + oy;
+  const int32_t minx = min(min(ax[0], ax[1]), ax[2]), maxx = max(max(ax[0], ax[1]), ax[2]);
+  const int32_t miny = min(min(ay[0], ay[1]), ay[2]), maxy = max(max(ay[0], ay[1]), ay[2]);
   const int ts = kSubpixelLog2 + kTileLog2;
-  int32_t tx0 = bx << kBinLog2, ty0 = by << kBinLog2;
-  int32_t lastX = min(P.widthTiles - tx0, 1 << kBinLog2) - 1;
-  int32_t lastY = min(P.heightTiles - ty0, 1 << kBinLog2) - 1;
+  const int32_t tx0 = bx << kBinLog2, ty0 = by << kBinLog2;
+  const int32_t lastX = min(P.widthTiles - tx0, 1 << kBinLog2) - 1;
+  const int32_t lastY = min(P.heightTiles - ty0, 1 << kBinLog2) - 1;
   r.x0 = max((minx >> ts) - tx0, 0); r.x1 = min((maxx >> ts) - tx0, lastX);
   r.y0 = max((miny >> ts) - ty0, 0); r.y1 = min((maxy >> ts) - ty0, lastY);
+  const int64_t BX = (int64_t)(tx0 + r.x0) << ts, BY = (int64_t)(ty0 + r.y0) << ts;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    int j = (k + 1) % 3;
-    r.dx[k] = r.ax[j] - r.ax[k];
-    r.dy[k] = r.ay[j] - r.ay[k];
-    r.c[k] = (int64_t)r.ax[k] * r.dy[k] - (int64_t)r.ay[k] * r.dx[k];
+    const int j = (k + 1) % 3;
+    const int32_t dx = ax[j] - ax[k], dy = ay[j] - ay[k];
+    const int64_t c = (int64_t)ax[k] * dy - (int64_t)ay[k] * dx;
+    const int64_t px = BX + ((dy > 0) ? 0 : (1 << ts));
+    const int64_t py = BY + ((dx > 0) ? (1 << ts) : 0);
+    int64_t g = c - (px * dy - py * dx);
+    g = g > (1ll << 30) ? (1ll << 30) : (g < -(1ll << 30) ? -(1ll << 30) : g);
+    r.G[k] = (int32_t)g;
+    r.sx[k] = dy * (1 << ts);
+    r.sy[k] = -dx * (1 << ts);
   }
   return r;
 }
 
-// Does the closed square of bin-local tile (x, y) reach the inner side of
-// every edge?  Inner side: (a - p) x d >= 0, i.e. p x d <= a x d.
-__device__ __forceinline__ bool tileHit(const TriTiles& r, int bx, int by, int x, int y) {
-  const int ts = kSubpixelLog2 + kTileLog2;
-  int64_t X0 = (int64_t)(((bx << kBinLog2) + x) << ts), Y0 = (int64_t)(((by << kBinLog2) + y) << ts);
-  int64_t X1 = X0 + (1 << ts), Y1 = Y0 + (1 << ts);
+// One round of up to 1024 bin entries: record the tiles each entry touches.
+// COUNT: add to cnt[tile]; otherwise set bits in emitMask[wave][tile].
+template <bool COUNT>
+__device__ __forceinline__ void coarseRecord(const TileTest& r, bool valid, uint32_t* cnt,
+                                             uint64_t (*emitMask)[kBinTiles]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int w = valid ? r.x1 - r.x0 + 1 : 0, h = valid ? r.y1 - r.y0 + 1 : 0;
+  const bool has = valid && w > 0 && h > 0;
+  const uint64_t bit = 1ull << lane;
+  const bool allSmall = __ballot(has && (w > 2 || h > 2)) == 0;
+  bool coop = false;
+  int ux0 = 0, ux1 = -1, uy0 = 0, uy1 = -1;
+  if (!allSmall) {
+    // Union box of the wave's triangles and the largest single box.
+    int a0 = has ? r.x0 : 1 << 20, a1 = has ? r.x1 : -1, b0 = has ? r.y0 : 1 << 20, b1 = has ? r.y1 : -1;
+    int area = has ? w * h : 0;
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    int64_t px = (r.dy[k] > 0) ? X0 : X1;      // minimises px*dy
-    int64_t py = (r.dx[k] > 0) ? Y1 : Y0;      // maximises py*dx
-    if (px * r.dy[k] - py * r.dx[k] > r.c[k]) return false;
+    for (int o = 32; o >= 1; o >>= 1) {
+      a0 = min(a0, __shfl_xor(a0, o, 64)); a1 = max(a1, __shfl_xor(a1, o, 64));
+      b0 = min(b0, __shfl_xor(b0, o, 64)); b1 = max(b1, __shfl_xor(b1, o, 64));
+      area = max(area, __shfl_xor(area, o, 64));
+    }
+    ux0 = a0; ux1 = a1; uy0 = b0; uy1 = b1;
+    coop = (ux1 - ux0 + 1) * (uy1 - uy0 + 1) <= 2 * area;
   }
-  return true;
+  if (coop) {
+    for (int y = uy0; y <= uy1; y++)
+      for (int x = ux0; x <= ux1; x++) {
+        const bool hit = has && x >= r.x0 && x <= r.x1 && y >= r.y0 && y <= r.y1 && r.hit(x, y);
+        const uint64_t m = __ballot(hit);
+        if (m != 0 && lane == 0) {
+          if (COUNT) atomicAdd(&cnt[x + (y << kBinLog2)], (uint32_t)__popcll(m));
+          else emitMask[wave][x + (y << kBinLog2)] = m;
+        }
+      }
+  } else if (has) {
+    for (int y = r.y0; y <= r.y1; y++)
+      for (int x = r.x0; x <= r.x1; x++)
+        if (r.hit(x, y)) {
+          if (COUNT) atomicAdd(&cnt[x + (y << kBinLog2)], 1u);
+          else atomicOr((unsigned long long*)&emitMask[wave][x + (y << kBinLog2)], (unsigned long long)bit);
+        }
+  }
 }
 
-__global__ void __launch_bounds__(kCoarseThreads) coarseKernel(const Params P, const uint32_t* binStart) {
+__global__ void __launch_bounds__(kCoarseThreads) coarseKernel(const Params P) {
+  __shared__ uint32_t segStart[kSegBatch + 1];
+  __shared__ uint32_t segAddr[kSegBatch];
   __shared__ uint32_t cnt[kBinTiles];
   __shared__ uint32_t run[kBinTiles];
   __shared__ uint64_t emitMask[kCoarseWaves][kBinTiles];
@@ -292,95 +177,114 @@ __global__ void __launch_bounds__(kCoarseThreads) coarseKernel(const Params P, c
   __shared__ uint32_t sBase;
 
   if (P.counters->overflow & 3u) return;
-  int bin = blockIdx.x;
-  int bx = bin % P.widthBins, by = bin / P.widthBins;
-  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t first = binStart[bin];
-  uint32_t num = P.binTotal[bin];
+  const int bin = blockIdx.x;
+  const int bx = bin % P.widthBins, by = bin / P.widthBins;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t* bCnt = P.binCnt + (size_t)bin * P.numChunks;
+  const uint32_t* bOff = P.binOff + (size_t)bin * P.numChunks;
 
   for (int i = threadIdx.x; i < kBinTiles; i += kCoarseThreads) cnt[i] = 0;
   __syncthreads();
 
-  // Pass 1: count entries per tile.
-  for (uint32_t e = threadIdx.x; e < num; e += kCoarseThreads) {
-    uint32_t slot = P.binList[first + e];
-    if (slot >= (uint32_t)P.maxSubtris) continue;
-    TriTiles r = triTiles(P, P.triHeader[slot], bx, by);
-    for (int y = r.y0; y <= r.y1; y++)
-      for (int x = r.x0; x <= r.x1; x++)
-        if (tileHit(r, bx, by, x, y)) atomicAdd(&cnt[x + (y << kBinLog2)], 1u);
-  }
-  __syncthreads();
-
-  // Allocate the bin's tile lists: exclusive scan over the 256 tiles.
-  uint32_t v = (threadIdx.x < kBinTiles) ? cnt[threadIdx.x] : 0u;
-  uint32_t total;
-  uint32_t ex = blockExclusiveScan<kCoarseThreads>(v, sums, total);
-  if (threadIdx.x == 0) {
-    uint32_t base = total ? atomicAdd(&P.counters->tileEntries, total) : 0u;
-    if (total && (uint64_t)base + total > (uint64_t)P.maxTileEntries) {
-      atomicOr(&P.counters->overflow, 4u);
-      base = 0xFFFFFFFFu;
-    }
-    sBase = base;
-  }
-  __syncthreads();
-  uint32_t base = sBase;
-  bool ok = (base != 0xFFFFFFFFu);
-  if (threadIdx.x < kBinTiles) {
-    int x = threadIdx.x & ((1 << kBinLog2) - 1), y = threadIdx.x >> kBinLog2;
-    int gx = (bx << kBinLog2) + x, gy = (by << kBinLog2) + y;
-    if (gx < P.widthTiles && gy < P.heightTiles) {
-      int tile = gx + gy * P.widthTiles;
-      P.tileStart[tile] = ok ? base + ex : 0u;
-      P.tileCount[tile] = ok ? v : 0u;
-    }
-    run[threadIdx.x] = base + ex;
-  }
-  if (!ok || total == 0) return;
-  __syncthreads();
-
-  // Pass 2: emit in submission order, 1024 entries per round.
-  for (uint32_t r0 = 0; r0 < num; r0 += kCoarseThreads) {
-    for (int i = threadIdx.x; i < kCoarseWaves * kBinTiles; i += kCoarseThreads)
-      emitMask[i / kBinTiles][i % kBinTiles] = 0ull;
-    __syncthreads();
-    uint32_t e = r0 + threadIdx.x;
-    bool valid = e < num;
-    uint32_t slot = 0;
-    TriTiles r;
-    r.x0 = 1; r.x1 = 0; r.y0 = 1; r.y1 = 0;
-    if (valid) {
-      slot = P.binList[first + e];
-      valid = slot < (uint32_t)P.maxSubtris;
-    }
-    if (valid) {
-      r = triTiles(P, P.triHeader[slot], bx, by);
-      uint64_t bit = 1ull << lane;
-      for (int y = r.y0; y <= r.y1; y++)
-        for (int x = r.x0; x <= r.x1; x++)
-          if (tileHit(r, bx, by, x, y)) atomicOr((unsigned long long*)&emitMask[wave][x + (y << kBinLog2)], bit);
-    }
-    __syncthreads();
-    if (threadIdx.x < kBinTiles) {
-      uint32_t acc = run[threadIdx.x];
-#pragma unroll
-      for (int w = 0; w < kCoarseWaves; w++) {
-        emitPre[w][threadIdx.x] = acc;
-        acc += (uint32_t)__popcll(emitMask[w][threadIdx.x]);
-      }
-      run[threadIdx.x] = acc;
-    }
-    __syncthreads();
-    if (valid) {
-      uint64_t lt = laneMaskLt();
-      for (int y = r.y0; y <= r.y1; y++)
-        for (int x = r.x0; x <= r.x1; x++) {
-          int ti = x + (y << kBinLog2);
-          uint64_t m = emitMask[wave][ti];
-          uint32_t p = emitPre[wave][ti] + (uint32_t)__popcll(m & lt);
-          if ((m & (1ull << lane)) && p < (uint32_t)P.maxTileEntries) P.tileList[p] = slot;
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) {
+      // Allocate the bin's tile lists: exclusive scan over the tiles.
+      const uint32_t v = (threadIdx.x < kBinTiles) ? cnt[threadIdx.x] : 0u;
+      uint32_t total;
+      const uint32_t ex = blockScanC<kCoarseThreads>(v, sums, total);
+      if (threadIdx.x == 0) {
+        uint32_t base = total ? atomicAdd(&P.counters->tileEntries, total) : 0u;
+        if (total && (uint64_t)base + total > (uint64_t)P.maxTileEntries) {
+          atomicOr(&P.counters->overflow, 4u);
+          base = 0xFFFFFFFFu;
         }
+        sBase = base;
+      }
+      __syncthreads();
+      const uint32_t base = sBase;
+      const bool ok = base != 0xFFFFFFFFu;
+      if (threadIdx.x < kBinTiles) {
+        const int x = threadIdx.x & ((1 << kBinLog2) - 1), y = threadIdx.x >> kBinLog2;
+        const int gx = (bx << kBinLog2) + x, gy = (by << kBinLog2) + y;
+        if (gx < P.widthTiles && gy < P.heightTiles) {
+          const int tile = gx + gy * P.widthTiles;
+          P.tileStart[tile] = ok ? base + ex : 0u;
+          P.tileCount[tile] = ok ? v : 0u;
+        }
+        run[threadIdx.x] = base + ex;
+      }
+      if (!ok || total == 0) return;
+      __syncthreads();
+    }
+
+    for (int cb = 0; cb < P.numChunks; cb += kSegBatch) {
+      // Segment table of chunks [cb, cb + nb): start of each segment in the
+      // bin's entry order and its address in the bin queue.
+      const int nb = min(kSegBatch, P.numChunks - cb);
+      const int i0 = 2 * threadIdx.x, i1 = i0 + 1;
+      const uint32_t n0 = (i0 < nb) ? bCnt[cb + i0] : 0u, n1 = (i1 < nb) ? bCnt[cb + i1] : 0u;
+      if (i0 < nb) segAddr[i0] = P.chunkBase[cb + i0] + bOff[cb + i0];
+      if (i1 < nb) segAddr[i1] = P.chunkBase[cb + i1] + bOff[cb + i1];
+      uint32_t segTotal;
+      const uint32_t s0 = blockScanC<kCoarseThreads>(n0 + n1, sums, segTotal);
+      if (i0 < nb) segStart[i0] = s0;
+      if (i1 < nb) segStart[i1] = s0 + n0;
+      if (threadIdx.x == 0) segStart[nb] = segTotal;
+      __syncthreads();
+
+      for (uint32_t r0 = 0; r0 < segTotal; r0 += kCoarseThreads) {
+        if (pass == 1) {
+          for (int i = threadIdx.x; i < kCoarseWaves * kBinTiles; i += kCoarseThreads)
+            emitMask[i / kBinTiles][i % kBinTiles] = 0ull;
+          __syncthreads();
+        }
+        const uint32_t g = r0 + threadIdx.x;
+        bool valid = g < segTotal;
+        uint32_t slot = 0;
+        TileTest r;
+        r.x0 = 1; r.x1 = 0; r.y0 = 1; r.y1 = 0;
+        if (valid) {
+          // Largest segment whose start is <= g.
+          int lo = 0, hi = nb - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (segStart[mid] <= g) lo = mid; else hi = mid - 1;
+          }
+          slot = P.binList[segAddr[lo] + (g - segStart[lo])];
+          valid = slot < (uint32_t)P.maxSubtris;
+          if (valid) r = tileTestOf(P, *reinterpret_cast<const uint4*>(&P.triHeader[slot]), bx, by);
+        }
+        if (pass == 0) {
+          coarseRecord<true>(r, valid, cnt, emitMask);
+          continue;
+        }
+        coarseRecord<false>(r, valid, cnt, emitMask);
+        __syncthreads();
+        if (threadIdx.x < kBinTiles) {
+          uint32_t acc = run[threadIdx.x];
+#pragma unroll
+          for (int q = 0; q < kCoarseWaves; q++) {
+            emitPre[q][threadIdx.x] = acc;
+            acc += (uint32_t)__popcll(emitMask[q][threadIdx.x]);
+          }
+          run[threadIdx.x] = acc;
+        }
+        __syncthreads();
+        if (valid && r.x0 <= r.x1 && r.y0 <= r.y1) {
+          const uint64_t bit = 1ull << lane, lt = lanesBelow();
+          for (int y = r.y0; y <= r.y1; y++)
+            for (int x = r.x0; x <= r.x1; x++) {
+              const int ti = x + (y << kBinLog2);
+              const uint64_t m = emitMask[wave][ti];
+              if (m & bit) {
+                const uint32_t p = emitPre[wave][ti] + (uint32_t)__popcll(m & lt);
+                if (p < (uint32_t)P.maxTileEntries) P.tileList[p] = slot;
+              }
+            }
+        }
+        __syncthreads();
+      }
+      __syncthreads();
     }
     __syncthreads();
   }
@@ -390,14 +294,8 @@ void launchReset(const Params& P, hipStream_t stream) {
   hipLaunchKernelGGL(resetCountersKernel, dim3(1), dim3(64), 0, stream, P.counters, (uint32_t)P.numTris);
 }
 
-void launchBin(const Params& P, uint32_t* binStart, hipStream_t stream) {
-  if (P.numChunks > 0) hipLaunchKernelGGL(binCountKernel, dim3(P.numChunks), dim3(kBinThreads), 0, stream, P);
-  hipLaunchKernelGGL(binScanKernel, dim3(P.numBins), dim3(1024), 0, stream, P, binStart);
-  if (P.numChunks > 0) hipLaunchKernelGGL(binScatterKernel, dim3(P.numChunks), dim3(kBinThreads), 0, stream, P, (const uint32_t*)binStart);
-}
-
-void launchCoarse(const Params& P, const uint32_t* binStart, hipStream_t stream) {
-  hipLaunchKernelGGL(coarseKernel, dim3(P.numBins), dim3(kCoarseThreads), 0, stream, P, binStart);
+void launchCoarse(const Params& P, hipStream_t stream) {
+  hipLaunchKernelGGL(coarseKernel, dim3(P.numBins), dim3(kCoarseThreads), 0, stream, P);
 }
 
 }  // namespace crhip

@@ -53,8 +53,8 @@ struct Counters {
   uint32_t numSubtris;     // slot allocator for clipped subtriangles (starts at numTris)
   uint32_t tileEntries;    // tile-list entry allocator
   uint32_t overflow;       // bit 0 subtris, bit 1 bin entries, bit 2 tile entries
-  uint32_t binEntries;     // total bin-list entries (written by the scan)
-  uint32_t binScanDone;    // blocks of the bin scan that have finished
+  uint32_t binEntries;     // bin-queue entry allocator
+  uint32_t unused0;
   uint32_t pad0[3];
   uint64_t fragments;      // (subtriangle, pixel) pairs with >= 1 covered sample
   uint64_t pad[27];
@@ -89,11 +89,13 @@ struct Params {
   Header* triHeader;
   TriData* triData;
   uint32_t* subtriTri;             // source triangle of slots >= numTris
-  // Bin lists (bin-major, each bin contiguous and in submission order).
-  int32_t binChunk;                // triangles per bin chunk
+  // Bin queue: chunk c (kSetupChunkTris triangles) owns a region starting at
+  // chunkBase[c]; bin b's entries of chunk c are binCnt[b][c] slots at
+  // region offset binOff[b][c], in submission order.
   int32_t numChunks;
-  uint32_t* binChunkCount;         // [numBins][numChunks]: counts, then offsets
-  uint32_t* binTotal;              // [numBins]
+  uint32_t* chunkBase;             // [numChunks]
+  uint32_t* binCnt;                // [numBins][numChunks]
+  uint32_t* binOff;                // [numBins][numChunks]
   int32_t maxBinEntries;
   uint32_t* binList;
   // Tile lists (each tile contiguous and in submission order).
@@ -204,12 +206,11 @@ __host__ __device__ inline int selectCentroid(int s, uint32_t mask) {
 }
 
 // Host-side launchers, one per stage file.
-void launchSetup(const Params& P, hipStream_t stream);
 void launchReset(const Params& P, hipStream_t stream);
-void launchBin(const Params& P, uint32_t* binStart, hipStream_t stream);
-void launchCoarse(const Params& P, const uint32_t* binStart, hipStream_t stream);
+void launchSetup(const Params& P, hipStream_t stream);     // setup + bin
+void launchCoarse(const Params& P, hipStream_t stream);
 void launchFine(const Params& P, hipStream_t stream);
 void launchFragmentSum(const Params& P, hipStream_t stream);
-constexpr int kBinChunkTris = 2048;
+constexpr int kSetupChunkTris = 1024;
 
 }  // namespace crhip

@@ -226,7 +226,10 @@ __device__ __forceinline__ float4 lerpClip(const float4& o, const float4& d1, co
 }
 
 // Slow path, kept out of line: frustum clip and fan (TriangleSetup.inl:326-412).
-__device__ __noinline__ void setupClipped(const Params P, int t, int3 vidx, float4 v0, float4 v1, float4 v2) {
+// Returns the number of subtriangles written (0 when the slot allocation
+// overflowed) and their first slot in *baseOut.
+__device__ __noinline__ int setupClipped(const Params P, int t, int3 vidx, float4 v0, float4 v1, float4 v2,
+                                         int* baseOut) {
   float4 od1 = make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, v1.w - v0.w);
   float4 od2 = make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, v2.w - v0.w);
   float4 ov0 = v0;
@@ -255,8 +258,10 @@ __device__ __noinline__ void setupClipped(const Params P, int t, int3 vidx, floa
     if (base + numSub > P.maxSubtris) {
       atomicOr(&P.counters->overflow, 1u);
       numVerts = 0;
+      numSub = 0;
     }
   }
+  *baseOut = base;
   v1 = tv1;
   for (int i = 2; i < numVerts; i++) {
     float4 vv2 = lerpClip(ov0, od1, od2, bary[i * 2], bary[i * 2 + 1]);
@@ -270,12 +275,12 @@ __device__ __noinline__ void setupClipped(const Params P, int t, int3 vidx, floa
     }
     v1 = vv2;
   }
+  return numSub;
 }
 
-__global__ void __launch_bounds__(256) triangleSetupKernel(const Params P) {
-  int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= P.numTris) return;
-
+// One triangle: returns its number of subtriangles and their first slot.
+__device__ __forceinline__ int setupOne(const Params& P, int t, int* base) {
+  *base = t;
   int3 vidx;
   const int32_t* ip = P.indices + (int64_t)t * 3;
   vidx.x = ip[0]; vidx.y = ip[1]; vidx.z = ip[2];
@@ -284,7 +289,7 @@ __global__ void __launch_bounds__(256) triangleSetupKernel(const Params P) {
   if ((uint32_t)vidx.x >= (uint32_t)P.numVertices || (uint32_t)vidx.y >= (uint32_t)P.numVertices ||
       (uint32_t)vidx.z >= (uint32_t)P.numVertices) {
     P.triSubtris[t] = 0;
-    return;
+    return 0;
   }
   float4 v0 = *reinterpret_cast<const float4*>(P.vertices + (int64_t)vidx.x * P.vertexStride);
   float4 v1 = *reinterpret_cast<const float4*>(P.vertices + (int64_t)vidx.y * P.vertexStride);
@@ -300,7 +305,7 @@ __global__ void __launch_bounds__(256) triangleSetupKernel(const Params P) {
                ((v0.w < -v0.z) & (v1.w < -v1.z) & (v2.w < -v2.z));
     if (out) {
       P.triSubtris[t] = 0;
-      return;
+      return 0;
     }
   }
 
@@ -319,15 +324,209 @@ __global__ void __launch_bounds__(256) triangleSetupKernel(const Params P) {
       if (res == 0)
         setupTriangle(P, &P.triHeader[t], &P.triData[t], vidx, v0, v1, v2, make_float2(0.0f, 0.0f),
                       make_float2(1.0f, 0.0f), make_float2(0.0f, 1.0f), p0, p1, p2, rcpW, d1, d2, area);
-      return;
+      return res == 0 ? 1 : 0;
     }
   }
-  setupClipped(P, t, vidx, v0, v1, v2);
+  return setupClipped(P, t, vidx, v0, v1, v2, base);
+}
+
+// ---------------------------------------------------------------------------
+// Fused setup + bin stage.
+//
+// One block sets up a chunk of kSetupChunk consecutive triangles (one per
+// thread) and immediately bins their subtriangles (128x128-pixel bins,
+// bounding-box test).  The chunk's entries go to a region of the bin queue
+// reserved with one atomic; inside the region the entries of each bin are
+// contiguous and in (triangle, subtriangle) order -- ranks come from per-wave
+// 64-bit emit masks and popcounts.  The coarse stage reads bin b as the
+// concatenation of its per-chunk segments in chunk order, which is
+// submission order, so no global scan is needed.  (The reference streams
+// triangles into linked 512-entry segments per (bin, stream) and merges 16
+// streams, BinRaster.inl:19-469, CoarseRaster.inl:170-257.)
+// ---------------------------------------------------------------------------
+
+constexpr int kSetupChunk = kSetupChunkTris;
+constexpr int kSetupWaves = kSetupChunk / 64;
+
+struct BinBox { int32_t x0, y0, x1, y1; };   // inclusive; empty when x0 > x1
+
+__device__ __forceinline__ BinBox binBoxOf(const Params& P, uint4 h) {
+  const int32_t ox = P.viewportWidth << (kSubpixelLog2 - 1), oy = P.viewportHeight << (kSubpixelLog2 - 1);
+  const int32_t x0 = (int32_t)(int16_t)(h.x & 0xFFFFu), y0 = (int32_t)(int16_t)(h.x >> 16);
+  const int32_t x1 = (int32_t)(int16_t)(h.y & 0xFFFFu), y1 = (int32_t)(int16_t)(h.y >> 16);
+  const int32_t x2 = (int32_t)(int16_t)(h.z & 0xFFFFu), y2 = (int32_t)(int16_t)(h.z >> 16);
+  const int32_t minx = min(min(x0, x1), x2) + ox, maxx = max(max(x0, x1), x2) + ox;
+  const int32_t miny = min(min(y0, y1), y2) + oy, maxy = max(max(y0, y1), y2) + oy;
+  BinBox b;
+  const int sh = kSubpixelLog2 + kTileLog2 + kBinLog2;
+  if (maxx < 0 || maxy < 0 || minx >= (P.widthPixels << kSubpixelLog2) || miny >= (P.heightPixels << kSubpixelLog2)) {
+    b.x0 = 1; b.x1 = 0; b.y0 = 1; b.y1 = 0;
+    return b;
+  }
+  b.x0 = max(minx >> sh, 0); b.x1 = min(maxx >> sh, P.widthBins - 1);
+  b.y0 = max(miny >> sh, 0); b.y1 = min(maxy >> sh, P.heightBins - 1);
+  return b;
+}
+
+__device__ __forceinline__ uint64_t laneLt() {
+  const uint32_t lane = threadIdx.x & 63;
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Exclusive scan of one value per thread over the whole block.
+template <int NT>
+__device__ __forceinline__ uint32_t blockScan(uint32_t v, uint32_t* sums, uint32_t& total) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sums[wave] = x;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const uint32_t w = (threadIdx.x < NW) ? sums[threadIdx.x] : 0u;
+    uint32_t sc = w;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(sc, o, 64);
+      if ((int)threadIdx.x >= o) sc += y;
+    }
+    if (threadIdx.x < NW) sums[threadIdx.x] = sc - w;
+    if (threadIdx.x == NW - 1) sums[NW] = sc;
+  }
+  __syncthreads();
+  total = sums[NW];
+  const uint32_t r = sums[wave] + x - v;
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(kSetupChunk) setupBinKernel(const Params P) {
+  __shared__ uint32_t cnt[kMaxBins];
+  __shared__ uint32_t run[kMaxBins];
+  __shared__ uint64_t emitMask[kSetupWaves][kMaxBins];
+  __shared__ uint32_t emitPre[kSetupWaves][kMaxBins];
+  __shared__ uint32_t list[kSetupChunk];
+  __shared__ uint32_t sums[kSetupWaves + 1];
+  __shared__ uint32_t sBase;
+
+  const int chunk = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = chunk * kSetupChunk + threadIdx.x;
+
+  // ---- setup ---------------------------------------------------------------
+  int base = t, k = 0;
+  if (t < P.numTris) k = setupOne(P, t, &base);
+
+  for (int i = threadIdx.x; i < P.numBins; i += kSetupChunk) cnt[i] = 0;
+  // Window = up to kSetupChunk consecutive subtriangles of the chunk in
+  // submission order; a chunk without clipped triangles is a single window.
+  const bool compact = __syncthreads_or(k > 1) != 0;
+  uint32_t total;
+  const uint32_t pos = blockScan<kSetupChunk>((uint32_t)k, sums, total);
+  const uint32_t numWin = (total + kSetupChunk - 1) / kSetupChunk;
+
+  // Emit masks of one window: bit `lane` of emitMask[wave][bin] is set when
+  // the window entry held by this thread touches the bin.
+  auto buildMasks = [&](uint32_t w, uint32_t& slot, BinBox& bb) -> bool {
+    for (int i = threadIdx.x; i < kSetupWaves * P.numBins; i += kSetupChunk)
+      emitMask[i / P.numBins][i % P.numBins] = 0ull;
+    if (!compact) {
+      // fast path: entry of this thread is its own subtriangle (k <= 1)
+      slot = (uint32_t)base;
+      bb.x0 = 1; bb.x1 = 0; bb.y0 = 1; bb.y1 = 0;
+      __syncthreads();
+      if (k == 1) bb = binBoxOf(P, *reinterpret_cast<const uint4*>(&P.triHeader[slot]));
+    } else {
+      for (int s = 0; s < k; s++) {
+        const uint32_t e = pos + (uint32_t)s;
+        if (e >= w * kSetupChunk && e < (w + 1) * kSetupChunk) list[e - w * kSetupChunk] = (uint32_t)base + s;
+      }
+      __syncthreads();
+      const uint32_t e = w * kSetupChunk + threadIdx.x;
+      bb.x0 = 1; bb.x1 = 0; bb.y0 = 1; bb.y1 = 0;
+      slot = 0;
+      if (e < total) {
+        slot = list[threadIdx.x];
+        bb = binBoxOf(P, *reinterpret_cast<const uint4*>(&P.triHeader[slot]));
+      }
+    }
+    const uint64_t bit = 1ull << lane;
+    for (int y = bb.y0; y <= bb.y1; y++)
+      for (int x = bb.x0; x <= bb.x1; x++)
+        atomicOr((unsigned long long*)&emitMask[wave][x + y * P.widthBins], (unsigned long long)bit);
+    __syncthreads();
+    return true;
+  };
+
+  // ---- pass 1: per-bin counts ---------------------------------------------
+  uint32_t slot0 = 0;
+  BinBox bb0;
+  for (uint32_t w = 0; w < numWin; w++) {
+    buildMasks(w, slot0, bb0);
+    for (int i = threadIdx.x; i < P.numBins; i += kSetupChunk) {
+      uint32_t acc = cnt[i];
+#pragma unroll
+      for (int q = 0; q < kSetupWaves; q++) acc += (uint32_t)__popcll(emitMask[q][i]);
+      cnt[i] = acc;
+    }
+    __syncthreads();
+  }
+
+  // ---- reserve the chunk's region; publish counts and offsets ---------------
+  uint32_t tot2;
+  const uint32_t c = (threadIdx.x < (unsigned)P.numBins) ? cnt[threadIdx.x] : 0u;
+  const uint32_t off = blockScan<kSetupChunk>(c, sums, tot2);
+  if (threadIdx.x == 0) {
+    uint32_t b = tot2 ? atomicAdd(&P.counters->binEntries, tot2) : 0u;
+    if (tot2 && (uint64_t)b + tot2 > (uint64_t)P.maxBinEntries) {
+      atomicOr(&P.counters->overflow, 2u);
+      b = 0xFFFFFFFFu;
+    }
+    sBase = b;
+    P.chunkBase[chunk] = b;
+  }
+  if (threadIdx.x < (unsigned)P.numBins) {
+    P.binCnt[(size_t)threadIdx.x * P.numChunks + chunk] = c;
+    P.binOff[(size_t)threadIdx.x * P.numChunks + chunk] = off;
+    run[threadIdx.x] = off;
+  }
+  __syncthreads();
+  const uint32_t regionBase = sBase;
+  if (regionBase == 0xFFFFFFFFu || tot2 == 0) return;
+
+  // ---- pass 2: write the entries in order ------------------------------------
+  for (uint32_t w = 0; w < numWin; w++) {
+    uint32_t slot = slot0;
+    BinBox bb = bb0;
+    if (numWin > 1) buildMasks(w, slot, bb);
+    for (int i = threadIdx.x; i < P.numBins; i += kSetupChunk) {
+      uint32_t acc = run[i];
+#pragma unroll
+      for (int q = 0; q < kSetupWaves; q++) {
+        emitPre[q][i] = acc;
+        acc += (uint32_t)__popcll(emitMask[q][i]);
+      }
+      run[i] = acc;
+    }
+    __syncthreads();
+    const uint64_t lt = laneLt();
+    for (int y = bb.y0; y <= bb.y1; y++)
+      for (int x = bb.x0; x <= bb.x1; x++) {
+        const int bin = x + y * P.widthBins;
+        const uint32_t p = regionBase + emitPre[wave][bin] + (uint32_t)__popcll(emitMask[wave][bin] & lt);
+        if (p < (uint32_t)P.maxBinEntries) P.binList[p] = slot;
+      }
+    __syncthreads();
+  }
 }
 
 void launchSetup(const Params& P, hipStream_t stream) {
   if (P.numTris <= 0) return;
-  hipLaunchKernelGGL(triangleSetupKernel, dim3((P.numTris + 255) / 256), dim3(256), 0, stream, P);
+  hipLaunchKernelGGL(setupBinKernel, dim3(P.numChunks), dim3(kSetupChunk), 0, stream, P);
 }
 
 }  // namespace crhip

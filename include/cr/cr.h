@@ -91,8 +91,8 @@ typedef struct cr_pipe_desc {
 } cr_pipe_desc;
 
 /* CudaRaster::Stats (CudaRaster.hpp:45-51): seconds per stage of the last
- * draw, from HIP events.  bin_time covers the bin count/scan/scatter
- * kernels. */
+ * draw, from HIP events.  Triangle setup and binning run as one kernel:
+ * setup_time covers both and bin_time is 0. */
 typedef struct cr_stats {
     float setup_time;
     float bin_time;
