@@ -37,14 +37,67 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def fine_algorithmic_bytes(scene, counters, triid):
+def fine_algorithmic_bytes(scene, visible_subtris, triid):
     """Compulsory HBM bytes of one fine-raster launch: every texel of the
-    colour, depth (and triangle-id) surfaces written once, each visible
-    subtriangle's 16 B header + 64 B plane-equation record and its three
-    16 B colour varyings read once."""
+    colour, depth (and triangle-id) surfaces written once (deferred clear:
+    nothing read), and per visible subtriangle its 16 B header, 64 B
+    plane-equation record and the three 16 B colour varyings read once."""
     hr, tw = scene.surface_shape
     surfaces = hr * tw * (12 if triid else 8)
-    return surfaces + counters["num_subtris_visible"] * (80 + 48)
+    return surfaces + visible_subtris * (16 + 64 + 3 * 16)
+
+
+def setup_algorithmic_bytes(scene, c, visible_subtris):
+    """Setup + binning: indices (12 B) and three clip positions (16 B each)
+    read per triangle; per visible subtriangle a 16 B header, a 64 B data
+    record and its 4 B source index written; 4 B per bin-queue entry and
+    8 B per (bin, chunk) count/offset pair written."""
+    chunks = (scene.num_tris + 1023) // 1024
+    bins = ((scene.width + 127) // 128) * ((scene.height + 127) // 128)
+    return (scene.num_tris * (12 + 3 * 16) + visible_subtris * (16 + 64 + 4)
+            + c["bin_entries"] * 4 + bins * chunks * 8)
+
+
+def coarse_algorithmic_bytes(scene, c, visible_subtris):
+    """Coarse: every bin-queue entry (4 B) and visible header (16 B) read,
+    the per-(bin, chunk) counts/offsets read, every tile-list entry (4 B)
+    and per-tile start/count (8 B) written."""
+    chunks = (scene.num_tris + 1023) // 1024
+    bins = ((scene.width + 127) // 128) * ((scene.height + 127) // 128)
+    tiles = ((scene.width + 7) // 8) * ((scene.height + 7) // 8)
+    return c["bin_entries"] * 4 + visible_subtris * 16 + bins * chunks * 8 + c["tile_entries"] * 4 + tiles * 8
+
+
+def measured_traffic(kernel, workload_key, triid):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE counters and scaled by a copy-kernel calibration, as recorded
+    in a profile summary under profiles/ (a line
+    `TRAFFIC <kernel> <workload> triid=<0|1> bytes=<n>`), or None."""
+    pdir = os.path.join(REPO, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    want = f"TRAFFIC {kernel} {workload_key} triid={int(triid)} "
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith((".txt", ".md")):
+            continue
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                if line.startswith(want):
+                    return int(line.split("bytes=")[1].split()[0])
+    return None
+
+
+def copy_calibration():
+    """A plain device-to-device copy of known size (256 MiB, three times):
+    the reference point for scaling FETCH_SIZE / WRITE_SIZE readings."""
+    import torch
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "copies": 3}))
 
 
 def frame_algorithmic_bytes(scene, triid):
@@ -79,6 +132,10 @@ def main():
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--no-triid", action="store_true", help="draw without the triangle-id surface")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-count", action="store_true",
+                    help="skip the counting draw (fragment and visible-subtriangle totals)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 256 MiB copy kernel (reference for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -86,6 +143,9 @@ def main():
         raise SystemExit(2)
 
     import torch
+    if args.calibrate_copy:
+        copy_calibration()
+        return
     import cudaraster
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
@@ -93,12 +153,15 @@ def main():
     triid = not args.no_triid
     r = cudaraster.SceneRenderer(scene, triangle_ids=triid)
 
-    # One counting draw (hierarchical Z off) for the exact fragment count.
-    r.cr.setDebugParams(count_fragments=True)
-    r.draw()
-    cnt = r.cr.getCounters()
-    fragments = cnt["fragments"]
-    r.cr.setDebugParams()
+    # One counting draw (hierarchical Z off; a separate kernel variant) for
+    # the exact fragment and visible-subtriangle totals.
+    fragments = visible = None
+    if not args.no_count:
+        r.cr.setDebugParams(count_fragments=True)
+        r.draw()
+        cnt = r.cr.getCounters()
+        fragments, visible = cnt["fragments"], cnt["visible_subtris"]
+        r.cr.setDebugParams()
     for _ in range(args.warmup):
         r.draw()
     torch.cuda.synchronize()
@@ -115,23 +178,26 @@ def main():
     counters = r.cr.getCounters()
     ms = elapsed / args.steps * 1e3
 
-    # Visible subtriangles (for the fine kernel's compulsory reads).
-    from cudaraster import scenes as _s  # noqa: F401
-    counters["num_subtris_visible"] = counters["bin_entries"] if counters["bin_entries"] else counters["num_subtris"]
     avg = {k: v / args.steps for k, v in stage.items()}
-    dominant = max(avg, key=avg.get)
-    kname = {"setupTime": "triangleSetupKernel", "binTime": "bin kernels", "coarseTime": "coarseKernel",
-             "fineTime": "fineKernel"}[dominant]
-    fine_bytes = fine_algorithmic_bytes(scene, counters, triid)
-    if dominant == "fineTime":
-        achieved = fine_bytes / avg["fineTime"] / 1e9
-    else:
-        achieved = frame_algorithmic_bytes(scene, triid) / (sum(avg.values())) / 1e9
+    # Per-kernel roofline: algorithmic bytes of one launch over its average
+    # duration (HIP events recorded around each kernel on the draw stream).
+    kernels = {}
+    for key, kname, fn in (("setupTime", "setupBinKernel", lambda: setup_algorithmic_bytes(scene, counters, visible)),
+                           ("coarseTime", "coarseKernel", lambda: coarse_algorithmic_bytes(scene, counters, visible)),
+                           ("fineTime", "fineKernel", lambda: fine_algorithmic_bytes(scene, visible, triid))):
+        b = fn() if visible is not None else None
+        kernels[kname] = {"ms": avg[key] * 1e3, "algorithmic_bytes": b,
+                          "GBps": b / avg[key] / 1e9 if b is not None and avg[key] > 0 else None}
+    kname = max(kernels, key=lambda k: kernels[k]["ms"])
+    alg_bytes = kernels[kname]["algorithmic_bytes"]
+    kernel_s = kernels[kname]["ms"] * 1e-3
+    achieved = kernels[kname]["GBps"]
+    traffic = measured_traffic(kname, args.workload, triid)
     line = {
         "metric": METRIC,
         "value": scene.num_tris * args.steps / elapsed / 1e6,
         "unit": "Mtris/s",
-        "gfragments_per_s": fragments * args.steps / elapsed / 1e9,
+        "gfragments_per_s": fragments * args.steps / elapsed / 1e9 if fragments is not None else None,
         "fragments_per_frame": fragments,
         "batched_over_plain_views_ratio": None,
         "n_gpus": 1,
@@ -147,9 +213,10 @@ def main():
                    "height": scene.height, "samples": scene.num_samples, "triangle_id_surface": triid},
         "stage_ms": {k: v * 1e3 for k, v in avg.items()},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_launch": fine_bytes if dominant == "fineTime"
-                     else frame_algorithmic_bytes(scene, triid)},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
+                     "traffic": traffic, "algorithmic_bytes_per_launch": alg_bytes,
+                     "kernel_ms": kernel_s * 1e3, "visible_subtris": visible},
+        "kernels": kernels,
         "frame_algorithmic_GBps": frame_algorithmic_bytes(scene, triid) / (ms * 1e-3) / 1e9,
         "counters": counters,
     }

@@ -79,7 +79,7 @@ struct cr_context {
   int64_t maxSubtris = 1, maxBinEntries = 1, maxTileEntries = 1;
   DevBuf triSubtris, triHeader, triData, subtriTri;
   DevBuf chunkBase, binCnt, binOff, binList;
-  DevBuf tileStart, tileCount, tileList, tileFrags;
+  DevBuf tileStart, tileCount, tileList, tileFrags, chunkSubtris;
   DevBuf counters;
   Counters* hostCounters = nullptr;   // pinned
   hipEvent_t ev[5] = {};
@@ -179,7 +179,10 @@ cr_status ensureBuffers(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   CR_HIP(ctx, ctx->tileStart.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileCount.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileList.ensure((size_t)ctx->maxTileEntries * 4));
-  if (ctx->dbg.count_fragments) CR_HIP(ctx, ctx->tileFrags.ensure((size_t)g.nt * 4));
+  if (ctx->dbg.count_fragments) {
+    CR_HIP(ctx, ctx->tileFrags.ensure((size_t)g.nt * 4));
+    CR_HIP(ctx, ctx->chunkSubtris.ensure((size_t)std::max(numChunks, 1) * 4));
+  }
   return CR_OK;
 }
 
@@ -211,6 +214,7 @@ Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   P.clearDepth = ctx->clearDepth;
   P.hiz = (ctx->dbg.disable_hiz || ctx->dbg.count_fragments) ? 0 : 1;
   P.tileFrags = ctx->dbg.count_fragments ? ctx->tileFrags.as<uint32_t>() : nullptr;
+  P.chunkSubtris = ctx->dbg.count_fragments ? ctx->chunkSubtris.as<uint32_t>() : nullptr;
   P.maxSubtris = (int32_t)ctx->maxSubtris;
   P.triSubtris = ctx->triSubtris.as<uint8_t>();
   P.triHeader = ctx->triHeader.as<Header>();
@@ -271,6 +275,7 @@ cr_status finishDraw(cr_context* ctx, bool* overflow) {
   ctx->last.bin_entries = c.binEntries;
   ctx->last.tile_entries = c.tileEntries;
   ctx->last.fragments = (int64_t)c.fragments;
+  ctx->last.visible_subtris = (int64_t)c.visibleSubtris;
   *overflow = c.overflow != 0;
   if (!*overflow) return CR_OK;
   if (c.overflow & 1u) ctx->maxSubtris = std::max<int64_t>(ctx->maxSubtris, (int64_t)c.numSubtris + kSubtrisSlack);
@@ -320,7 +325,7 @@ cr_status drawImpl(cr_context* ctx, hipStream_t stream, bool sync) {
 
 extern "C" {
 
-int32_t cr_abi_version(void) { return (1 << 16) | 0; }
+int32_t cr_abi_version(void) { return (1 << 16) | 1; }
 
 const char* cr_last_error(const cr_context* ctx) { return ctx ? ctx->err.c_str() : g_lastError.c_str(); }
 
@@ -351,7 +356,7 @@ cr_status cr_context_destroy(cr_context* ctx) {
   if (ctx->pending) (void)hipStreamSynchronize(ctx->pendingStream);
   for (DevBuf* b : {&ctx->triSubtris, &ctx->triHeader, &ctx->triData, &ctx->subtriTri, &ctx->chunkBase,
                     &ctx->binCnt, &ctx->binOff, &ctx->binList, &ctx->tileStart, &ctx->tileCount,
-                    &ctx->tileList, &ctx->tileFrags, &ctx->counters})
+                    &ctx->tileList, &ctx->tileFrags, &ctx->chunkSubtris, &ctx->counters})
     b->release();
   if (ctx->hostCounters) (void)hipHostFree(ctx->hostCounters);
   for (int i = 0; i < 5; i++)

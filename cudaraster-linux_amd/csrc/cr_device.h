@@ -57,7 +57,8 @@ struct Counters {
   uint32_t unused0;
   uint32_t pad0[3];
   uint64_t fragments;      // (subtriangle, pixel) pairs with >= 1 covered sample
-  uint64_t pad[27];
+  uint64_t visibleSubtris; // subtriangles that survive setup (counting draws only)
+  uint64_t pad[26];
 };
 static_assert(sizeof(Counters) == 256, "counter block");
 
@@ -83,6 +84,7 @@ struct Params {
   uint32_t clearColor, clearDepth;
   int32_t hiz;                     // 1: tile-level early depth rejection
   uint32_t* tileFrags;             // per-tile fragment counts (counting draws only), else null
+  uint32_t* chunkSubtris;          // per-setup-chunk visible subtriangles (counting draws only), else null
   // Setup output.
   int32_t maxSubtris;
   uint8_t* triSubtris;

@@ -119,7 +119,8 @@ __device__ __forceinline__ void waveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int S, bool VIS>
+// COUNT: counting draw (fragment totals per tile into P.tileFrags).
+template <int S, bool VIS, bool COUNT>
 __global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P, const uint4* __restrict__ hdr,
                                                               const uint4* __restrict__ tdata,
                                                               const uint32_t* __restrict__ tileList,
@@ -147,7 +148,7 @@ __global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P, co
   if (group >= groups || tile >= P.numTiles) return;
   const uint32_t cnt = P.tileCount[tile];
   if (cnt == 0 && !P.deferredClear) {
-    if (P.tileFrags != nullptr && lane == 0) P.tileFrags[tile] = 0;
+    if (COUNT && lane == 0) P.tileFrags[tile] = 0;
     return;
   }
 
@@ -278,7 +279,7 @@ __global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P, co
       }
       const uint64_t covBallot = __ballot(any);
       if (covBallot == 0) continue;
-      frags += (uint64_t)__popcll(covBallot);
+      if (COUNT) frags += (uint64_t)__popcll(covBallot);
 
       bool wr[N];
       uint32_t dz[N];
@@ -391,22 +392,32 @@ __global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P, co
   }
   // Fragment statistics only on counting draws: one plain store per tile
   // (a device-wide atomic per tile serialises across the XCDs).
-  if (P.tileFrags != nullptr && lane == 0) P.tileFrags[tile] = (uint32_t)frags;
+  if (COUNT && lane == 0) P.tileFrags[tile] = (uint32_t)frags;
 }
 
-template <int S, bool VIS>
+template <int S, bool VIS, bool COUNT = false>
 static void launchFineT(const Params& P, hipStream_t stream) {
   const int groups = (P.numTiles + kFineWaves - 1) / kFineWaves;
   const int per = (groups + 7) >> 3;
   const uint4* hdr = reinterpret_cast<const uint4*>(P.triHeader);
   const uint4* td = reinterpret_cast<const uint4*>(P.triData);
-  hipLaunchKernelGGL((fineKernel<S, VIS>), dim3(per * 8), dim3(64 * kFineWaves), 0, stream, P, hdr, td, P.tileList,
+  hipLaunchKernelGGL((fineKernel<S, VIS, COUNT>), dim3(per * 8), dim3(64 * kFineWaves), 0, stream, P, hdr, td, P.tileList,
                      P.vertices, P.subtriTri, P.color, P.depth, P.triid);
 }
 
 // Host-side launcher.  Deferred shading needs the visibility word
 // (slot << 4 | centroid) to hold every slot: slots < 2^28.
 void launchFine(const Params& P, hipStream_t stream) {
+  if (P.tileFrags != nullptr) {
+    // Counting draws run the in-order variant with per-tile totals.
+    switch (P.samplesLog2) {
+      case 0: launchFineT<0, false, true>(P, stream); break;
+      case 1: launchFineT<1, false, true>(P, stream); break;
+      case 2: launchFineT<2, false, true>(P, stream); break;
+      default: launchFineT<3, false, true>(P, stream); break;
+    }
+    return;
+  }
   const bool vis = (P.blend == kBlendReplace || P.blend == kBlendDepthOnly) && P.maxSubtris < (1 << 28);
   switch (P.samplesLog2 * 2 + (vis ? 1 : 0)) {
     case 0: launchFineT<0, false>(P, stream); break;
@@ -420,10 +431,11 @@ void launchFine(const Params& P, hipStream_t stream) {
   }
 }
 
-__global__ void __launch_bounds__(1024) fragmentSumKernel(const uint32_t* tileFrags, int numTiles, Counters* c) {
+// Sums a u32 array into a u64 counter (one workgroup; counting draws only).
+__global__ void __launch_bounds__(1024) sumCountsKernel(const uint32_t* v, int n, uint64_t* out) {
   __shared__ unsigned long long part[16];
   unsigned long long acc = 0;
-  for (int i = threadIdx.x; i < numTiles; i += 1024) acc += tileFrags[i];
+  for (int i = threadIdx.x; i < n; i += 1024) acc += v[i];
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
@@ -431,14 +443,17 @@ __global__ void __launch_bounds__(1024) fragmentSumKernel(const uint32_t* tileFr
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
     for (int w = 0; w < 16; w++) t += part[w];
-    c->fragments = t;
+    *out = t;
   }
 }
 
 void launchFragmentSum(const Params& P, hipStream_t stream) {
-  if (P.tileFrags == nullptr) return;
-  hipLaunchKernelGGL(fragmentSumKernel, dim3(1), dim3(1024), 0, stream, (const uint32_t*)P.tileFrags, P.numTiles,
-                     P.counters);
+  if (P.tileFrags != nullptr)
+    hipLaunchKernelGGL(sumCountsKernel, dim3(1), dim3(1024), 0, stream, (const uint32_t*)P.tileFrags, P.numTiles,
+                       &P.counters->fragments);
+  if (P.chunkSubtris != nullptr && P.numChunks > 0)
+    hipLaunchKernelGGL(sumCountsKernel, dim3(1), dim3(1024), 0, stream, (const uint32_t*)P.chunkSubtris,
+                       P.numChunks, &P.counters->visibleSubtris);
 }
 
 }  // namespace crhip

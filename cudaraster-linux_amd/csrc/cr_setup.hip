@@ -428,6 +428,7 @@ __global__ void __launch_bounds__(kSetupChunk) setupBinKernel(const Params P) {
   uint32_t total;
   const uint32_t pos = blockScan<kSetupChunk>((uint32_t)k, sums, total);
   const uint32_t numWin = (total + kSetupChunk - 1) / kSetupChunk;
+  if (P.chunkSubtris != nullptr && threadIdx.x == 0) P.chunkSubtris[chunk] = total;
 
   // Emit masks of one window: bit `lane` of emitMask[wave][bin] is set when
   // the window entry held by this thread touches the bin.

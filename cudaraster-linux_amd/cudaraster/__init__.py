@@ -59,7 +59,7 @@ class Stats(ctypes.Structure):
 
 class Counters(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in ("num_tris", "num_subtris", "bin_entries", "tile_entries",
-                                              "fragments", "retries")]
+                                              "fragments", "retries", "visible_subtris")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

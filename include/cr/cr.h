@@ -110,6 +110,8 @@ typedef struct cr_counters {
     int64_t fragments;           /* (subtriangle, pixel) pairs with >= 1 covered sample;
                                     counted only by draws with count_fragments set, else 0 */
     int64_t retries;             /* capacity grow-and-retry rounds */
+    int64_t visible_subtris;     /* subtriangles that survive setup's culls;
+                                    counted only by draws with count_fragments set, else 0 */
 } cr_counters;
 
 /* CudaRaster::DebugParams (CudaRaster.hpp:53-67) has per-stage host

@@ -58,6 +58,7 @@ def test_random_tris_all_sample_counts(oracle_lib, samples_log2):
     ref = oracle_lib.render(s)
     assert_parity(got, ref, f"random s={samples_log2}")
     assert got["counters"]["fragments"] == ref["stats"]["fragments"]
+    assert got["counters"]["visible_subtris"] == ref["stats"]["numSubtris"]
 
 
 @pytest.mark.parametrize("blend", [0, 1, 2, 3])
@@ -92,10 +93,12 @@ def test_clip_path(oracle_lib, samples_log2):
     _torch()
     from cudaraster import scenes
     s = scenes.clip_mix(4000, 192, 160, samples_log2=samples_log2, seed=71 + samples_log2)
-    got = render_gpu(s)
+    got = render_gpu(s, debug={"count_fragments": True})
     ref = oracle_lib.render(s)
     assert ref["stats"]["clipped"] > 100 and ref["stats"]["culledFrustum"] > 100
     assert_parity(got, ref, "clip")
+    assert got["counters"]["fragments"] == ref["stats"]["fragments"]
+    assert got["counters"]["visible_subtris"] == ref["stats"]["numSubtris"]
     assert got["counters"]["num_subtris"] > s.num_tris      # clipped fans took extra slots
 
 
