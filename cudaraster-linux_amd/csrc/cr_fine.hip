@@ -11,7 +11,7 @@
 //    visibility, triangle id) in registers for the whole tile, so fragments
 //    of one pixel are applied in order by construction -- no intra-wave
 //    conflict resolution and exactly one coalesced read/write of the tile;
-//  * the tile's triangle list is consumed in chunks of 64: lane j sets up
+//  * the tile's triangle list is consumed in chunks of 32: lane j sets up
 //    triangle j (tile-relative edge thresholds per sample, depth plane per
 //    sample, hierarchical-Z test), survivors are compacted into an LDS
 //    record array, and the wave then walks the records with broadcast LDS

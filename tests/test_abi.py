@@ -64,3 +64,37 @@ def test_null_arguments(lib):
     assert lib.cr_set_surfaces(None, None, None) == cudaraster.CR_ERR_INVALID
     assert lib.cr_context_create(None) == cudaraster.CR_ERR_INVALID
     assert lib.cr_context_destroy(None) == cudaraster.CR_OK
+
+
+def test_cpp_facade_compiles_and_links(tmp_path):
+    """The header-only C++ facade (reference class and method names) builds
+    against cr.h and links against libcr.so with the host compiler."""
+    import shutil
+    import subprocess
+    import cudaraster
+    if shutil.which("g++") is None:
+        pytest.skip("no host C++ compiler")
+    src = tmp_path / "client.cpp"
+    src.write_text(r'''
+#include "cr/CudaRaster.hpp"
+int run(const void* vb, const void* ib, int nv, int nt) {
+  cr::CudaSurface color(1024, 768, cr::CudaSurface::FormatRGBA8, 1);
+  cr::CudaSurface depth(1024, 768, cr::CudaSurface::FormatDepth32, 1);
+  cr::CudaRaster raster;
+  raster.setSurfaces(&color, &depth);
+  raster.setPixelPipe("PixelPipe_gouraud");
+  const float clear[4] = {0.2f, 0.4f, 0.8f, 1.0f};
+  raster.deferredClear(clear, 1.0f);
+  raster.setVertexBuffer(vb, 0, nv);
+  raster.setIndexBuffer(ib, 0, nt);
+  raster.drawTriangles();
+  return (int)raster.getStats().fineTime + (int)color.download().size();
+}
+int main(int argc, char**) { return argc > 5 ? run(nullptr, nullptr, 0, 0) : 0; }
+''')
+    libdir = os.path.dirname(cudaraster.lib_path())
+    exe = tmp_path / "client"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(REPO, "include"), str(src),
+                        "-L", libdir, "-lcr", f"-Wl,-rpath,{libdir}", "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
