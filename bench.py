@@ -107,6 +107,80 @@ def frame_algorithmic_bytes(scene, triid):
     return scene.vertices.nbytes + 12 * scene.num_tris + hr * tw * (12 if triid else 8)
 
 
+def multi_view_ratio(reps=5, triid=True):
+    """Batched multi-view workload: 64 seeded views (256^2..2048^2, 10k-500k
+    triangles, 1x or 4x MSAA).  Times `reps` passes of 64 sequential plain
+    draws (each synchronous, the reference's usage: set surfaces, pipe,
+    buffers, deferredClear, drawTriangles) against `reps` batched draws
+    (autotuned, graph replay, synchronous), after checking that every view's
+    colour, depth and triangle-id surfaces are identical between the two."""
+    import numpy as np
+    import torch
+    import cudaraster
+    from cudaraster import scenes
+    views = scenes.multi_view(64, seed=3)
+    br = cudaraster.BatchRenderer(views, triangle_ids=triid)
+    cr = cudaraster.CudaRaster().init()
+    plain_surf = []
+    for sc in views:
+        n = sc.num_samples
+        plain_surf.append((cudaraster.CudaSurface((sc.width, sc.height), cudaraster.FORMAT_RGBA8, n),
+                           cudaraster.CudaSurface((sc.width, sc.height), cudaraster.FORMAT_DEPTH32, n),
+                           cudaraster.CudaSurface((sc.width, sc.height), cudaraster.FORMAT_TRIANGLE_ID, n)
+                           if triid else None))
+    pipes = [cudaraster.scene_pipe(sc) for sc in views]
+
+    def plain_pass():
+        for k, sc in enumerate(views):
+            col, dep, tid = plain_surf[k]
+            vb, ib = br.buffers[k]
+            cr.setSurfaces(col, dep)
+            cr.setTriangleIdSurface(tid)
+            cr.setPixelPipe(pipes[k])
+            cr.deferredClear(sc.clear_color, sc.clear_depth)
+            cr.setVertexBuffer(vb, 0, sc.vertices.shape[0])
+            cr.setIndexBuffer(ib, 0, sc.num_tris)
+            cr.drawTriangles()
+
+    plain_pass()
+    br.batch.autotune()
+    br.draw()
+    identical = True
+    for k in range(len(views)):
+        got = br.read(k)
+        col, dep, tid = plain_surf[k]
+        identical &= bool(np.array_equal(got["color"], col.download()) and np.array_equal(got["depth"], dep.download()))
+        if triid:
+            identical &= bool(np.array_equal(got["triid"], tid.download()))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain_pass()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / reps
+    gpu_ms = []
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        br.draw()
+        gpu_ms.append(br.batch.info()["last_gpu_ms"])
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    info = br.batch.info()
+    out = {"views": len(views), "triangles": int(sum(s.num_tris for s in views)),
+           "plain_views_per_s": len(views) / t_plain, "batched_views_per_s": len(views) / t_batch,
+           "ratio": t_plain / t_batch, "plain_ms_per_pass": t_plain * 1e3, "batched_ms_per_pass": t_batch * 1e3,
+           "batched_gpu_ms": statistics.median(gpu_ms), "identical_images": identical,
+           "fine_tiles_per_group": info["fine_tiles_per_group"], "uses_graph": bool(info["uses_graph"]),
+           "reserved_bytes": info["reserved_bytes"]}
+    br.close()
+    cr.close()
+    for trio in plain_surf:
+        for s in trio:
+            if s is not None:
+                s.close()
+    return out
+
+
 def cpu_baseline(scene, runs=5):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
@@ -134,6 +208,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true",
                     help="skip the counting draw (fragment and visible-subtriangle totals)")
+    ap.add_argument("--no-multi-view", action="store_true",
+                    help="skip the batched/plain multi-view ratio (64 views)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB copy kernel (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -221,6 +297,10 @@ def main():
         "counters": counters,
     }
     r.close()
+    if not args.no_multi_view:
+        mv = multi_view_ratio(triid=triid)
+        line["batched_over_plain_views_ratio"] = mv["ratio"]
+        line["multi_view"] = mv
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene)
     print(json.dumps(line), flush=True)

@@ -18,78 +18,13 @@
 
 #include "cr/cr.h"
 #include "cr_device.h"
+#include "cr_host.h"
 
 using namespace crhip;
 
-struct cr_surface {
-  int32_t width = 0, height = 0, format = 0, samples = 1, samplesLog2 = 0;
-  int32_t roundedW = 0, roundedH = 0, textureW = 0;
-  uint32_t* dev = nullptr;
-  size_t bytes = 0;
-  int device = 0;
-};
-
-namespace {
-
-struct DevBuf {
-  void* ptr = nullptr;
-  size_t bytes = 0;
-  hipError_t ensure(size_t need) {
-    if (need <= bytes && ptr) return hipSuccess;
-    if (ptr) {
-      hipError_t e = hipFree(ptr);
-      if (e != hipSuccess) return e;
-      ptr = nullptr;
-      bytes = 0;
-    }
-    size_t alloc = std::max<size_t>(need, 256);
-    hipError_t e = hipMalloc(&ptr, alloc);
-    if (e == hipSuccess) bytes = alloc;
-    return e;
-  }
-  void release() {
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    bytes = 0;
-  }
-  template <class T> T* as() const { return static_cast<T*>(ptr); }
-};
+namespace crhost {
 
 thread_local std::string g_lastError;
-
-}  // namespace
-
-struct cr_context {
-  std::string err;
-  int device = 0;
-  cr_surface* color = nullptr;
-  cr_surface* depth = nullptr;
-  cr_surface* triid = nullptr;
-  bool deferredClear = false;
-  uint32_t clearColor = 0, clearDepth = 0;
-  bool pipeSet = false;
-  cr_pipe_desc pipe{};
-  const uint8_t* vbuf = nullptr;
-  int64_t vofs = 0, numVerts = 0;
-  const uint8_t* ibuf = nullptr;
-  int64_t iofs = 0;
-  int32_t numTris = 0;
-  cr_debug_params dbg{};
-
-  int64_t maxSubtris = 1, maxBinEntries = 1, maxTileEntries = 1;
-  DevBuf triSubtris, triHeader, triData, subtriTri;
-  DevBuf chunkBase, binCnt, binOff, binList;
-  DevBuf tileStart, tileCount, tileList, tileFrags, chunkSubtris;
-  DevBuf counters;
-  Counters* hostCounters = nullptr;   // pinned
-  hipEvent_t ev[5] = {};
-  bool evRecorded = false;
-  hipStream_t pendingStream = nullptr;
-  bool pending = false;
-  cr_counters last{};
-};
-
-namespace {
 
 cr_status fail(cr_context* ctx, cr_status st, const char* fmt, ...) {
   char buf[512];
@@ -102,11 +37,7 @@ cr_status fail(cr_context* ctx, cr_status st, const char* fmt, ...) {
   return st;
 }
 
-#define CR_HIP(ctx, call)                                                                      \
-  do {                                                                                         \
-    hipError_t e_ = (call);                                                                    \
-    if (e_ != hipSuccess) return fail((ctx), CR_ERR_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
-  } while (0)
+namespace {
 
 int popc8(int v) {
   int c = 0;
@@ -134,24 +65,41 @@ uint32_t encodeDepth(uint32_t depth) {
   return (uint32_t)v;
 }
 
-struct Geometry {
-  int32_t vw, vh, wp, hp, wt, ht, nt, wb, hb, nb;
-};
+}  // namespace
 
-Geometry geometryOf(const cr_surface* s) {
-  Geometry g;
-  g.vw = s->width;
-  g.vh = s->height;
-  g.wp = s->roundedW;
-  g.hp = s->roundedH;
-  g.wt = g.wp >> kTileLog2;
-  g.ht = g.hp >> kTileLog2;
-  g.nt = g.wt * g.ht;
-  g.wb = (g.wt + (1 << kBinLog2) - 1) >> kBinLog2;
-  g.hb = (g.ht + (1 << kBinLog2) - 1) >> kBinLog2;
-  g.nb = g.wb * g.hb;
-  return g;
+void clearValues(const float rgba[4], float depth, uint32_t* color, uint32_t* encodedDepth) {
+  *color = hostChannel(rgba[0]) | (hostChannel(rgba[1]) << 8) | (hostChannel(rgba[2]) << 16) |
+           (hostChannel(rgba[3]) << 24);
+  double d = (double)depth * 4294967296.0;
+  uint64_t q;
+  if (!(d > 0.0)) q = 0;
+  else if (d >= 18446744073709551616.0) q = UINT64_MAX;
+  else q = (uint64_t)d;
+  *encodedDepth = encodeDepth((uint32_t)std::min<uint64_t>(q, 0xFFFFFFFFull));
 }
+
+cr_status validatePipe(cr_context* ctx, const cr_pipe_desc* p) {
+  if (p->samples_log2 < 0 || p->samples_log2 > 3) return fail(ctx, CR_ERR_INVALID, "pipe: samples_log2 must be 0..3");
+  if (p->vertex_struct_size < 16 || (p->vertex_struct_size & 15))
+    return fail(ctx, CR_ERR_INVALID, "pipe: vertex_struct_size must be a positive multiple of 16 (Vec4f records)");
+  if (p->fragment_shader != CR_SHADER_GOURAUD && p->fragment_shader != CR_SHADER_CONSTANT)
+    return fail(ctx, CR_ERR_INVALID, "pipe: unknown fragment shader %d", p->fragment_shader);
+  if (p->fragment_shader == CR_SHADER_GOURAUD && p->vertex_struct_size < 32)
+    return fail(ctx, CR_ERR_INVALID, "pipe: GouraudShader needs a colour varying (vertex_struct_size >= 32)");
+  if (p->blend_shader < CR_BLEND_REPLACE || p->blend_shader > CR_BLEND_DEPTH_ONLY)
+    return fail(ctx, CR_ERR_INVALID, "pipe: unknown blend shader %d", p->blend_shader);
+  if (p->render_mode_flags & CR_RENDER_QUADS)
+    return fail(ctx, CR_ERR_UNSUPPORTED, "pipe: RenderModeFlag_EnableQuads is not supported");
+  if (p->render_mode_flags & ~(uint32_t)(CR_RENDER_DEPTH | CR_RENDER_LERP | CR_RENDER_QUADS))
+    return fail(ctx, CR_ERR_INVALID, "pipe: unknown render mode flags 0x%x", p->render_mode_flags);
+  return CR_OK;
+}
+
+}  // namespace crhost
+
+using namespace crhost;
+
+namespace {
 
 cr_status validateDraw(cr_context* ctx) {
   if (!ctx->color) return fail(ctx, CR_ERR_STATE, "CudaRaster: Surfaces not set!");
@@ -254,19 +202,6 @@ cr_status launchAll(cr_context* ctx, const Params& P, hipStream_t stream) {
   return CR_OK;
 }
 
-// Slack added when growing, as the reference (CudaRaster.cpp:239-241).
-constexpr int64_t kSubtrisSlack = 4096;
-constexpr int64_t kBinSlack = 256 * 512;
-constexpr int64_t kTileSlack = 4096 * 32;
-constexpr int64_t kMaxEntries = (int64_t)INT32_MAX - 1;
-constexpr int32_t kMaxTris = 1 << 28;
-
-void initialCapacities(cr_context* ctx, const Geometry& g) {
-  int64_t nt = ctx->numTris;
-  ctx->maxSubtris = std::max(ctx->maxSubtris, nt + kSubtrisSlack);
-  ctx->maxBinEntries = std::max(ctx->maxBinEntries, nt + (int64_t)g.nb * 64 + kBinSlack);
-  ctx->maxTileEntries = std::max(ctx->maxTileEntries, 2 * nt + (int64_t)g.nt + kTileSlack);
-}
 
 cr_status finishDraw(cr_context* ctx, bool* overflow) {
   const Counters& c = *ctx->hostCounters;
@@ -296,7 +231,7 @@ cr_status drawImpl(cr_context* ctx, hipStream_t stream, bool sync) {
   }
   Geometry g = geometryOf(ctx->color);
   int32_t numChunks = (ctx->numTris + kSetupChunkTris - 1) / kSetupChunkTris;
-  initialCapacities(ctx, g);
+  initialCapacities(ctx->numTris, g, ctx->maxSubtris, ctx->maxBinEntries, ctx->maxTileEntries);
   ctx->last.retries = 0;
   for (;;) {
     st = ensureBuffers(ctx, g, numChunks);
@@ -479,14 +414,7 @@ cr_status cr_deferred_clear(cr_context* ctx, const float rgba[4], float depth) {
   if (!ctx || !rgba) return fail(ctx, CR_ERR_INVALID, "cr_deferred_clear: NULL argument");
   // CudaRaster.cpp:174-179
   ctx->deferredClear = true;
-  ctx->clearColor = hostChannel(rgba[0]) | (hostChannel(rgba[1]) << 8) | (hostChannel(rgba[2]) << 16) |
-                    (hostChannel(rgba[3]) << 24);
-  double d = (double)depth * 4294967296.0;
-  uint64_t q;
-  if (!(d > 0.0)) q = 0;
-  else if (d >= 18446744073709551616.0) q = UINT64_MAX;
-  else q = (uint64_t)d;
-  ctx->clearDepth = encodeDepth((uint32_t)std::min<uint64_t>(q, 0xFFFFFFFFull));
+  clearValues(rgba, depth, &ctx->clearColor, &ctx->clearDepth);
   return CR_OK;
 }
 
@@ -496,19 +424,8 @@ cr_status cr_set_pixel_pipe(cr_context* ctx, const cr_pipe_desc* p) {
     ctx->pipeSet = false;
     return CR_OK;
   }
-  if (p->samples_log2 < 0 || p->samples_log2 > 3) return fail(ctx, CR_ERR_INVALID, "pipe: samples_log2 must be 0..3");
-  if (p->vertex_struct_size < 16 || (p->vertex_struct_size & 15))
-    return fail(ctx, CR_ERR_INVALID, "pipe: vertex_struct_size must be a positive multiple of 16 (Vec4f records)");
-  if (p->fragment_shader != CR_SHADER_GOURAUD && p->fragment_shader != CR_SHADER_CONSTANT)
-    return fail(ctx, CR_ERR_INVALID, "pipe: unknown fragment shader %d", p->fragment_shader);
-  if (p->fragment_shader == CR_SHADER_GOURAUD && p->vertex_struct_size < 32)
-    return fail(ctx, CR_ERR_INVALID, "pipe: GouraudShader needs a colour varying (vertex_struct_size >= 32)");
-  if (p->blend_shader < CR_BLEND_REPLACE || p->blend_shader > CR_BLEND_DEPTH_ONLY)
-    return fail(ctx, CR_ERR_INVALID, "pipe: unknown blend shader %d", p->blend_shader);
-  if (p->render_mode_flags & CR_RENDER_QUADS)
-    return fail(ctx, CR_ERR_UNSUPPORTED, "pipe: RenderModeFlag_EnableQuads is not supported");
-  if (p->render_mode_flags & ~(uint32_t)(CR_RENDER_DEPTH | CR_RENDER_LERP | CR_RENDER_QUADS))
-    return fail(ctx, CR_ERR_INVALID, "pipe: unknown render mode flags 0x%x", p->render_mode_flags);
+  cr_status st = validatePipe(ctx, p);
+  if (st != CR_OK) return st;
   ctx->pipe = *p;
   ctx->pipeSet = true;
   return CR_OK;

@@ -166,7 +166,8 @@ __device__ __forceinline__ void coarseRecord(const TileTest& r, bool valid, uint
   }
 }
 
-__global__ void __launch_bounds__(kCoarseThreads) coarseKernel(const Params P) {
+// Coarse raster of bin `bin` of one draw (one 1024-lane workgroup).
+__device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   __shared__ uint32_t segStart[kSegBatch + 1];
   __shared__ uint32_t segAddr[kSegBatch];
   __shared__ uint32_t cnt[kBinTiles];
@@ -177,7 +178,6 @@ __global__ void __launch_bounds__(kCoarseThreads) coarseKernel(const Params P) {
   __shared__ uint32_t sBase;
 
   if (P.counters->overflow & 3u) return;
-  const int bin = blockIdx.x;
   const int bx = bin % P.widthBins, by = bin / P.widthBins;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t* bCnt = P.binCnt + (size_t)bin * P.numChunks;
@@ -290,12 +290,42 @@ __global__ void __launch_bounds__(kCoarseThreads) coarseKernel(const Params P) {
   }
 }
 
+__global__ void __launch_bounds__(kCoarseThreads) coarseKernel(const Params P) { coarseBody(P, blockIdx.x); }
+
+__global__ void __launch_bounds__(kCoarseThreads) coarseBatchKernel(const Params* __restrict__ views,
+                                                                    const int32_t* __restrict__ blockStart,
+                                                                    int numViews) {
+  const int v = findView(blockStart, numViews, (int)blockIdx.x);
+  coarseBody(views[v], (int)blockIdx.x - blockStart[v]);
+}
+
+// One workgroup per view: clears the view's counter block.
+__global__ void __launch_bounds__(64) resetBatchKernel(const Params* __restrict__ views) {
+  const Params& P = views[blockIdx.x];
+  uint32_t* w = reinterpret_cast<uint32_t*>(P.counters);
+  for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 4); i += 64) w[i] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) P.counters->numSubtris = (uint32_t)P.numTris;
+}
+
 void launchReset(const Params& P, hipStream_t stream) {
   hipLaunchKernelGGL(resetCountersKernel, dim3(1), dim3(64), 0, stream, P.counters, (uint32_t)P.numTris);
 }
 
 void launchCoarse(const Params& P, hipStream_t stream) {
   hipLaunchKernelGGL(coarseKernel, dim3(P.numBins), dim3(kCoarseThreads), 0, stream, P);
+}
+
+void launchResetBatch(const Params* views, int numViews, hipStream_t stream) {
+  if (numViews <= 0) return;
+  hipLaunchKernelGGL(resetBatchKernel, dim3(numViews), dim3(64), 0, stream, views);
+}
+
+void launchCoarseBatch(const Params* views, const int32_t* blockStart, int numViews, int totalBlocks,
+                       hipStream_t stream) {
+  if (totalBlocks <= 0) return;
+  hipLaunchKernelGGL(coarseBatchKernel, dim3(totalBlocks), dim3(kCoarseThreads), 0, stream, views, blockStart,
+                     numViews);
 }
 
 }  // namespace crhip

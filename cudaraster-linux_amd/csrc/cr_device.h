@@ -215,4 +215,29 @@ void launchFine(const Params& P, hipStream_t stream);
 void launchFragmentSum(const Params& P, hipStream_t stream);
 constexpr int kSetupChunkTris = 1024;
 
+// Batched draws: `views` is a device array of per-view Params, each stage
+// one launch over the workgroups of all views.
+void launchResetBatch(const Params* views, int numViews, hipStream_t stream);
+void launchSetupBatch(const Params* views, const int32_t* blockStart, int numViews, int totalBlocks,
+                      hipStream_t stream);
+void launchCoarseBatch(const Params* views, const int32_t* blockStart, int numViews, int totalBlocks,
+                       hipStream_t stream);
+// Fine-stage variant of a draw: samplesLog2 * 2 + (deferred shading ? 1 : 0).
+int fineVariant(const Params& P);
+void launchFineBatch(int variant, int tilesPerGroup, const Params* views, const int32_t* groupStart, int numViews,
+                     int totalGroups, hipStream_t stream);
+constexpr int kFineBatchShapes[3] = {2, 4, 8};   // tiles (waves) per workgroup the batch path can use
+
+// Largest v with start[v] <= b: the view owning workgroup (or group) b.
+// start has numViews + 1 nondecreasing entries; views without work have
+// start[v] == start[v + 1] and are never returned for b < start[numViews].
+__device__ __forceinline__ int findView(const int32_t* __restrict__ start, int numViews, int b) {
+  int lo = 0, hi = numViews - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 }  // namespace crhip

@@ -119,33 +119,34 @@ __device__ __forceinline__ void waveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
+// workgroup b runs on XCD b % 8; group (b % 8) * per + b / 8 gives each XCD
+// one contiguous band of tile groups, and triangles shared by neighbouring
+// tiles stay in one L2.
+__device__ __forceinline__ int xcdGroup(int groups) {
+  const int per = (groups + 7) >> 3;
+  return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+
+// Fine raster of tile group `group` (W tiles, one wave each) of one draw.
 // COUNT: counting draw (fragment totals per tile into P.tileFrags).
-template <int S, bool VIS, bool COUNT>
-__global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P, const uint4* __restrict__ hdr,
-                                                              const uint4* __restrict__ tdata,
-                                                              const uint32_t* __restrict__ tileList,
-                                                              const uint8_t* __restrict__ verts,
-                                                              const uint32_t* __restrict__ subtriTri,
-                                                              uint32_t* __restrict__ color,
-                                                              uint32_t* __restrict__ depth,
-                                                              uint32_t* __restrict__ triid) {
+template <int S, bool VIS, bool COUNT, int W>
+__device__ __forceinline__ void fineBody(const Params& P, const int group, const uint4* __restrict__ hdr,
+                                         const uint4* __restrict__ tdata, const uint32_t* __restrict__ tileList,
+                                         const uint8_t* __restrict__ verts, const uint32_t* __restrict__ subtriTri,
+                                         uint32_t* __restrict__ color, uint32_t* __restrict__ depth,
+                                         uint32_t* __restrict__ triid) {
   constexpr int N = 1 << S;
   using L = Rec<S>;
   constexpr int RS = VIS ? L::SIZE_VIS : L::SIZE_FULL;
-  __shared__ __attribute__((aligned(16))) uint32_t recAll[kFineWaves * kChunk * RS];
+  __shared__ __attribute__((aligned(16))) uint32_t recAll[W * kChunk * RS];
 
   if (P.counters->overflow) return;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   uint32_t* rec = recAll + wave * (kChunk * RS);
-  // XCD-aware order: workgroups go round-robin to the 8 XCDs; each XCD gets
-  // one contiguous band of tiles so triangles shared by neighbouring tiles
-  // stay in one L2.
-  const int groups = (P.numTiles + kFineWaves - 1) / kFineWaves;
-  const int per = (groups + 7) >> 3;
-  const int group = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-  const int tile = group * kFineWaves + wave;
-  if (group >= groups || tile >= P.numTiles) return;
+  const int tile = group * W + wave;
+  if (tile >= P.numTiles) return;
   const uint32_t cnt = P.tileCount[tile];
   if (cnt == 0 && !P.deferredClear) {
     if (COUNT && lane == 0) P.tileFrags[tile] = 0;
@@ -395,14 +396,76 @@ __global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P, co
   if (COUNT && lane == 0) P.tileFrags[tile] = (uint32_t)frags;
 }
 
+template <int S, bool VIS, bool COUNT>
+__global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P) {
+  const int groups = (P.numTiles + kFineWaves - 1) / kFineWaves;
+  const int group = xcdGroup(groups);
+  if (group >= groups) return;
+  fineBody<S, VIS, COUNT, kFineWaves>(P, group, reinterpret_cast<const uint4*>(P.triHeader),
+                                      reinterpret_cast<const uint4*>(P.triData), P.tileList, P.vertices,
+                                      P.subtriTri, P.color, P.depth, P.triid);
+}
+
+// Batched draws: the tile groups of every view of one pipe variant form one
+// grid (groupStart[v] = first group of view v, numViews + 1 entries; views of
+// other variants own no groups).
+template <int S, bool VIS, int W>
+__global__ void __launch_bounds__(64 * W) fineBatchKernel(const Params* __restrict__ views,
+                                                          const int32_t* __restrict__ groupStart, int numViews,
+                                                          int totalGroups) {
+  const int g = xcdGroup(totalGroups);
+  if (g >= totalGroups) return;
+  const int v = findView(groupStart, numViews, g);
+  const Params& P = views[v];
+  fineBody<S, VIS, false, W>(P, g - groupStart[v], reinterpret_cast<const uint4*>(P.triHeader),
+                             reinterpret_cast<const uint4*>(P.triData), P.tileList, P.vertices, P.subtriTri,
+                             P.color, P.depth, P.triid);
+}
+
 template <int S, bool VIS, bool COUNT = false>
 static void launchFineT(const Params& P, hipStream_t stream) {
   const int groups = (P.numTiles + kFineWaves - 1) / kFineWaves;
   const int per = (groups + 7) >> 3;
-  const uint4* hdr = reinterpret_cast<const uint4*>(P.triHeader);
-  const uint4* td = reinterpret_cast<const uint4*>(P.triData);
-  hipLaunchKernelGGL((fineKernel<S, VIS, COUNT>), dim3(per * 8), dim3(64 * kFineWaves), 0, stream, P, hdr, td, P.tileList,
-                     P.vertices, P.subtriTri, P.color, P.depth, P.triid);
+  hipLaunchKernelGGL((fineKernel<S, VIS, COUNT>), dim3(per * 8), dim3(64 * kFineWaves), 0, stream, P);
+}
+
+template <int S, bool VIS, int W>
+static void launchFineBatchT(const Params* views, const int32_t* groupStart, int numViews, int totalGroups,
+                             hipStream_t stream) {
+  const int per = (totalGroups + 7) >> 3;
+  hipLaunchKernelGGL((fineBatchKernel<S, VIS, W>), dim3(per * 8), dim3(64 * W), 0, stream, views, groupStart,
+                     numViews, totalGroups);
+}
+
+template <int S, bool VIS>
+static void launchFineBatchW(int W, const Params* views, const int32_t* groupStart, int numViews, int totalGroups,
+                             hipStream_t stream) {
+  switch (W) {
+    case 2: launchFineBatchT<S, VIS, 2>(views, groupStart, numViews, totalGroups, stream); break;
+    case 8: launchFineBatchT<S, VIS, 8>(views, groupStart, numViews, totalGroups, stream); break;
+    default: launchFineBatchT<S, VIS, 4>(views, groupStart, numViews, totalGroups, stream); break;
+  }
+}
+
+int fineVariant(const Params& P) {
+  const bool vis = (P.blend == kBlendReplace || P.blend == kBlendDepthOnly) && P.maxSubtris < (1 << 28);
+  return P.samplesLog2 * 2 + (vis ? 1 : 0);
+}
+
+void launchFineBatch(int variant, int tilesPerGroup, const Params* views, const int32_t* groupStart, int numViews,
+                     int totalGroups, hipStream_t stream) {
+  if (totalGroups <= 0) return;
+  const int W = tilesPerGroup;
+  switch (variant) {
+    case 0: launchFineBatchW<0, false>(W, views, groupStart, numViews, totalGroups, stream); break;
+    case 1: launchFineBatchW<0, true>(W, views, groupStart, numViews, totalGroups, stream); break;
+    case 2: launchFineBatchW<1, false>(W, views, groupStart, numViews, totalGroups, stream); break;
+    case 3: launchFineBatchW<1, true>(W, views, groupStart, numViews, totalGroups, stream); break;
+    case 4: launchFineBatchW<2, false>(W, views, groupStart, numViews, totalGroups, stream); break;
+    case 5: launchFineBatchW<2, true>(W, views, groupStart, numViews, totalGroups, stream); break;
+    case 6: launchFineBatchW<3, false>(W, views, groupStart, numViews, totalGroups, stream); break;
+    default: launchFineBatchW<3, true>(W, views, groupStart, numViews, totalGroups, stream); break;
+  }
 }
 
 // Host-side launcher.  Deferred shading needs the visibility word
@@ -418,8 +481,7 @@ void launchFine(const Params& P, hipStream_t stream) {
     }
     return;
   }
-  const bool vis = (P.blend == kBlendReplace || P.blend == kBlendDepthOnly) && P.maxSubtris < (1 << 28);
-  switch (P.samplesLog2 * 2 + (vis ? 1 : 0)) {
+  switch (fineVariant(P)) {
     case 0: launchFineT<0, false>(P, stream); break;
     case 1: launchFineT<0, true>(P, stream); break;
     case 2: launchFineT<1, false>(P, stream); break;

@@ -1,0 +1,131 @@
+// Host-side pieces shared by the C ABI implementation (cr_api.hip) and the
+// batched-draw runtime (cr_batch.hip): surface and context objects, device
+// buffers, error reporting, the reference's host quantisers and the draw
+// geometry.  Internal to libcr.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+
+#include "cr/cr.h"
+#include "cr_device.h"
+
+struct cr_surface {
+  int32_t width = 0, height = 0, format = 0, samples = 1, samplesLog2 = 0;
+  int32_t roundedW = 0, roundedH = 0, textureW = 0;
+  uint32_t* dev = nullptr;
+  size_t bytes = 0;
+  int device = 0;
+};
+
+namespace crhost {
+
+// A device allocation that only grows.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t need) {
+    if (need <= bytes && ptr) return hipSuccess;
+    if (ptr) {
+      hipError_t e = hipFree(ptr);
+      if (e != hipSuccess) return e;
+      ptr = nullptr;
+      bytes = 0;
+    }
+    size_t alloc = std::max<size_t>(need, 256);
+    hipError_t e = hipMalloc(&ptr, alloc);
+    if (e == hipSuccess) bytes = alloc;
+    return e;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct Geometry {
+  int32_t vw, vh, wp, hp, wt, ht, nt, wb, hb, nb;
+};
+
+inline Geometry geometryOf(const cr_surface* s) {
+  Geometry g;
+  g.vw = s->width;
+  g.vh = s->height;
+  g.wp = s->roundedW;
+  g.hp = s->roundedH;
+  g.wt = g.wp >> crhip::kTileLog2;
+  g.ht = g.hp >> crhip::kTileLog2;
+  g.nt = g.wt * g.ht;
+  g.wb = (g.wt + (1 << crhip::kBinLog2) - 1) >> crhip::kBinLog2;
+  g.hb = (g.ht + (1 << crhip::kBinLog2) - 1) >> crhip::kBinLog2;
+  g.nb = g.wb * g.hb;
+  return g;
+}
+
+// Slack added when growing, as the reference (CudaRaster.cpp:239-241).
+constexpr int64_t kSubtrisSlack = 4096;
+constexpr int64_t kBinSlack = 256 * 512;
+constexpr int64_t kTileSlack = 4096 * 32;
+constexpr int64_t kMaxEntries = (int64_t)INT32_MAX - 1;
+constexpr int32_t kMaxTris = 1 << 28;
+
+// First-draw capacities of a draw of nt triangles over geometry g.
+inline void initialCapacities(int64_t nt, const Geometry& g, int64_t& subtris, int64_t& bin, int64_t& tile) {
+  subtris = std::max(subtris, nt + kSubtrisSlack);
+  bin = std::max(bin, nt + (int64_t)g.nb * 64 + kBinSlack);
+  tile = std::max(tile, 2 * nt + (int64_t)g.nt + kTileSlack);
+}
+
+// Records `fmt` as the context's (and the thread's) last error; returns st.
+cr_status fail(cr_context* ctx, cr_status st, const char* fmt, ...);
+
+// Vec4f::toABGR of the clear colour and the encoded clear depth
+// (CudaRaster.cpp:174-179, base/Math.cpp:41-48, cuda/Util.hpp:290-298).
+void clearValues(const float rgba[4], float depth, uint32_t* color, uint32_t* encodedDepth);
+
+// setPixelPipe's checks of a pipe descriptor (CudaRaster.cpp:183-216).
+cr_status validatePipe(cr_context* ctx, const cr_pipe_desc* p);
+
+}  // namespace crhost
+
+#define CR_HIP(ctx, call)                                                                      \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return crhost::fail((ctx), CR_ERR_DEVICE, "%s: %s", #call, hipGetErrorString(e_));       \
+  } while (0)
+
+struct cr_context {
+  std::string err;
+  int device = 0;
+  cr_surface* color = nullptr;
+  cr_surface* depth = nullptr;
+  cr_surface* triid = nullptr;
+  bool deferredClear = false;
+  uint32_t clearColor = 0, clearDepth = 0;
+  bool pipeSet = false;
+  cr_pipe_desc pipe{};
+  const uint8_t* vbuf = nullptr;
+  int64_t vofs = 0, numVerts = 0;
+  const uint8_t* ibuf = nullptr;
+  int64_t iofs = 0;
+  int32_t numTris = 0;
+  cr_debug_params dbg{};
+
+  int64_t maxSubtris = 1, maxBinEntries = 1, maxTileEntries = 1;
+  crhost::DevBuf triSubtris, triHeader, triData, subtriTri;
+  crhost::DevBuf chunkBase, binCnt, binOff, binList;
+  crhost::DevBuf tileStart, tileCount, tileList, tileFrags, chunkSubtris;
+  crhost::DevBuf counters;
+  crhip::Counters* hostCounters = nullptr;   // pinned
+  hipEvent_t ev[5] = {};
+  bool evRecorded = false;
+  hipStream_t pendingStream = nullptr;
+  bool pending = false;
+  cr_counters last{};
+};

@@ -404,7 +404,8 @@ __device__ __forceinline__ uint32_t blockScan(uint32_t v, uint32_t* sums, uint32
   return r;
 }
 
-__global__ void __launch_bounds__(kSetupChunk) setupBinKernel(const Params P) {
+// Setup and binning of chunk `chunk` of one draw (one 1024-lane workgroup).
+__device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
   __shared__ uint32_t cnt[kMaxBins];
   __shared__ uint32_t run[kMaxBins];
   __shared__ uint64_t emitMask[kSetupWaves][kMaxBins];
@@ -413,7 +414,6 @@ __global__ void __launch_bounds__(kSetupChunk) setupBinKernel(const Params P) {
   __shared__ uint32_t sums[kSetupWaves + 1];
   __shared__ uint32_t sBase;
 
-  const int chunk = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int t = chunk * kSetupChunk + threadIdx.x;
 
@@ -525,9 +525,27 @@ __global__ void __launch_bounds__(kSetupChunk) setupBinKernel(const Params P) {
   }
 }
 
+__global__ void __launch_bounds__(kSetupChunk) setupBinKernel(const Params P) { setupBinBody(P, blockIdx.x); }
+
+// Batched draws: one workgroup per chunk of every view; blockStart[v] is the
+// first workgroup of view v (numViews + 1 entries).
+__global__ void __launch_bounds__(kSetupChunk) setupBinBatchKernel(const Params* __restrict__ views,
+                                                                   const int32_t* __restrict__ blockStart,
+                                                                   int numViews) {
+  const int v = findView(blockStart, numViews, (int)blockIdx.x);
+  setupBinBody(views[v], (int)blockIdx.x - blockStart[v]);
+}
+
 void launchSetup(const Params& P, hipStream_t stream) {
   if (P.numTris <= 0) return;
   hipLaunchKernelGGL(setupBinKernel, dim3(P.numChunks), dim3(kSetupChunk), 0, stream, P);
+}
+
+void launchSetupBatch(const Params* views, const int32_t* blockStart, int numViews, int totalBlocks,
+                      hipStream_t stream) {
+  if (totalBlocks <= 0) return;
+  hipLaunchKernelGGL(setupBinBatchKernel, dim3(totalBlocks), dim3(kSetupChunk), 0, stream, views, blockStart,
+                     numViews);
 }
 
 }  // namespace crhip

@@ -37,6 +37,8 @@ EXPORTS = [
     "cr_pipe_lookup", "cr_set_vertex_buffer", "cr_set_index_buffer",
     "cr_draw_triangles", "cr_draw_triangles_async", "cr_draw_check", "cr_reserve",
     "cr_get_stats", "cr_get_counters", "cr_get_profiling_info", "cr_set_debug_params", "cr_abi_version",
+    "cr_batch_create", "cr_batch_destroy", "cr_batch_draw", "cr_batch_autotune", "cr_batch_reserve",
+    "cr_batch_use_graph", "cr_batch_get_info", "cr_batch_get_view_counters",
 ]
 
 
@@ -67,6 +69,25 @@ class Counters(ctypes.Structure):
 
 class DebugParams(ctypes.Structure):
     _fields_ = [("disable_hiz", ctypes.c_int32), ("count_fragments", ctypes.c_int32)]
+
+
+class View(ctypes.Structure):
+    """cr_view: one view of a batched draw."""
+    _fields_ = [("color", ctypes.c_void_p), ("depth", ctypes.c_void_p), ("tri_id", ctypes.c_void_p),
+                ("vertices", ctypes.c_void_p), ("num_vertices", ctypes.c_int64),
+                ("indices", ctypes.c_void_p), ("num_tris", ctypes.c_int32), ("deferred_clear", ctypes.c_int32),
+                ("pipe", PipeDesc), ("clear_color", ctypes.c_float * 4), ("clear_depth", ctypes.c_float)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("num_views", ctypes.c_int32), ("fine_tiles_per_group", ctypes.c_int32),
+                ("uses_graph", ctypes.c_int32), ("tuned", ctypes.c_int32), ("retries", ctypes.c_int64),
+                ("reserved_bytes", ctypes.c_int64), ("last_gpu_ms", ctypes.c_float),
+                ("tuned_gpu_ms", ctypes.c_float)]
+
+    def as_dict(self):
+        return {n: (float(getattr(self, n)) if t is ctypes.c_float else int(getattr(self, n)))
+                for n, t in self._fields_}
 
 
 _lib = None
@@ -115,6 +136,14 @@ def load(path: str | None = None):
         "cr_get_counters": [vp, ctypes.POINTER(Counters)],
         "cr_get_profiling_info": [vp, ctypes.c_char_p, ctypes.c_size_t],
         "cr_set_debug_params": [vp, ctypes.POINTER(DebugParams)],
+        "cr_batch_create": [vp, ctypes.POINTER(View), i32, ctypes.POINTER(vp)],
+        "cr_batch_destroy": [vp],
+        "cr_batch_draw": [vp, vp],
+        "cr_batch_autotune": [vp, vp],
+        "cr_batch_reserve": [vp, i32, i64, i64, i64],
+        "cr_batch_use_graph": [vp, i32],
+        "cr_batch_get_info": [vp, ctypes.POINTER(BatchInfo)],
+        "cr_batch_get_view_counters": [vp, i32, ctypes.POINTER(Counters)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -376,3 +405,99 @@ class SceneRenderer:
         for s in (self.color, self.depth, self.triid):
             if s is not None:
                 s.close()
+
+
+def _stream_handle(stream):
+    return None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
+
+
+class Batch:
+    """A batched multi-view draw (cr_batch_*): every view's surfaces, buffers
+    and pipe are fixed at creation; draw() renders all of them, with one
+    launch per stage over all views, replayed from a captured hipGraph."""
+
+    def __init__(self, raster, views, keep=()):
+        self._raster = raster
+        self._keep = list(keep)
+        arr = (View * len(views))(*views)
+        h = ctypes.c_void_p()
+        _check(load().cr_batch_create(raster.ctx, arr, len(views), ctypes.byref(h)), raster.ctx)
+        self._h = h
+
+    def draw(self, stream=None):
+        _check(load().cr_batch_draw(self._h, _stream_handle(stream)), self._raster.ctx)
+
+    def autotune(self, stream=None):
+        _check(load().cr_batch_autotune(self._h, _stream_handle(stream)), self._raster.ctx)
+
+    def reserve(self, view, max_subtris, max_bin_entries, max_tile_entries):
+        _check(load().cr_batch_reserve(self._h, int(view), int(max_subtris), int(max_bin_entries),
+                                       int(max_tile_entries)), self._raster.ctx)
+
+    def use_graph(self, enable=True):
+        _check(load().cr_batch_use_graph(self._h, 1 if enable else 0), self._raster.ctx)
+
+    def info(self) -> dict:
+        i = BatchInfo()
+        _check(load().cr_batch_get_info(self._h, ctypes.byref(i)), self._raster.ctx)
+        return i.as_dict()
+
+    def view_counters(self, view) -> dict:
+        c = Counters()
+        _check(load().cr_batch_get_view_counters(self._h, int(view), ctypes.byref(c)), self._raster.ctx)
+        return c.as_dict()
+
+    def close(self):
+        if self._h:
+            load().cr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchRenderer:
+    """All scenes of a list as one batch: per scene its own surfaces, device
+    buffers and pipe (what a SceneRenderer per scene would hold)."""
+
+    def __init__(self, scene_list, triangle_ids=True, device="cuda"):
+        self.scenes = list(scene_list)
+        self.cr = CudaRaster().init()
+        self.surfaces, self.buffers, views = [], [], []
+        for sc in self.scenes:
+            n = sc.num_samples
+            col = CudaSurface((sc.width, sc.height), FORMAT_RGBA8, n)
+            dep = CudaSurface((sc.width, sc.height), FORMAT_DEPTH32, n)
+            tid = CudaSurface((sc.width, sc.height), FORMAT_TRIANGLE_ID, n) if triangle_ids else None
+            vb, ib = device_buffers(sc, device)
+            self.surfaces.append((col, dep, tid))
+            self.buffers.append((vb, ib))
+            v = View()
+            v.color, v.depth, v.tri_id = col.handle.value, dep.handle.value, (tid.handle.value if tid else None)
+            v.vertices, v.num_vertices = vb.data_ptr(), sc.vertices.shape[0]
+            v.indices, v.num_tris = ib.data_ptr(), sc.num_tris
+            v.deferred_clear = 1 if sc.deferred_clear else 0
+            v.pipe = scene_pipe(sc)
+            v.clear_color[:] = [float(x) for x in sc.clear_color]
+            v.clear_depth = float(sc.clear_depth)
+            views.append(v)
+        self.batch = Batch(self.cr, views)
+
+    def draw(self, stream=None):
+        self.batch.draw(stream)
+
+    def read(self, i):
+        col, dep, tid = self.surfaces[i]
+        return {"color": col.download(), "depth": dep.download(),
+                "triid": tid.download() if tid is not None else None}
+
+    def close(self):
+        self.batch.close()
+        self.cr.close()
+        for trio in self.surfaces:
+            for s in trio:
+                if s is not None:
+                    s.close()

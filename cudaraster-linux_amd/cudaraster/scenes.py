@@ -270,3 +270,28 @@ def config(i: int, scale: float = 1.0) -> Scene:
         res = max(64, int(round(2048 * math.sqrt(scale) / 8)) * 8) if scale < 1.0 else 2048
         return skewed(ns, nb, res, res)
     raise ValueError(f"no plain-method workload {i}")
+
+
+def multi_view(num_views=64, seed=3, resolutions=(256, 512, 1024, 2048), tri_range=(10_000, 500_000),
+               samples_choices=(0, 2), scale=1.0) -> list:
+    """Batched multi-view workload: per view a resolution drawn from
+    `resolutions` (square), a triangle count log-uniform in `tri_range`, and a
+    sample count 2^s with s drawn from `samples_choices`, all from one seed;
+    each view's triangles follow the 1M-triangle workload's distribution
+    (random_tris defaults).  `scale` shrinks triangle counts and resolutions
+    for oracle-sized runs."""
+    rng = np.random.default_rng(seed)
+    res = rng.choice(np.asarray(resolutions), size=num_views)
+    lo, hi = math.log(tri_range[0]), math.log(tri_range[1])
+    tris = np.exp(lo + (hi - lo) * rng.random(num_views))
+    sl2 = rng.choice(np.asarray(samples_choices), size=num_views)
+    seeds = rng.integers(0, 2 ** 31, size=num_views)
+    views = []
+    for k in range(num_views):
+        r = int(res[k])
+        n = int(tris[k])
+        if scale < 1.0:
+            r = max(64, int(round(r * math.sqrt(scale) / 8)) * 8)
+            n = max(1, int(n * scale))
+        views.append(random_tris(n, r, r, samples_log2=int(sl2[k]), seed=int(seeds[k]), name=f"view{k}"))
+    return views

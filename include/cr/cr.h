@@ -191,6 +191,68 @@ cr_status cr_get_profiling_info(cr_context* ctx, char* buf, size_t len);
 /* CudaRaster::setDebugParams (CudaRaster.cpp:501-504). */
 cr_status cr_set_debug_params(cr_context* ctx, const cr_debug_params* p);
 
+/* ---- batched multi-view draws (no reference counterpart) ----
+ * The reference draws one view per drawTriangles() call, each synchronous.
+ * A batch draws many independent views -- each with its own surfaces,
+ * buffers, pipe, resolution and sample count -- with one launch per stage
+ * over all views, replayed from a captured hipGraph.  Every view's output is
+ * bit-identical to a plain draw of the same state (cr_set_surfaces,
+ * cr_set_pixel_pipe, cr_set_*_buffer, cr_deferred_clear, cr_draw_triangles).
+ *
+ * Views may not share a colour, depth or triangle-id surface (they run
+ * concurrently).  Vertex and index buffers are device pointers that must
+ * stay valid, unchanged in size, for the life of the batch; their contents
+ * may change between draws. */
+typedef struct cr_view {
+    cr_surface* color;           /* CR_FORMAT_RGBA8 */
+    cr_surface* depth;           /* CR_FORMAT_DEPTH32, same size and samples */
+    cr_surface* tri_id;          /* optional CR_FORMAT_TRIANGLE_ID, or NULL */
+    const void* vertices;        /* device pointer (offset applied), 16-byte aligned */
+    int64_t num_vertices;
+    const void* indices;         /* device pointer (offset applied), int3 per triangle */
+    int32_t num_tris;
+    int32_t deferred_clear;      /* 1: clear to clear_color / clear_depth first (cr_deferred_clear) */
+    cr_pipe_desc pipe;
+    float clear_color[4];
+    float clear_depth;
+} cr_view;
+
+typedef struct cr_batch cr_batch;
+
+typedef struct cr_batch_info {
+    int32_t num_views;
+    int32_t fine_tiles_per_group;  /* fine-stage workgroup shape in use (tiles = waves per workgroup) */
+    int32_t uses_graph;            /* 1: draws replay a captured hipGraph */
+    int32_t tuned;                 /* 1: cr_batch_autotune has run */
+    int64_t retries;               /* capacity grow rounds of the last draw */
+    int64_t reserved_bytes;        /* device bytes of the batch's intermediate buffers */
+    float last_gpu_ms;             /* GPU time of the last draw's launch (events around it) */
+    float tuned_gpu_ms;            /* best GPU time seen by the autotuner */
+} cr_batch_info;
+
+/* Copies the view list; validates each view as cr_set_surfaces /
+ * cr_set_pixel_pipe / cr_draw_triangles would. */
+cr_status cr_batch_create(cr_context* ctx, const cr_view* views, int32_t num_views, cr_batch** out);
+cr_status cr_batch_destroy(cr_batch* batch);
+/* Draws every view; returns once all are complete (grows the capacities and
+ * re-draws the views that overflowed, as cr_draw_triangles). */
+cr_status cr_batch_draw(cr_batch* batch, void* stream);
+/* Project-written autotuner: draws the batch to measure each view's exact
+ * queue demand, sets per-view queue depths to it plus a margin, times every
+ * fine-stage workgroup shape and keeps the fastest, then captures the graph.
+ * Surfaces of views without deferred clear are restored afterwards. */
+cr_status cr_batch_autotune(cr_batch* batch, void* stream);
+/* Sets one view's capacities exactly (subtriangle slots, bin-queue and
+ * tile-list entries; each >= 1). Smaller than the demand is allowed: the
+ * draw then grows them. */
+cr_status cr_batch_reserve(cr_batch* batch, int32_t view, int64_t max_subtris, int64_t max_bin_entries,
+                           int64_t max_tile_entries);
+/* 0: launch the stages directly on each draw instead of replaying a graph. */
+cr_status cr_batch_use_graph(cr_batch* batch, int32_t enable);
+cr_status cr_batch_get_info(const cr_batch* batch, cr_batch_info* out);
+/* Counters of one view from the last draw (fragments and visible_subtris are 0). */
+cr_status cr_batch_get_view_counters(const cr_batch* batch, int32_t view, cr_counters* out);
+
 /* Version of the ABI (major << 16 | minor). */
 int32_t cr_abi_version(void);
 

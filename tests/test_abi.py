@@ -98,3 +98,32 @@ int main(int argc, char**) { return argc > 5 ? run(nullptr, nullptr, 0, 0) : 0; 
                         "-L", libdir, "-lcr", f"-Wl,-rpath,{libdir}", "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_struct_layouts_match_ctypes(tmp_path):
+    """The ctypes mirrors of cr_view, cr_batch_info, cr_counters and
+    cr_pipe_desc have the C layout (size and every field offset)."""
+    import shutil
+    import subprocess
+    import cudaraster
+    if shutil.which("gcc") is None:
+        pytest.skip("no host C compiler")
+    mirrors = {"cr_view": cudaraster.View, "cr_batch_info": cudaraster.BatchInfo,
+               "cr_counters": cudaraster.Counters, "cr_pipe_desc": cudaraster.PipeDesc,
+               "cr_debug_params": cudaraster.DebugParams}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cr/cr.h"', 'int main(void) {']
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
