@@ -181,6 +181,42 @@ def multi_view_ratio(reps=5, triid=True):
     return out
 
 
+def differentiable_views(reps=3):
+    """Differentiable workload: 32 views at 512x512, 200k triangles each, 8
+    FP32 channels; per view one exact visibility draw, the interpolation
+    forward, an L2 loss against the view's target and the backward pass to
+    positions and attributes (torch autograd over the HIP kernels)."""
+    import torch
+    from cudaraster import scenes
+    from cudaraster.diff import DiffRasterizer
+    views = scenes.diff_views(32)
+    dr = DiffRasterizer(512, 512)
+    dev = []
+    for v in views:
+        dev.append((torch.from_numpy(v.pos).cuda().requires_grad_(True),
+                    torch.from_numpy(v.attr).cuda().requires_grad_(True),
+                    torch.from_numpy(v.tris).cuda(), torch.from_numpy(v.target).cuda()))
+
+    def one_pass():
+        for pos, attr, tris, target in dev:
+            pos.grad = None
+            attr.grad = None
+            out = dr.render(pos, attr, tris)
+            ((out - target) ** 2).sum().backward()
+
+    one_pass()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one_pass()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    dr.close()
+    return {"views": len(views), "triangles_per_view": int(views[0].tris.shape[0]), "resolution": 512,
+            "channels": int(views[0].attr.shape[1]), "ms_per_pass": t * 1e3, "views_per_s": len(views) / t,
+            "steps": "visibility draw + interpolate forward + L2 loss + backward (positions, attributes)"}
+
+
 def cpu_baseline(scene, runs=5):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
@@ -208,6 +244,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true",
                     help="skip the counting draw (fragment and visible-subtriangle totals)")
+    ap.add_argument("--no-diff", action="store_true",
+                    help="skip the differentiable workload (32 views, forward + backward)")
     ap.add_argument("--no-multi-view", action="store_true",
                     help="skip the batched/plain multi-view ratio (64 views)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -301,6 +339,8 @@ def main():
         mv = multi_view_ratio(triid=triid)
         line["batched_over_plain_views_ratio"] = mv["ratio"]
         line["multi_view"] = mv
+    if not args.no_diff:
+        line["differentiable"] = differentiable_views()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene)
     print(json.dumps(line), flush=True)

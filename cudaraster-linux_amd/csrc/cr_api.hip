@@ -260,7 +260,7 @@ cr_status drawImpl(cr_context* ctx, hipStream_t stream, bool sync) {
 
 extern "C" {
 
-int32_t cr_abi_version(void) { return (1 << 16) | 1; }
+int32_t cr_abi_version(void) { return (1 << 16) | 2; }
 
 const char* cr_last_error(const cr_context* ctx) { return ctx ? ctx->err.c_str() : g_lastError.c_str(); }
 
@@ -291,7 +291,7 @@ cr_status cr_context_destroy(cr_context* ctx) {
   if (ctx->pending) (void)hipStreamSynchronize(ctx->pendingStream);
   for (DevBuf* b : {&ctx->triSubtris, &ctx->triHeader, &ctx->triData, &ctx->subtriTri, &ctx->chunkBase,
                     &ctx->binCnt, &ctx->binOff, &ctx->binList, &ctx->tileStart, &ctx->tileCount,
-                    &ctx->tileList, &ctx->tileFrags, &ctx->chunkSubtris, &ctx->counters})
+                    &ctx->tileList, &ctx->tileFrags, &ctx->chunkSubtris, &ctx->counters, &ctx->interpScratch})
     b->release();
   if (ctx->hostCounters) (void)hipHostFree(ctx->hostCounters);
   for (int i = 0; i < 5; i++)
@@ -539,6 +539,43 @@ cr_status cr_get_profiling_info(cr_context* ctx, char* buf, size_t len) {
            (long long)c.num_tris, (long long)c.num_subtris, (long long)c.bin_entries, (long long)c.tile_entries,
            (long long)c.fragments, ctx->dbg.count_fragments ? "" : " (not counted: set count_fragments)",
            (long long)c.retries);
+  return CR_OK;
+}
+
+cr_status cr_interpolate(cr_context* ctx, const cr_surface* triId, const float* pos, int64_t numVertices,
+                         const float* attr, int32_t numChannels, const int32_t* tris, int32_t numTris, float* out,
+                         int32_t* triImage, void* stream) {
+  if (!ctx || !triId || !out) return fail(ctx, CR_ERR_INVALID, "cr_interpolate: NULL argument");
+  if (triId->format != CR_FORMAT_TRIANGLE_ID || triId->samples != 1)
+    return fail(ctx, CR_ERR_INVALID, "cr_interpolate: needs a single-sample CR_FORMAT_TRIANGLE_ID surface");
+  if (numChannels < 0 || numVertices < 0 || numTris < 0 || (numChannels > 0 && !attr) ||
+      (numTris > 0 && (!pos || !tris)) || ((uintptr_t)pos & 15) || ((uintptr_t)tris & 3))
+    return fail(ctx, CR_ERR_INVALID, "cr_interpolate: bad buffers or counts");
+  InterpGeom G{triId->width, triId->height, triId->textureW, numChannels, numTris, numVertices};
+  CR_HIP(ctx, hipSetDevice(ctx->device));
+  launchInterpForward(G, triId->dev, pos, attr, tris, out, triImage, static_cast<hipStream_t>(stream));
+  CR_HIP(ctx, hipGetLastError());
+  return CR_OK;
+}
+
+cr_status cr_interpolate_backward(cr_context* ctx, const int32_t* triImage, int32_t width, int32_t height,
+                                  const float* pos, int64_t numVertices, const float* attr, int32_t numChannels,
+                                  const int32_t* tris, int32_t numTris, const float* gradOut, float* gradPos,
+                                  float* gradAttr, void* stream) {
+  if (!ctx || !triImage || !gradOut || !gradPos || (numChannels > 0 && (!attr || !gradAttr)))
+    return fail(ctx, CR_ERR_INVALID, "cr_interpolate_backward: NULL argument");
+  if (width <= 0 || height <= 0 || width > kMaxViewport || height > kMaxViewport || numChannels < 0 ||
+      numVertices < 0 || numTris < 0 || (numTris > 0 && (!pos || !tris)) || ((uintptr_t)pos & 15) ||
+      ((uintptr_t)tris & 3))
+    return fail(ctx, CR_ERR_INVALID, "cr_interpolate_backward: bad sizes or buffers");
+  CR_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t nPos = (size_t)numVertices * 4, nAttr = (size_t)numVertices * (size_t)numChannels;
+  CR_HIP(ctx, ctx->interpScratch.ensure((nPos + nAttr) * sizeof(double)));
+  double* acc = ctx->interpScratch.as<double>();
+  InterpGeom G{width, height, width, numChannels, numTris, numVertices};
+  launchInterpBackward(G, triImage, pos, attr, tris, gradOut, acc, acc + nPos, gradPos, gradAttr,
+                       static_cast<hipStream_t>(stream));
+  CR_HIP(ctx, hipGetLastError());
   return CR_OK;
 }
 

@@ -228,6 +228,22 @@ void launchFineBatch(int variant, int tilesPerGroup, const Params* views, const 
                      int totalGroups, hipStream_t stream);
 constexpr int kFineBatchShapes[3] = {2, 4, 8};   // tiles (waves) per workgroup the batch path can use
 
+// Differentiable interpolation over a single-sample triangle-id surface.
+struct InterpGeom {
+  int32_t width, height;        // viewport = surface size
+  int32_t texW;                 // texel pitch of the id surface (rounded width, one sample)
+  int32_t numChannels;
+  int32_t numTris;
+  int64_t numVertices;
+};
+void launchInterpForward(const InterpGeom& G, const uint32_t* ids, const float* pos, const float* attr,
+                         const int32_t* tris, float* out, int32_t* triImage, hipStream_t stream);
+// accPos (4 per vertex) and accAttr (C per vertex) are FP64 scratch; the
+// results are rounded into gradPos / gradAttr.
+void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const float* pos, const float* attr,
+                          const int32_t* tris, const float* gradOut, double* accPos, double* accAttr,
+                          float* gradPos, float* gradAttr, hipStream_t stream);
+
 // Largest v with start[v] <= b: the view owning workgroup (or group) b.
 // start has numViews + 1 nondecreasing entries; views without work have
 // start[v] == start[v + 1] and are never returned for b < start[numViews].

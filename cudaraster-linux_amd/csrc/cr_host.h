@@ -122,6 +122,7 @@ struct cr_context {
   crhost::DevBuf chunkBase, binCnt, binOff, binList;
   crhost::DevBuf tileStart, tileCount, tileList, tileFrags, chunkSubtris;
   crhost::DevBuf counters;
+  crhost::DevBuf interpScratch;       // FP64 gradient accumulators of cr_interpolate_backward
   crhip::Counters* hostCounters = nullptr;   // pinned
   hipEvent_t ev[5] = {};
   bool evRecorded = false;

@@ -39,6 +39,7 @@ EXPORTS = [
     "cr_get_stats", "cr_get_counters", "cr_get_profiling_info", "cr_set_debug_params", "cr_abi_version",
     "cr_batch_create", "cr_batch_destroy", "cr_batch_draw", "cr_batch_autotune", "cr_batch_reserve",
     "cr_batch_use_graph", "cr_batch_get_info", "cr_batch_get_view_counters",
+    "cr_interpolate", "cr_interpolate_backward",
 ]
 
 
@@ -144,6 +145,8 @@ def load(path: str | None = None):
         "cr_batch_use_graph": [vp, i32],
         "cr_batch_get_info": [vp, ctypes.POINTER(BatchInfo)],
         "cr_batch_get_view_counters": [vp, i32, ctypes.POINTER(Counters)],
+        "cr_interpolate": [vp, vp, vp, i64, vp, i32, vp, i32, vp, vp, vp],
+        "cr_interpolate_backward": [vp, vp, i32, i32, vp, i64, vp, i32, vp, i32, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -1,0 +1,120 @@
+"""Differentiable rendering on top of the rasterizer (libcr.so's
+cr_interpolate / cr_interpolate_backward behind a torch.autograd.Function).
+
+A frame is rasterized exactly (coverage, depth test, per-pixel triangle ids:
+the same kernels as any draw); C float attributes per vertex are then
+interpolated with perspective-correct barycentrics at pixel centres.  The
+backward pass propagates dL/dimage to the attributes and to the clip-space
+positions (x, y, w) through the barycentrics, with visibility fixed by the
+forward triangle ids.  torch provides device memory and streams only; both
+passes are HIP kernels in libcr.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import (CrError, CudaRaster, CudaSurface, FORMAT_DEPTH32, FORMAT_RGBA8, FORMAT_TRIANGLE_ID, _check,
+               load, pipe_desc, SHADER_CONSTANT, BLEND_REPLACE, RENDER_DEPTH)
+
+
+def _stream():
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+class Interpolate:
+    """Namespace for the autograd function (built lazily so that importing
+    this module does not need torch)."""
+    _fn = None
+
+    @classmethod
+    def function(cls):
+        if cls._fn is not None:
+            return cls._fn
+        import torch
+
+        class _Interpolate(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, pos, attr, tris, tri_image, raster):
+                H, W = tri_image.shape
+                C = attr.shape[1]
+                out = torch.empty((H, W, C), dtype=torch.float32, device=pos.device)
+                raster._interpolate(pos, attr, tris, out, tri_image)
+                ctx.save_for_backward(pos, attr, tris, tri_image)
+                ctx.raster = raster
+                return out
+
+            @staticmethod
+            def backward(ctx, grad_out):
+                pos, attr, tris, tri_image = ctx.saved_tensors
+                gp, ga = ctx.raster._interpolate_backward(pos, attr, tris, tri_image, grad_out.contiguous())
+                return gp, ga, None, None, None
+
+        cls._fn = _Interpolate
+        return cls._fn
+
+
+class DiffRasterizer:
+    """Single-sample rasterizer + differentiable interpolation for one
+    resolution.  render(pos, attr, tris) -> image [H, W, C] (row 0 at the
+    bottom); gradients flow to pos ([V, 4] clip space) and attr ([V, C])."""
+
+    def __init__(self, width, height):
+        self.width, self.height = int(width), int(height)
+        self.cr = CudaRaster().init()
+        self.color = CudaSurface((width, height), FORMAT_RGBA8, 1)
+        self.depth = CudaSurface((width, height), FORMAT_DEPTH32, 1)
+        self.ids = CudaSurface((width, height), FORMAT_TRIANGLE_ID, 1)
+        self.cr.setSurfaces(self.color, self.depth)
+        self.cr.setTriangleIdSurface(self.ids)
+        # Positions only (16-byte vertices), depth test on, constant colour.
+        self.cr.setPixelPipe(pipe_desc(16, SHADER_CONSTANT, BLEND_REPLACE, 0, RENDER_DEPTH))
+
+    def rasterize(self, pos, tris):
+        """Exact visibility: returns the triangle id image [H, W] (int32, -1
+        where empty) by drawing pos/tris and reading the id surface."""
+        import torch
+        if pos.dtype != torch.float32 or pos.dim() != 2 or pos.shape[1] != 4 or not pos.is_contiguous():
+            raise CrError(1, "pos must be a contiguous float32 [V, 4] device tensor")
+        if tris.dtype != torch.int32 or tris.dim() != 2 or tris.shape[1] != 3 or not tris.is_contiguous():
+            raise CrError(1, "tris must be a contiguous int32 [T, 3] device tensor")
+        torch.cuda.current_stream().synchronize()
+        self.cr.deferredClear((0.0, 0.0, 0.0, 0.0), 1.0)
+        self.cr.setVertexBuffer(pos.detach(), 0, pos.shape[0])
+        self.cr.setIndexBuffer(tris, 0, tris.shape[0])
+        self.cr.drawTriangles()
+        H, W = self.height, self.width
+        tri_image = torch.empty((H, W), dtype=torch.int32, device=pos.device)
+        dummy = torch.empty(1, dtype=torch.float32, device=pos.device)   # no channels: ids only
+        self._interpolate(pos.detach(), None, tris, dummy, tri_image, channels=0)
+        return tri_image
+
+    def render(self, pos, attr, tris, tri_image=None):
+        if tri_image is None:
+            tri_image = self.rasterize(pos, tris)
+        return Interpolate.function().apply(pos, attr, tris, tri_image, self)
+
+    # -- raw calls ------------------------------------------------------------
+    def _interpolate(self, pos, attr, tris, out, tri_image, channels=None):
+        C = attr.shape[1] if channels is None else channels
+        if attr is not None and (attr.dtype.itemsize != 4 or not attr.is_contiguous()):
+            raise CrError(1, "attr must be a contiguous float32 [V, C] device tensor")
+        _check(load().cr_interpolate(self.cr.ctx, self.ids.handle, pos.data_ptr(), pos.shape[0],
+                                     attr.data_ptr() if attr is not None else None, C, tris.data_ptr(),
+                                     tris.shape[0], out.data_ptr(), tri_image.data_ptr(), _stream()), self.cr.ctx)
+
+    def _interpolate_backward(self, pos, attr, tris, tri_image, grad_out):
+        import torch
+        gp = torch.empty_like(pos)
+        ga = torch.empty_like(attr)
+        H, W = tri_image.shape
+        _check(load().cr_interpolate_backward(self.cr.ctx, tri_image.data_ptr(), W, H, pos.data_ptr(),
+                                              pos.shape[0], attr.data_ptr(), attr.shape[1], tris.data_ptr(),
+                                              tris.shape[0], grad_out.data_ptr(), gp.data_ptr(), ga.data_ptr(),
+                                              _stream()), self.cr.ctx)
+        return gp, ga
+
+    def close(self):
+        self.cr.close()
+        for s in (self.color, self.depth, self.ids):
+            s.close()

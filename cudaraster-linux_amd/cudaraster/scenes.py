@@ -295,3 +295,41 @@ def multi_view(num_views=64, seed=3, resolutions=(256, 512, 1024, 2048), tri_ran
             n = max(1, int(n * scale))
         views.append(random_tris(n, r, r, samples_log2=int(sl2[k]), seed=int(seeds[k]), name=f"view{k}"))
     return views
+
+
+@dataclasses.dataclass
+class DiffView:
+    """One view of the differentiable workload: clip positions [V, 4],
+    attributes [V, C] uniform in [-1, 1], triangles [T, 3], and an L2 target
+    image [H, W, C] (row 0 at the bottom)."""
+    name: str
+    width: int
+    height: int
+    pos: np.ndarray
+    attr: np.ndarray
+    tris: np.ndarray
+    target: np.ndarray
+
+    def raster_scene(self) -> Scene:
+        """The visibility draw: positions only, depth test, 1x."""
+        return Scene(self.name, self.width, self.height, 0, FLAG_DEPTH, BLEND_REPLACE, SHADER_CONSTANT,
+                     np.ascontiguousarray(self.pos, dtype=np.float32), np.ascontiguousarray(self.tris))
+
+
+def diff_view(num_tris=200_000, width=512, height=512, channels=8, seed=4, area_range=(0.5, 256.0)) -> DiffView:
+    """Random mesh with the 1M-triangle workload's distribution at the given
+    resolution, `channels` FP32 attributes per vertex uniform in [-1, 1] and a
+    seeded random target image."""
+    base = random_tris(num_tris, width, height, samples_log2=0, seed=seed, area_range=area_range)
+    rng = np.random.default_rng(seed + 7919)
+    V = base.vertices.shape[0]
+    attr = rng.uniform(-1.0, 1.0, size=(V, channels)).astype(np.float32)
+    target = rng.uniform(-1.0, 1.0, size=(height, width, channels)).astype(np.float32)
+    pos = np.ascontiguousarray(base.vertices[:, :4])
+    return DiffView(f"diff{seed}", width, height, pos, attr, np.ascontiguousarray(base.indices), target)
+
+
+def diff_views(num_views=32, num_tris=200_000, width=512, height=512, channels=8, seed=4, scale=1.0) -> list:
+    """The differentiable workload: `num_views` independent views."""
+    n = max(1, int(num_tris * scale))
+    return [diff_view(n, width, height, channels, seed=seed + 101 * k) for k in range(num_views)]

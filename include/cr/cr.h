@@ -253,6 +253,34 @@ cr_status cr_batch_get_info(const cr_batch* batch, cr_batch_info* out);
 /* Counters of one view from the last draw (fragments and visible_subtris are 0). */
 cr_status cr_batch_get_view_counters(const cr_batch* batch, int32_t view, cr_counters* out);
 
+/* ---- differentiable interpolation (no reference counterpart) ----
+ * The reference's fragment shaders interpolate varyings with the fixed-point
+ * barycentric planes (FineRaster.inl:21-119) and have no backward pass.
+ * These two calls make a rasterized frame differentiable with respect to
+ * per-vertex attributes and clip-space positions, visibility fixed by the
+ * rasterizer's triangle ids (interior barycentric gradients; no edge or
+ * occlusion terms).  Images are row-major, row 0 at the bottom (the surface
+ * convention), pixel (x, y) at NDC ((2x+1)/W - 1, (2y+1)/H - 1).  Both calls
+ * are asynchronous on `stream`; barycentrics and gradients are evaluated in
+ * FP64 and rounded to FP32 once.
+ *
+ * pos:  float4 clip position per vertex (16-byte aligned); attr: C floats per
+ * vertex; tris: int3 per triangle (the index buffer the frame was drawn with).
+ *
+ * Forward: out[y][x][c] = sum_k b_k attr[tris[t][k]][c] for the triangle t
+ * in the single-sample id surface, 0 where no triangle is; tri_image[y][x]
+ * = t or -1 (may be NULL). */
+cr_status cr_interpolate(cr_context* ctx, const cr_surface* tri_id, const float* pos, int64_t num_vertices,
+                         const float* attr, int32_t num_channels, const int32_t* tris, int32_t num_tris,
+                         float* out, int32_t* tri_image, void* stream);
+/* Backward: overwrites grad_pos[V][4] (z component 0) and grad_attr[V][C]
+ * with dL/dpos and dL/dattr given grad_out[y][x][c] = dL/dout.  Uses FP64
+ * scratch held by the context: calls on one context must not overlap. */
+cr_status cr_interpolate_backward(cr_context* ctx, const int32_t* tri_image, int32_t width, int32_t height,
+                                  const float* pos, int64_t num_vertices, const float* attr,
+                                  int32_t num_channels, const int32_t* tris, int32_t num_tris,
+                                  const float* grad_out, float* grad_pos, float* grad_attr, void* stream);
+
 /* Version of the ABI (major << 16 | minor). */
 int32_t cr_abi_version(void);
 

@@ -1,0 +1,125 @@
+"""Differentiable interpolation on the GPU (cr_interpolate /
+cr_interpolate_backward through torch.autograd): triangle ids exact against
+the oracle, forward against the FP64 reference, gradients against the FP64
+reference gradient and FP64 central finite differences (h = 1e-4, 1e-4
+relative, on sampled parameters)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def run_view(view):
+    torch = _torch()
+    from cudaraster.diff import DiffRasterizer
+    dr = DiffRasterizer(view.width, view.height)
+    try:
+        pos = torch.from_numpy(view.pos).cuda().requires_grad_(True)
+        attr = torch.from_numpy(view.attr).cuda().requires_grad_(True)
+        tris = torch.from_numpy(view.tris).cuda()
+        target = torch.from_numpy(view.target).cuda()
+        ids = dr.rasterize(pos, tris)
+        out = dr.render(pos, attr, tris, ids)
+        loss = ((out - target) ** 2).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        return {"ids": ids.cpu().numpy(), "out": out.detach().cpu().numpy(), "gp": pos.grad.cpu().numpy(),
+                "ga": attr.grad.cpu().numpy(), "loss": float(loss.detach())}
+    finally:
+        dr.close()
+
+
+def oracle_ids(oracle_lib, view):
+    r = oracle_lib.render(view.raster_scene())
+    ids = r["triid"][: view.height, : view.width].astype(np.int64)
+    ids[ids == 0xFFFFFFFF] = -1
+    return ids
+
+
+def check_view(oracle_lib, view, n_fd, seed):
+    import diff_ref
+    got = run_view(view)
+    ids = oracle_ids(oracle_lib, view)
+    assert np.array_equal(got["ids"], ids), "triangle ids differ from the oracle"
+    ref_out = diff_ref.interpolate(view.pos, view.attr, view.tris, ids)
+    assert np.abs(got["out"] - ref_out).max() < 1e-6
+    gp, ga = diff_ref.backward(view.pos, view.attr, view.tris, ids, 2.0 * (ref_out - view.target))
+    for name, g, r in (("pos", got["gp"], gp), ("attr", got["ga"], ga)):
+        scale = np.abs(r).max()
+        assert np.abs(g - r).max() <= 1e-6 * scale + 1e-30, f"{name} gradient vs FP64 reference"
+    assert np.all(got["gp"][:, 2] == 0.0)
+    rng = np.random.default_rng(seed)
+    used = np.unique(view.tris[ids[ids >= 0]].ravel())
+    params = []
+    for _ in range(n_fd):
+        v = int(rng.choice(used))
+        params.append(("attr", v, int(rng.integers(view.attr.shape[1]))) if rng.random() < 0.5
+                      else ("pos", v, int(rng.choice([0, 1, 3]))))
+    # FP64 central differences with h = 1e-4.  Near the near plane (w < 1) a
+    # sub-pixel triangle is not much larger than h in clip units, and there
+    # the O(h^2) truncation term of the difference itself exceeds 1e-4; such
+    # parameters are recognised by D(h) and D(h/2) disagreeing by more than
+    # 1e-5 and are checked against h = 1e-6 instead (all parameters are).
+    fd = diff_ref.finite_difference(view.pos, view.attr, view.tris, ids, view.target, params, h=1e-4)
+    fd_half = diff_ref.finite_difference(view.pos, view.attr, view.tris, ids, view.target, params, h=5e-5)
+    fd_fine = diff_ref.finite_difference(view.pos, view.attr, view.tris, ids, view.target, params, h=1e-6)
+    an = np.array([got["gp"][v, c] if k == "pos" else got["ga"][v, c] for k, v, c in params])
+    floor = 1e-3 * np.abs(fd_fine).max()
+
+    def rel(a, b):
+        return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+    converged = rel(fd_half, fd) <= 1e-5
+    assert converged.mean() > 0.85, converged.mean()
+    err = rel(an, fd)[converged]
+    assert err.max() < 1e-4, err.max()
+    err = rel(an, fd_fine)
+    assert err.max() < 1e-4, (err.max(), params[int(err.argmax())])
+    got["fd_converged"] = int(converged.sum())
+    return got
+
+
+def test_small_view(oracle_lib):
+    from cudaraster import scenes
+    check_view(oracle_lib, scenes.diff_view(20000, 128, 96, channels=8, seed=21), n_fd=300, seed=1)
+
+
+def test_full_size_view(oracle_lib):
+    """One view of the differentiable workload at full size: 200k triangles,
+    512x512, 8 channels; 1000 sampled parameters against finite differences."""
+    from cudaraster import scenes
+    check_view(oracle_lib, scenes.diff_view(200_000, 512, 512, channels=8, seed=4), n_fd=1000, seed=2)
+
+
+def test_gradients_accumulate_over_views_independently():
+    """Two rasterizers (resolutions) in one autograd graph: gradients add."""
+    torch = _torch()
+    from cudaraster import scenes
+    from cudaraster.diff import DiffRasterizer
+    v = scenes.diff_view(5000, 96, 96, channels=4, seed=31)
+    a, b = DiffRasterizer(96, 96), DiffRasterizer(64, 64)
+    try:
+        pos = torch.from_numpy(v.pos).cuda().requires_grad_(True)
+        attr = torch.from_numpy(v.attr).cuda().requires_grad_(True)
+        tris = torch.from_numpy(v.tris).cuda()
+        oa = a.render(pos, attr, tris)
+        ob = b.render(pos, attr, tris)
+        (oa.square().sum() + ob.square().sum()).backward()
+        g_both = attr.grad.clone()
+        attr.grad = None
+        pos.grad = None
+        a.render(pos, attr, tris).square().sum().backward()
+        g_a = attr.grad.clone()
+        attr.grad = None
+        b.render(pos, attr, tris).square().sum().backward()
+        assert torch.allclose(g_both, g_a + attr.grad, rtol=1e-6, atol=1e-6)
+    finally:
+        a.close()
+        b.close()
