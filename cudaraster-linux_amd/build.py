@@ -23,9 +23,13 @@ ARCH = os.environ.get("CR_OFFLOAD_ARCH", "gfx950")
 # -ffp-contract=off: the only fused multiply-adds are explicit fmaf() calls.
 # IEEE fp32 division/sqrt and preserved denormals match the reference's
 # default nvcc flags (-prec-div=true, -ftz=false).  Packed fp32 ops are
-# disabled for correctness on this hardware generation.
+# disabled for correctness on this hardware generation.  No SLP
+# vectorisation: it packs per-sample coverage booleans into bit vectors in
+# vector registers instead of keeping them as 64-bit lane masks in scalar
+# registers, which costs the fine stage a third of its per-triangle work.
 HIPCC_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-mcode-object-version=5",
+    f"--offload-arch={ARCH}", "-O3", "-fno-slp-vectorize", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-mcode-object-version=5",
     "-fno-gpu-flush-denormals-to-zero", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unknown-warning-option", "-Wno-bitwise-instead-of-logical",

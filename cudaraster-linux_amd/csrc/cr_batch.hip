@@ -198,7 +198,7 @@ cr_status buildTables(cr_batch* b, Tables& T, const std::vector<int>& idx) {
   T.coarseBlocks = (int)acc;
   most = std::max(most, acc);
   T.fine.clear();
-  for (int var = 0; var < 8; var++) {
+  for (int var = 0; var < kFineVariants; var++) {
     const int ofs = (int)st.size();
     acc = 0;
     for (int k = 0; k < n; k++) {

@@ -222,7 +222,8 @@ void launchSetupBatch(const Params* views, const int32_t* blockStart, int numVie
                       hipStream_t stream);
 void launchCoarseBatch(const Params* views, const int32_t* blockStart, int numViews, int totalBlocks,
                        hipStream_t stream);
-// Fine-stage variant of a draw: samplesLog2 * 2 + (deferred shading ? 1 : 0).
+// Fine-stage variant of a draw: (samplesLog2 * 2 + deferred shading) * 2 + depth test.
+constexpr int kFineVariants = 16;
 int fineVariant(const Params& P);
 void launchFineBatch(int variant, int tilesPerGroup, const Params* views, const int32_t* groupStart, int numViews,
                      int totalGroups, hipStream_t stream);

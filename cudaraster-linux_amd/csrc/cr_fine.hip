@@ -26,18 +26,32 @@
 
 namespace crhip {
 
+// Per-(triangle, tile) record in LDS, written by the lane that sets the
+// triangle up and read by every lane of the wave.  Everything that does not
+// depend on the pixel is precomputed there: the per-record loop only does
+// what each pixel needs.
 template <int S>
 struct Rec {
   static constexpr int N = 1 << S;
   static constexpr int R = 0;                  // 3*N coverage thresholds, R[k*N+i]
-  static constexpr int D = 3 * N;              // dx0 dy0 dx1 dy1 dx2 dy2
-  static constexpr int Z = D + 6;              // zx, zy, Zs[0..N-1]
-  static constexpr int SLOT = Z + 2 + N;       // subtriangle slot
+  static constexpr int D = 3 * N;              // per edge: dy, -dx (so that L = lx16*dy + ly16*(-dx))
+  static constexpr int Z = D + 6;              // depth plane: zx lo16, zx hi16, zy lo16, zy hi16
+  static constexpr int ZS = Z + 4;             // depth at the tile's first pixel, per sample [N]
+  static constexpr int SLOT = ZS + N;          // subtriangle slot
   static constexpr int TRI = SLOT + 1;         // source triangle index
   static constexpr int SH = TRI + 1;           // in-order shading: W/U/V planes (9) + colours (12)
   static constexpr int SIZE_VIS = (SH + 3) & ~3;
   static constexpr int SIZE_FULL = (SH + 21 + 3) & ~3;
 };
+
+// (hi << 16) + lo as one full-rate instruction.  Written out so that the
+// compiler does not fold the shift back into a quarter-rate 32-bit multiply
+// of the 24-bit products above it.
+__device__ __forceinline__ uint32_t shl16Add(uint32_t hi, uint32_t lo) {
+  uint32_t r;
+  asm("v_lshl_add_u32 %0, %1, 16, %2" : "=v"(r) : "v"(hi), "v"(lo));
+  return r;
+}
 
 __device__ __forceinline__ uint32_t waveMax(uint32_t v) {
 #pragma unroll
@@ -130,7 +144,7 @@ __device__ __forceinline__ int xcdGroup(int groups) {
 
 // Fine raster of tile group `group` (W tiles, one wave each) of one draw.
 // COUNT: counting draw (fragment totals per tile into P.tileFrags).
-template <int S, bool VIS, bool COUNT, int W>
+template <int S, bool VIS, bool COUNT, int W, bool DEPTH>
 __device__ __forceinline__ void fineBody(const Params& P, const int group, const uint4* __restrict__ hdr,
                                          const uint4* __restrict__ tdata, const uint32_t* __restrict__ tileList,
                                          const uint8_t* __restrict__ verts, const uint32_t* __restrict__ subtriTri,
@@ -158,7 +172,6 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
   const int px = (tx << kTileLog2) + lx, py = (ty << kTileLog2) + ly;
   const size_t pitch = (size_t)P.widthPixels * N;
   const size_t texel = (size_t)py * pitch + (size_t)tx * (8 * N) + lx;
-  const bool depthOn = (P.flags & kFlagDepth) != 0;
   const bool lerpOn = (P.flags & kFlagLerp) != 0;
   const bool idOn = triid != nullptr;
   const bool gouraud = P.shader == kShaderGouraud;
@@ -186,7 +199,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
 
   uint64_t frags = 0;
   const uint32_t start = P.tileStart[tile];
-  const bool hiz = depthOn && P.hiz;
+  const bool hiz = DEPTH && P.hiz;
 
   for (uint32_t c = 0; c < cnt; c += kChunk) {
     // ---- lane j < kChunk sets up triangle j of this chunk ------------------
@@ -205,7 +218,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
       slot = tileList[start + c + lane];
       if (slot < (uint32_t)P.maxSubtris) {
         h = hdr[slot];
-        if (depthOn) z = tdata[(size_t)slot * 4];
+        if (DEPTH) z = tdata[(size_t)slot * 4];
         keep = !hiz || ((h.w & 0xFFFFF000u) < tileZMax);
       }
     }
@@ -235,14 +248,18 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
           const int32_t oy = (2 * i + 1) << (3 - S);
           r[L::R + k * N + i] = (uint32_t)(ck - (ox * dy - oy * dx));
         }
-        r[L::D + 2 * k] = (uint32_t)dx;
-        r[L::D + 2 * k + 1] = (uint32_t)dy;
+        r[L::D + 2 * k] = (uint32_t)dy;
+        r[L::D + 2 * k + 1] = (uint32_t)(-dx);
       }
-      r[L::Z] = z.x;
-      r[L::Z + 1] = z.y;
+      if (DEPTH) {
+        r[L::Z] = z.x & 0xFFFFu;
+        r[L::Z + 1] = z.x >> 16;
+        r[L::Z + 2] = z.y & 0xFFFFu;
+        r[L::Z + 3] = z.y >> 16;
 #pragma unroll
-      for (int i = 0; i < N; i++)
-        r[L::Z + 2 + i] = z.x * (tileSx + (uint32_t)msaaPattern(S, i)) + z.y * (tileSy + (uint32_t)i) + z.z;
+        for (int i = 0; i < N; i++)
+          r[L::ZS + i] = z.x * (tileSx + (uint32_t)msaaPattern(S, i)) + z.y * (tileSy + (uint32_t)i) + z.z;
+      }
       r[L::SLOT] = slot;
       if (!VIS) r[L::TRI] = (slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris];
       if (!VIS && gouraud) {
@@ -269,7 +286,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
       int32_t Lk[3];
 #pragma unroll
       for (int k = 0; k < 3; k++)
-        Lk[k] = __mul24(lx16, (int32_t)cur[L::D + 2 * k + 1]) - __mul24(ly16, (int32_t)cur[L::D + 2 * k]);
+        Lk[k] = __mul24(lx16, (int32_t)cur[L::D + 2 * k]) + __mul24(ly16, (int32_t)cur[L::D + 2 * k + 1]);
       bool cov[N];
       bool any = false;
 #pragma unroll
@@ -285,11 +302,15 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
       bool wr[N];
       uint32_t dz[N];
       bool anyWr = false;
-      if (depthOn) {
-        const uint32_t pl = r[L::Z] * lxN + r[L::Z + 1] * lyN;
+      if (DEPTH) {
+        // zx*lxN + zy*lyN mod 2^32 from 16-bit halves: lxN, lyN < 64, so
+        // every product fits the full-rate 24-bit multiplier.
+        const uint32_t hi = __umul24(r[L::Z + 1], lxN) + __umul24(r[L::Z + 3], lyN);
+        const uint32_t lo = __umul24(r[L::Z], lxN) + __umul24(r[L::Z + 2], lyN);
+        const uint32_t pl = shl16Add(hi, lo);
 #pragma unroll
         for (int i = 0; i < N; i++) {
-          dz[i] = r[L::Z + 2 + i] + pl;
+          dz[i] = r[L::ZS + i] + pl;
           wr[i] = cov[i] & (dz[i] < dep[i]);               // LESS (FineRaster.inl:697)
           anyWr = anyWr | wr[i];
         }
@@ -313,7 +334,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
         const uint32_t v = (r[L::SLOT] << 4) | (uint32_t)code;
 #pragma unroll
         for (int i = 0; i < N; i++) {
-          if (depthOn) dep[i] = wr[i] ? dz[i] : dep[i];
+          if (DEPTH) dep[i] = wr[i] ? dz[i] : dep[i];
           aux[i] = wr[i] ? v : aux[i];
         }
       } else {
@@ -335,7 +356,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
 #pragma unroll
         for (int i = 0; i < N; i++) {
           if (wr[i]) {
-            if (depthOn) dep[i] = dz[i];
+            if (DEPTH) dep[i] = dz[i];
             col[i] = blendPixel(P.blend, src, col[i]);
             aux[i] = tri;
           }
@@ -396,20 +417,20 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
   if (COUNT && lane == 0) P.tileFrags[tile] = (uint32_t)frags;
 }
 
-template <int S, bool VIS, bool COUNT>
+template <int S, bool VIS, bool COUNT, bool DEPTH>
 __global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P) {
   const int groups = (P.numTiles + kFineWaves - 1) / kFineWaves;
   const int group = xcdGroup(groups);
   if (group >= groups) return;
-  fineBody<S, VIS, COUNT, kFineWaves>(P, group, reinterpret_cast<const uint4*>(P.triHeader),
-                                      reinterpret_cast<const uint4*>(P.triData), P.tileList, P.vertices,
-                                      P.subtriTri, P.color, P.depth, P.triid);
+  fineBody<S, VIS, COUNT, kFineWaves, DEPTH>(P, group, reinterpret_cast<const uint4*>(P.triHeader),
+                                             reinterpret_cast<const uint4*>(P.triData), P.tileList, P.vertices,
+                                             P.subtriTri, P.color, P.depth, P.triid);
 }
 
 // Batched draws: the tile groups of every view of one pipe variant form one
 // grid (groupStart[v] = first group of view v, numViews + 1 entries; views of
 // other variants own no groups).
-template <int S, bool VIS, int W>
+template <int S, bool VIS, int W, bool DEPTH>
 __global__ void __launch_bounds__(64 * W) fineBatchKernel(const Params* __restrict__ views,
                                                           const int32_t* __restrict__ groupStart, int numViews,
                                                           int totalGroups) {
@@ -417,55 +438,62 @@ __global__ void __launch_bounds__(64 * W) fineBatchKernel(const Params* __restri
   if (g >= totalGroups) return;
   const int v = findView(groupStart, numViews, g);
   const Params& P = views[v];
-  fineBody<S, VIS, false, W>(P, g - groupStart[v], reinterpret_cast<const uint4*>(P.triHeader),
-                             reinterpret_cast<const uint4*>(P.triData), P.tileList, P.vertices, P.subtriTri,
-                             P.color, P.depth, P.triid);
+  fineBody<S, VIS, false, W, DEPTH>(P, g - groupStart[v], reinterpret_cast<const uint4*>(P.triHeader),
+                                    reinterpret_cast<const uint4*>(P.triData), P.tileList, P.vertices,
+                                    P.subtriTri, P.color, P.depth, P.triid);
 }
 
-template <int S, bool VIS, bool COUNT = false>
-static void launchFineT(const Params& P, hipStream_t stream) {
+// Variant v = (samplesLog2 * 2 + deferred shading) * 2 + depth test.
+template <int V>
+struct FineVariant {
+  static constexpr int S = V >> 2;
+  static constexpr bool VIS = ((V >> 1) & 1) != 0;
+  static constexpr bool DEPTH = (V & 1) != 0;
+};
+
+template <int V, bool COUNT>
+static void launchFineV(const Params& P, hipStream_t stream) {
+  using FV = FineVariant<V>;
   const int groups = (P.numTiles + kFineWaves - 1) / kFineWaves;
   const int per = (groups + 7) >> 3;
-  hipLaunchKernelGGL((fineKernel<S, VIS, COUNT>), dim3(per * 8), dim3(64 * kFineWaves), 0, stream, P);
+  hipLaunchKernelGGL((fineKernel<FV::S, FV::VIS, COUNT, FV::DEPTH>), dim3(per * 8), dim3(64 * kFineWaves), 0,
+                     stream, P);
 }
 
-template <int S, bool VIS, int W>
-static void launchFineBatchT(const Params* views, const int32_t* groupStart, int numViews, int totalGroups,
-                             hipStream_t stream) {
+template <int V, int W>
+static void launchFineBatchVW(const Params* views, const int32_t* groupStart, int numViews, int totalGroups,
+                              hipStream_t stream) {
+  using FV = FineVariant<V>;
   const int per = (totalGroups + 7) >> 3;
-  hipLaunchKernelGGL((fineBatchKernel<S, VIS, W>), dim3(per * 8), dim3(64 * W), 0, stream, views, groupStart,
-                     numViews, totalGroups);
+  hipLaunchKernelGGL((fineBatchKernel<FV::S, FV::VIS, W, FV::DEPTH>), dim3(per * 8), dim3(64 * W), 0, stream,
+                     views, groupStart, numViews, totalGroups);
 }
 
-template <int S, bool VIS>
-static void launchFineBatchW(int W, const Params* views, const int32_t* groupStart, int numViews, int totalGroups,
-                             hipStream_t stream) {
-  switch (W) {
-    case 2: launchFineBatchT<S, VIS, 2>(views, groupStart, numViews, totalGroups, stream); break;
-    case 8: launchFineBatchT<S, VIS, 8>(views, groupStart, numViews, totalGroups, stream); break;
-    default: launchFineBatchT<S, VIS, 4>(views, groupStart, numViews, totalGroups, stream); break;
-  }
-}
+using FineLaunch = void (*)(const Params&, hipStream_t);
+using FineBatchLaunch = void (*)(const Params*, const int32_t*, int, int, hipStream_t);
+
+template <int... V>
+struct FineTables {
+  static constexpr FineLaunch plain[] = {&launchFineV<V, false>...};
+  static constexpr FineBatchLaunch batch2[] = {&launchFineBatchVW<V, 2>...};
+  static constexpr FineBatchLaunch batch4[] = {&launchFineBatchVW<V, 4>...};
+  static constexpr FineBatchLaunch batch8[] = {&launchFineBatchVW<V, 8>...};
+};
+using AllFine = FineTables<0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15>;
+static_assert(kFineVariants == 16, "variant table");
 
 int fineVariant(const Params& P) {
   const bool vis = (P.blend == kBlendReplace || P.blend == kBlendDepthOnly) && P.maxSubtris < (1 << 28);
-  return P.samplesLog2 * 2 + (vis ? 1 : 0);
+  const bool depth = (P.flags & kFlagDepth) != 0;
+  return (P.samplesLog2 * 2 + (vis ? 1 : 0)) * 2 + (depth ? 1 : 0);
 }
 
 void launchFineBatch(int variant, int tilesPerGroup, const Params* views, const int32_t* groupStart, int numViews,
                      int totalGroups, hipStream_t stream) {
-  if (totalGroups <= 0) return;
-  const int W = tilesPerGroup;
-  switch (variant) {
-    case 0: launchFineBatchW<0, false>(W, views, groupStart, numViews, totalGroups, stream); break;
-    case 1: launchFineBatchW<0, true>(W, views, groupStart, numViews, totalGroups, stream); break;
-    case 2: launchFineBatchW<1, false>(W, views, groupStart, numViews, totalGroups, stream); break;
-    case 3: launchFineBatchW<1, true>(W, views, groupStart, numViews, totalGroups, stream); break;
-    case 4: launchFineBatchW<2, false>(W, views, groupStart, numViews, totalGroups, stream); break;
-    case 5: launchFineBatchW<2, true>(W, views, groupStart, numViews, totalGroups, stream); break;
-    case 6: launchFineBatchW<3, false>(W, views, groupStart, numViews, totalGroups, stream); break;
-    default: launchFineBatchW<3, true>(W, views, groupStart, numViews, totalGroups, stream); break;
-  }
+  if (totalGroups <= 0 || variant < 0 || variant >= kFineVariants) return;
+  const FineBatchLaunch* t = tilesPerGroup == 2 ? AllFine::batch2 : (tilesPerGroup == 8 ? AllFine::batch8
+                                                                                         : AllFine::batch4);
+  t[variant](views, groupStart, numViews, totalGroups, stream);
 }
 
 // Host-side launcher.  Deferred shading needs the visibility word
@@ -473,24 +501,20 @@ void launchFineBatch(int variant, int tilesPerGroup, const Params* views, const 
 void launchFine(const Params& P, hipStream_t stream) {
   if (P.tileFrags != nullptr) {
     // Counting draws run the in-order variant with per-tile totals.
-    switch (P.samplesLog2) {
-      case 0: launchFineT<0, false, true>(P, stream); break;
-      case 1: launchFineT<1, false, true>(P, stream); break;
-      case 2: launchFineT<2, false, true>(P, stream); break;
-      default: launchFineT<3, false, true>(P, stream); break;
+    const bool depth = (P.flags & kFlagDepth) != 0;
+    switch (P.samplesLog2 * 2 + (depth ? 1 : 0)) {
+      case 0: launchFineV<0, true>(P, stream); break;
+      case 1: launchFineV<1, true>(P, stream); break;
+      case 2: launchFineV<4, true>(P, stream); break;
+      case 3: launchFineV<5, true>(P, stream); break;
+      case 4: launchFineV<8, true>(P, stream); break;
+      case 5: launchFineV<9, true>(P, stream); break;
+      case 6: launchFineV<12, true>(P, stream); break;
+      default: launchFineV<13, true>(P, stream); break;
     }
     return;
   }
-  switch (fineVariant(P)) {
-    case 0: launchFineT<0, false>(P, stream); break;
-    case 1: launchFineT<0, true>(P, stream); break;
-    case 2: launchFineT<1, false>(P, stream); break;
-    case 3: launchFineT<1, true>(P, stream); break;
-    case 4: launchFineT<2, false>(P, stream); break;
-    case 5: launchFineT<2, true>(P, stream); break;
-    case 6: launchFineT<3, false>(P, stream); break;
-    default: launchFineT<3, true>(P, stream); break;
-  }
+  AllFine::plain[fineVariant(P)](P, stream);
 }
 
 // Sums a u32 array into a u64 counter (one workgroup; counting draws only).
