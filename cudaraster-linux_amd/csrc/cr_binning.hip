@@ -121,6 +121,38 @@ __device__ __forceinline__ TileTest tileTestOf(const Params& P, uint4 h, int bx,
   return r;
 }
 
+// Boxes larger than this many tiles are handled by the whole wave, one
+// triangle at a time, instead of by their own lane.
+constexpr int kCoopArea = 16;
+
+// Lane j's tile test, broadcast to every lane of the wave (j uniform).
+__device__ __forceinline__ TileTest readTileTest(const TileTest& r, int j) {
+  TileTest q;
+  q.x0 = __builtin_amdgcn_readlane(r.x0, j);
+  q.y0 = __builtin_amdgcn_readlane(r.y0, j);
+  q.x1 = __builtin_amdgcn_readlane(r.x1, j);
+  q.y1 = __builtin_amdgcn_readlane(r.y1, j);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    q.G[k] = __builtin_amdgcn_readlane(r.G[k], j);
+    q.sx[k] = __builtin_amdgcn_readlane(r.sx[k], j);
+    q.sy[k] = __builtin_amdgcn_readlane(r.sy[k], j);
+  }
+  return q;
+}
+
+// Calls f(x, y) for every tile of q's box that q hits, the wave's lanes
+// covering the box 16 x 4 tiles at a time (bins are at most 16 x 16 tiles).
+template <class F>
+__device__ __forceinline__ void forEachHitCoop(const TileTest& q, F&& f) {
+  const int lane = threadIdx.x & 63;
+  const int x = q.x0 + (lane & 15);
+  for (int y0 = q.y0; y0 <= q.y1; y0 += 4) {
+    const int y = y0 + (lane >> 4);
+    if (x <= q.x1 && y <= q.y1 && q.hit(x, y)) f(x, y);
+  }
+}
+
 // One round of up to 1024 bin entries: record the tiles each entry touches.
 // COUNT: add to cnt[tile]; otherwise set bits in emitMask[wave][tile].
 template <bool COUNT>
@@ -156,13 +188,26 @@ __device__ __forceinline__ void coarseRecord(const TileTest& r, bool valid, uint
           else emitMask[wave][x + (y << kBinLog2)] = m;
         }
       }
-  } else if (has) {
+    return;
+  }
+  // Scattered boxes: small ones by their own lane, large ones by the wave.
+  const bool big = has && w * h > kCoopArea;
+  if (has && !big) {
     for (int y = r.y0; y <= r.y1; y++)
       for (int x = r.x0; x <= r.x1; x++)
         if (r.hit(x, y)) {
           if (COUNT) atomicAdd(&cnt[x + (y << kBinLog2)], 1u);
           else atomicOr((unsigned long long*)&emitMask[wave][x + (y << kBinLog2)], (unsigned long long)bit);
         }
+  }
+  for (uint64_t b = __ballot(big); b != 0; b &= b - 1) {
+    const int j = (int)__builtin_ctzll(b);
+    const TileTest q = readTileTest(r, j);
+    const uint64_t bj = 1ull << j;
+    forEachHitCoop(q, [&](int x, int y) {
+      if (COUNT) atomicAdd(&cnt[x + (y << kBinLog2)], 1u);
+      else atomicOr((unsigned long long*)&emitMask[wave][x + (y << kBinLog2)], (unsigned long long)bj);
+    });
   }
 }
 
@@ -270,7 +315,12 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
           run[threadIdx.x] = acc;
         }
         __syncthreads();
-        if (valid && r.x0 <= r.x1 && r.y0 <= r.y1) {
+        // Write the entries: entry `lane` goes to tile ti at its rank among
+        // the wave's entries that hit ti.  Large boxes are walked by the
+        // whole wave, one entry at a time.
+        const bool hasBox = valid && r.x0 <= r.x1 && r.y0 <= r.y1;
+        const bool bigBox = hasBox && (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1) > kCoopArea;
+        if (hasBox && !bigBox) {
           const uint64_t bit = 1ull << lane, lt = lanesBelow();
           for (int y = r.y0; y <= r.y1; y++)
             for (int x = r.x0; x <= r.x1; x++) {
@@ -281,6 +331,25 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
                 if (p < (uint32_t)P.maxTileEntries) P.tileList[p] = slot;
               }
             }
+        }
+        for (uint64_t b = __ballot(bigBox); b != 0; b &= b - 1) {
+          const int j = (int)__builtin_ctzll(b);
+          const uint64_t bj = 1ull << j, ltj = bj - 1;
+          const uint32_t sj = (uint32_t)__builtin_amdgcn_readlane((int)slot, j);
+          const int qx0 = __builtin_amdgcn_readlane(r.x0, j), qx1 = __builtin_amdgcn_readlane(r.x1, j);
+          const int qy0 = __builtin_amdgcn_readlane(r.y0, j), qy1 = __builtin_amdgcn_readlane(r.y1, j);
+          const int x = qx0 + (lane & 15);
+          for (int y0 = qy0; y0 <= qy1; y0 += 4) {
+            const int y = y0 + (lane >> 4);
+            if (x <= qx1 && y <= qy1) {
+              const int ti = x + (y << kBinLog2);
+              const uint64_t m = emitMask[wave][ti];
+              if (m & bj) {
+                const uint32_t p = emitPre[wave][ti] + (uint32_t)__popcll(m & ltj);
+                if (p < (uint32_t)P.maxTileEntries) P.tileList[p] = sj;
+              }
+            }
+          }
         }
         __syncthreads();
       }
