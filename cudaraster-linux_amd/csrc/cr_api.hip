@@ -149,6 +149,7 @@ Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   P.widthTiles = g.wt;
   P.heightTiles = g.ht;
   P.numTiles = g.nt;
+  P.binLog2 = g.bl;
   P.widthBins = g.wb;
   P.heightBins = g.hb;
   P.numBins = g.nb;

@@ -42,15 +42,18 @@ struct ViewState {
   size_t oTri = 0, oSlot = 0, oClip = 0, oChunk = 0, oBinCC = 0, oBin = 0, oTile = 0, oTileList = 0;
 };
 
-// One launch set: the device Params table of a list of views and the
-// per-stage workgroup prefix tables.
+// One launch set: the device Params table of a list of views and, per
+// stage, the order in which the stage lists them with its workgroup prefix.
 struct Tables {
-  struct Fine {
-    int variant, groups, ofs;
+  struct Stage {
+    int variant = -1;          // fine stages: pipe variant
+    int n = 0;                 // listed views
+    int blocks = 0;            // workgroups (fine: tile groups)
+    int ofs = 0, orderOfs = 0; // prefix (n + 1 entries) and order (n entries) in `starts`
   };
   int n = 0;
-  int setupBlocks = 0, coarseBlocks = 0, setupOfs = 0, coarseOfs = 0;
-  std::vector<Fine> fine;
+  Stage setup, coarse;
+  std::vector<Stage> fine;
   DevBuf params, starts;
 };
 
@@ -67,6 +70,8 @@ struct cr_batch {
   Counters* hostCounters = nullptr;   // pinned, one block per view
   Tables full, subset;
   int tilesPerGroup = 4;
+  int banded = 0;                     // fine tile groups in XCD bands (1) or round-robin (0)
+  bool haveCounts = false;            // hostCounters hold every view's demand from a complete draw
   bool laidOut = false, useGraph = true, graphValid = false, tuned = false;
   hipStream_t capStream = nullptr;
   hipGraph_t graph = nullptr;
@@ -138,6 +143,7 @@ Params viewParams(const cr_batch* b, int i) {
   P.widthTiles = v.g.wt;
   P.heightTiles = v.g.ht;
   P.numTiles = v.g.nt;
+  P.binLog2 = v.g.bl;
   P.widthBins = v.g.wb;
   P.heightBins = v.g.hb;
   P.numBins = v.g.nb;
@@ -172,43 +178,69 @@ Params viewParams(const cr_batch* b, int i) {
   return P;
 }
 
+// Appends one stage: the listed Params indices `list` (in launch order) with
+// `blocks[k]` workgroups each.
+void appendStage(std::vector<int32_t>& st, Tables::Stage& S, const std::vector<int>& list,
+                 const std::vector<int64_t>& blocks, int64_t& most) {
+  S.n = (int)list.size();
+  S.ofs = (int)st.size();
+  int64_t acc = 0;
+  for (int k : list) {
+    st.push_back((int32_t)acc);
+    acc += blocks[k];
+  }
+  st.push_back((int32_t)acc);
+  S.orderOfs = (int)st.size();
+  for (int k : list) st.push_back(k);
+  S.blocks = (int)std::min<int64_t>(acc, INT32_MAX);
+  most = std::max(most, acc);
+}
+
+// Params indices sorted by decreasing cost per workgroup, so that the
+// longest workgroups start first (ties keep submission order).
+std::vector<int> longestFirst(const std::vector<double>& cost, const std::vector<int>& list) {
+  std::vector<int> out = list;
+  std::stable_sort(out.begin(), out.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+  return out;
+}
+
 cr_status buildTables(cr_batch* b, Tables& T, const std::vector<int>& idx) {
   cr_context* ctx = b->ctx;
   const int n = (int)idx.size();
   const int W = b->tilesPerGroup;
   std::vector<Params> ps(n);
   for (int k = 0; k < n; k++) ps[k] = viewParams(b, idx[k]);
+  // Cost per workgroup: measured entries per bin / per tile once a draw has
+  // counted them, else triangles per bin / per tile.
+  std::vector<double> binCost(n), tileCost(n);
+  std::vector<int64_t> chunks(n), bins(n), groups(n);
+  for (int k = 0; k < n; k++) {
+    const Params& P = ps[k];
+    const Counters& c = b->hostCounters[idx[k]];
+    const bool measured = b->haveCounts && !c.overflow;
+    binCost[k] = (measured ? (double)c.binEntries : (double)P.numTris) / std::max(P.numBins, 1);
+    tileCost[k] = (measured ? (double)c.tileEntries : 2.0 * P.numTris) * (1 << P.samplesLog2) /
+                  std::max(P.numTiles, 1);
+    chunks[k] = P.numTris > 0 ? P.numChunks : 0;
+    bins[k] = P.numBins;
+    groups[k] = (P.numTiles + W - 1) / W;
+  }
+  std::vector<int> all(n);
+  for (int k = 0; k < n; k++) all[k] = k;
   std::vector<int32_t> st;
-  int64_t acc = 0, most = 0;
-  T.setupOfs = (int)st.size();
-  for (int k = 0; k < n; k++) {
-    st.push_back((int32_t)acc);
-    acc += ps[k].numTris > 0 ? ps[k].numChunks : 0;
-  }
-  st.push_back((int32_t)acc);
-  T.setupBlocks = (int)acc;
-  most = std::max(most, acc);
-  T.coarseOfs = (int)st.size();
-  acc = 0;
-  for (int k = 0; k < n; k++) {
-    st.push_back((int32_t)acc);
-    acc += ps[k].numBins;
-  }
-  st.push_back((int32_t)acc);
-  T.coarseBlocks = (int)acc;
-  most = std::max(most, acc);
+  int64_t most = 0;
+  appendStage(st, T.setup, all, chunks, most);
+  appendStage(st, T.coarse, longestFirst(binCost, all), bins, most);
   T.fine.clear();
   for (int var = 0; var < kFineVariants; var++) {
-    const int ofs = (int)st.size();
-    acc = 0;
-    for (int k = 0; k < n; k++) {
-      st.push_back((int32_t)acc);
-      if (fineVariant(ps[k]) == var) acc += (ps[k].numTiles + W - 1) / W;
-    }
-    st.push_back((int32_t)acc);
-    most = std::max(most, acc);
-    if (acc > 0) T.fine.push_back({var, (int)acc, ofs});
-    else st.resize(ofs);
+    std::vector<int> list;
+    for (int k = 0; k < n; k++)
+      if (fineVariant(ps[k]) == var) list.push_back(k);
+    if (list.empty()) continue;
+    Tables::Stage S;
+    S.variant = var;
+    appendStage(st, S, longestFirst(tileCost, list), groups, most);
+    if (S.blocks > 0) T.fine.push_back(S);
   }
   if (most >= (int64_t)1 << 30) return fail(ctx, CR_ERR_OVERFLOW, "batch: more than 2^30 workgroups in one stage");
   CR_HIP(ctx, T.params.ensure((size_t)std::max(n, 1) * sizeof(Params)));
@@ -229,10 +261,10 @@ cr_status launchTables(cr_batch* b, const Tables& T, hipStream_t stream) {
   const Params* pv = T.params.as<Params>();
   const int32_t* s = T.starts.as<int32_t>();
   launchResetBatch(pv, T.n, stream);
-  launchSetupBatch(pv, s + T.setupOfs, T.n, T.setupBlocks, stream);
-  launchCoarseBatch(pv, s + T.coarseOfs, T.n, T.coarseBlocks, stream);
-  for (const Tables::Fine& f : T.fine)
-    launchFineBatch(f.variant, b->tilesPerGroup, pv, s + f.ofs, T.n, f.groups, stream);
+  launchSetupBatch(pv, s + T.setup.ofs, s + T.setup.orderOfs, T.setup.n, T.setup.blocks, stream);
+  launchCoarseBatch(pv, s + T.coarse.ofs, s + T.coarse.orderOfs, T.coarse.n, T.coarse.blocks, stream);
+  for (const Tables::Stage& f : T.fine)
+    launchFineBatch(f.variant, b->tilesPerGroup, b->banded, pv, s + f.ofs, s + f.orderOfs, f.n, f.blocks, stream);
   CR_HIP(b->ctx, hipGetLastError());
   return CR_OK;
 }
@@ -283,7 +315,10 @@ cr_status settleOverflow(cr_batch* b, hipStream_t stream) {
         return fail(b->ctx, CR_ERR_OVERFLOW, "batch view %zu: internal queue capacity exceeded (2^31 entries)", i);
       over.push_back((int)i);
     }
-    if (over.empty()) return CR_OK;
+    if (over.empty()) {
+      b->haveCounts = true;
+      return CR_OK;
+    }
     b->retries++;
     cr_status st = layout(b);
     if (st == CR_OK) st = buildTables(b, b->full, allViews(b));
@@ -454,27 +489,34 @@ cr_status cr_batch_autotune(cr_batch* b, void* stream_) {
   }
   st = layout(b);
   if (st != CR_OK) return st;
-  // 3. Fine-stage workgroup shape: the fastest of the compiled shapes.
-  int bestW = b->tilesPerGroup;
+  // 3. Fine-stage workgroup shape and placement: the fastest combination of
+  //    the compiled shapes and banded / round-robin tile groups (the stages
+  //    list views longest-first by the measured entries per bin and tile).
+  int bestW = b->tilesPerGroup, bestBanded = b->banded;
   float bestMs = 1e30f;
-  for (int W : kFineBatchShapes) {
-    b->tilesPerGroup = W;
-    st = buildTables(b, b->full, allViews(b));
-    if (st != CR_OK) return st;
-    float ms[3];
-    for (int rep = 0; rep < 4; rep++) {
-      st = saveSurfaces(b, stream, true);
-      if (st == CR_OK) st = drawBatch(b, stream, false);
+  for (int banded = 0; banded < 2; banded++) {
+    for (int W : kFineBatchShapes) {
+      b->tilesPerGroup = W;
+      b->banded = banded;
+      st = buildTables(b, b->full, allViews(b));
       if (st != CR_OK) return st;
-      if (rep > 0) ms[rep - 1] = b->lastMs;
-    }
-    std::sort(ms, ms + 3);
-    if (ms[1] < bestMs) {
-      bestMs = ms[1];
-      bestW = W;
+      float ms[3];
+      for (int rep = 0; rep < 4; rep++) {
+        st = saveSurfaces(b, stream, true);
+        if (st == CR_OK) st = drawBatch(b, stream, false);
+        if (st != CR_OK) return st;
+        if (rep > 0) ms[rep - 1] = b->lastMs;
+      }
+      std::sort(ms, ms + 3);
+      if (ms[1] < bestMs) {
+        bestMs = ms[1];
+        bestW = W;
+        bestBanded = banded;
+      }
     }
   }
   b->tilesPerGroup = bestW;
+  b->banded = bestBanded;
   st = buildTables(b, b->full, allViews(b));
   if (st == CR_OK) st = saveSurfaces(b, stream, true);
   if (st != CR_OK) return st;
@@ -513,6 +555,7 @@ cr_status cr_batch_get_info(const cr_batch* b, cr_batch_info* out) {
   memset(out, 0, sizeof(*out));
   out->num_views = (int32_t)b->views.size();
   out->fine_tiles_per_group = b->tilesPerGroup;
+  out->fine_banded = b->banded;
   out->uses_graph = b->useGraph ? 1 : 0;
   out->tuned = b->tuned ? 1 : 0;
   out->retries = b->retries;

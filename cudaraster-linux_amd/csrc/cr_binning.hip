@@ -89,6 +89,7 @@ struct TileTest {
 // (x, y) = p x d at the box origin + rel, |rel| < 2^29 (<= 15 tile steps of
 // 128 subpixels times |d| < 2^17), so the threshold c - p0 x d clamped to
 // +-2^30 decides every tile exactly in 32 bits.
+template <bool BOX_ONLY = false>
 __device__ __forceinline__ TileTest tileTestOf(const Params& P, uint4 h, int bx, int by) {
   TileTest r;
   const int32_t ox = P.viewportWidth << (kSubpixelLog2 - 1), oy = P.viewportHeight << (kSubpixelLog2 - 1);
@@ -99,11 +100,13 @@ __device__ __forceinline__ TileTest tileTestOf(const Params& P, uint4 h, int bx,
   const int32_t minx = min(min(ax[0], ax[1]), ax[2]), maxx = max(max(ax[0], ax[1]), ax[2]);
   const int32_t miny = min(min(ay[0], ay[1]), ay[2]), maxy = max(max(ay[0], ay[1]), ay[2]);
   const int ts = kSubpixelLog2 + kTileLog2;
-  const int32_t tx0 = bx << kBinLog2, ty0 = by << kBinLog2;
-  const int32_t lastX = min(P.widthTiles - tx0, 1 << kBinLog2) - 1;
-  const int32_t lastY = min(P.heightTiles - ty0, 1 << kBinLog2) - 1;
+  const int bl = P.binLog2;
+  const int32_t tx0 = bx << bl, ty0 = by << bl;
+  const int32_t lastX = min(P.widthTiles - tx0, 1 << bl) - 1;
+  const int32_t lastY = min(P.heightTiles - ty0, 1 << bl) - 1;
   r.x0 = max((minx >> ts) - tx0, 0); r.x1 = min((maxx >> ts) - tx0, lastX);
   r.y0 = max((miny >> ts) - ty0, 0); r.y1 = min((maxy >> ts) - ty0, lastY);
+  if (BOX_ONLY) return r;
   const int64_t BX = (int64_t)(tx0 + r.x0) << ts, BY = (int64_t)(ty0 + r.y0) << ts;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
@@ -143,20 +146,23 @@ __device__ __forceinline__ TileTest readTileTest(const TileTest& r, int j) {
 
 // Calls f(x, y) for every tile of q's box that q hits, the wave's lanes
 // covering the box 16 x 4 tiles at a time (bins are at most 16 x 16 tiles).
-template <class F>
+template <bool BOX_ONLY, class F>
 __device__ __forceinline__ void forEachHitCoop(const TileTest& q, F&& f) {
   const int lane = threadIdx.x & 63;
   const int x = q.x0 + (lane & 15);
   for (int y0 = q.y0; y0 <= q.y1; y0 += 4) {
     const int y = y0 + (lane >> 4);
-    if (x <= q.x1 && y <= q.y1 && q.hit(x, y)) f(x, y);
+    if (x <= q.x1 && y <= q.y1 && (BOX_ONLY || q.hit(x, y))) f(x, y);
   }
 }
 
 // One round of up to 1024 bin entries: record the tiles each entry touches.
-// COUNT: add to cnt[tile]; otherwise set bits in emitMask[wave][tile].
+// COUNT: add to cnt[tile] for every tile of the entry's box (an upper bound
+// of the exact count: list space only, the exact lists are built by the
+// emit pass); otherwise set bits in emitMask[wave][tile] for the tiles the
+// entry's edges reach.
 template <bool COUNT>
-__device__ __forceinline__ void coarseRecord(const TileTest& r, bool valid, uint32_t* cnt,
+__device__ __forceinline__ void coarseRecord(const TileTest& r, bool valid, const int bl, uint32_t* cnt,
                                              uint64_t (*emitMask)[kBinTiles]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int w = valid ? r.x1 - r.x0 + 1 : 0, h = valid ? r.y1 - r.y0 + 1 : 0;
@@ -181,11 +187,11 @@ __device__ __forceinline__ void coarseRecord(const TileTest& r, bool valid, uint
   if (coop) {
     for (int y = uy0; y <= uy1; y++)
       for (int x = ux0; x <= ux1; x++) {
-        const bool hit = has && x >= r.x0 && x <= r.x1 && y >= r.y0 && y <= r.y1 && r.hit(x, y);
+        const bool hit = has && x >= r.x0 && x <= r.x1 && y >= r.y0 && y <= r.y1 && (COUNT || r.hit(x, y));
         const uint64_t m = __ballot(hit);
         if (m != 0 && lane == 0) {
-          if (COUNT) atomicAdd(&cnt[x + (y << kBinLog2)], (uint32_t)__popcll(m));
-          else emitMask[wave][x + (y << kBinLog2)] = m;
+          if (COUNT) atomicAdd(&cnt[x + (y << bl)], (uint32_t)__popcll(m));
+          else emitMask[wave][x + (y << bl)] = m;
         }
       }
     return;
@@ -195,18 +201,18 @@ __device__ __forceinline__ void coarseRecord(const TileTest& r, bool valid, uint
   if (has && !big) {
     for (int y = r.y0; y <= r.y1; y++)
       for (int x = r.x0; x <= r.x1; x++)
-        if (r.hit(x, y)) {
-          if (COUNT) atomicAdd(&cnt[x + (y << kBinLog2)], 1u);
-          else atomicOr((unsigned long long*)&emitMask[wave][x + (y << kBinLog2)], (unsigned long long)bit);
+        if (COUNT || r.hit(x, y)) {
+          if (COUNT) atomicAdd(&cnt[x + (y << bl)], 1u);
+          else atomicOr((unsigned long long*)&emitMask[wave][x + (y << bl)], (unsigned long long)bit);
         }
   }
   for (uint64_t b = __ballot(big); b != 0; b &= b - 1) {
     const int j = (int)__builtin_ctzll(b);
     const TileTest q = readTileTest(r, j);
     const uint64_t bj = 1ull << j;
-    forEachHitCoop(q, [&](int x, int y) {
-      if (COUNT) atomicAdd(&cnt[x + (y << kBinLog2)], 1u);
-      else atomicOr((unsigned long long*)&emitMask[wave][x + (y << kBinLog2)], (unsigned long long)bj);
+    forEachHitCoop<COUNT>(q, [&](int x, int y) {
+      if (COUNT) atomicAdd(&cnt[x + (y << bl)], 1u);
+      else atomicOr((unsigned long long*)&emitMask[wave][x + (y << bl)], (unsigned long long)bj);
     });
   }
 }
@@ -223,6 +229,7 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   __shared__ uint32_t sBase;
 
   if (P.counters->overflow & 3u) return;
+  const int bl = P.binLog2, tpb = 1 << (2 * bl);   // tiles per bin side log2, tiles per bin
   const int bx = bin % P.widthBins, by = bin / P.widthBins;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t* bCnt = P.binCnt + (size_t)bin * P.numChunks;
@@ -230,11 +237,12 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
 
   for (int i = threadIdx.x; i < kBinTiles; i += kCoarseThreads) cnt[i] = 0;
   __syncthreads();
+  uint32_t myStart = 0;                 // thread t < tpb: first list entry of the bin's tile t
 
   for (int pass = 0; pass < 2; pass++) {
     if (pass == 1) {
       // Allocate the bin's tile lists: exclusive scan over the tiles.
-      const uint32_t v = (threadIdx.x < kBinTiles) ? cnt[threadIdx.x] : 0u;
+      const uint32_t v = (threadIdx.x < (unsigned)tpb) ? cnt[threadIdx.x] : 0u;
       uint32_t total;
       const uint32_t ex = blockScanC<kCoarseThreads>(v, sums, total);
       if (threadIdx.x == 0) {
@@ -248,15 +256,16 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
       __syncthreads();
       const uint32_t base = sBase;
       const bool ok = base != 0xFFFFFFFFu;
-      if (threadIdx.x < kBinTiles) {
-        const int x = threadIdx.x & ((1 << kBinLog2) - 1), y = threadIdx.x >> kBinLog2;
-        const int gx = (bx << kBinLog2) + x, gy = (by << kBinLog2) + y;
+      if (threadIdx.x < (unsigned)tpb) {
+        const int x = threadIdx.x & ((1 << bl) - 1), y = threadIdx.x >> bl;
+        const int gx = (bx << bl) + x, gy = (by << bl) + y;
         if (gx < P.widthTiles && gy < P.heightTiles) {
           const int tile = gx + gy * P.widthTiles;
           P.tileStart[tile] = ok ? base + ex : 0u;
-          P.tileCount[tile] = ok ? v : 0u;
+          P.tileCount[tile] = 0u;            // exact count written after the emit pass
         }
         run[threadIdx.x] = base + ex;
+        myStart = base + ex;
       }
       if (!ok || total == 0) return;
       __syncthreads();
@@ -297,15 +306,18 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
           }
           slot = P.binList[segAddr[lo] + (g - segStart[lo])];
           valid = slot < (uint32_t)P.maxSubtris;
-          if (valid) r = tileTestOf(P, *reinterpret_cast<const uint4*>(&P.triHeader[slot]), bx, by);
+          if (valid) {
+            const uint4 h = *reinterpret_cast<const uint4*>(&P.triHeader[slot]);
+            r = pass == 0 ? tileTestOf<true>(P, h, bx, by) : tileTestOf<false>(P, h, bx, by);
+          }
         }
         if (pass == 0) {
-          coarseRecord<true>(r, valid, cnt, emitMask);
+          coarseRecord<true>(r, valid, bl, cnt, emitMask);
           continue;
         }
-        coarseRecord<false>(r, valid, cnt, emitMask);
+        coarseRecord<false>(r, valid, bl, cnt, emitMask);
         __syncthreads();
-        if (threadIdx.x < kBinTiles) {
+        if (threadIdx.x < (unsigned)tpb) {
           uint32_t acc = run[threadIdx.x];
 #pragma unroll
           for (int q = 0; q < kCoarseWaves; q++) {
@@ -324,7 +336,7 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
           const uint64_t bit = 1ull << lane, lt = lanesBelow();
           for (int y = r.y0; y <= r.y1; y++)
             for (int x = r.x0; x <= r.x1; x++) {
-              const int ti = x + (y << kBinLog2);
+              const int ti = x + (y << bl);
               const uint64_t m = emitMask[wave][ti];
               if (m & bit) {
                 const uint32_t p = emitPre[wave][ti] + (uint32_t)__popcll(m & lt);
@@ -342,7 +354,7 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
           for (int y0 = qy0; y0 <= qy1; y0 += 4) {
             const int y = y0 + (lane >> 4);
             if (x <= qx1 && y <= qy1) {
-              const int ti = x + (y << kBinLog2);
+              const int ti = x + (y << bl);
               const uint64_t m = emitMask[wave][ti];
               if (m & bj) {
                 const uint32_t p = emitPre[wave][ti] + (uint32_t)__popcll(m & ltj);
@@ -357,15 +369,23 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
     }
     __syncthreads();
   }
+  // Exact list lengths: the emit pass advanced run[t] past every entry it
+  // wrote for tile t (the box-based allocation may leave unused room).
+  if (threadIdx.x < (unsigned)tpb) {
+    const int x = threadIdx.x & ((1 << bl) - 1), y = threadIdx.x >> bl;
+    const int gx = (bx << bl) + x, gy = (by << bl) + y;
+    if (gx < P.widthTiles && gy < P.heightTiles) P.tileCount[gx + gy * P.widthTiles] = run[threadIdx.x] - myStart;
+  }
 }
 
 __global__ void __launch_bounds__(kCoarseThreads) coarseKernel(const Params P) { coarseBody(P, blockIdx.x); }
 
 __global__ void __launch_bounds__(kCoarseThreads) coarseBatchKernel(const Params* __restrict__ views,
                                                                     const int32_t* __restrict__ blockStart,
+                                                                    const int32_t* __restrict__ order,
                                                                     int numViews) {
-  const int v = findView(blockStart, numViews, (int)blockIdx.x);
-  coarseBody(views[v], (int)blockIdx.x - blockStart[v]);
+  const int k = findView(blockStart, numViews, (int)blockIdx.x);
+  coarseBody(views[order[k]], (int)blockIdx.x - blockStart[k]);
 }
 
 // One workgroup per view: clears the view's counter block.
@@ -390,11 +410,11 @@ void launchResetBatch(const Params* views, int numViews, hipStream_t stream) {
   hipLaunchKernelGGL(resetBatchKernel, dim3(numViews), dim3(64), 0, stream, views);
 }
 
-void launchCoarseBatch(const Params* views, const int32_t* blockStart, int numViews, int totalBlocks,
-                       hipStream_t stream) {
+void launchCoarseBatch(const Params* views, const int32_t* blockStart, const int32_t* order, int numViews,
+                       int totalBlocks, hipStream_t stream) {
   if (totalBlocks <= 0) return;
   hipLaunchKernelGGL(coarseBatchKernel, dim3(totalBlocks), dim3(kCoarseThreads), 0, stream, views, blockStart,
-                     numViews);
+                     order, numViews);
 }
 
 }  // namespace crhip

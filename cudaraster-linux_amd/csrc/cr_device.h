@@ -20,7 +20,7 @@ namespace crhip {
 // Constants.hpp:21-84
 constexpr int kSubpixelLog2 = 4;
 constexpr int kTileLog2 = 3;                  // 8x8 pixels per tile
-constexpr int kBinLog2 = 4;                   // 16x16 tiles per bin
+constexpr int kBinLog2 = 4;                   // largest bin: 16x16 tiles (per draw: Params::binLog2)
 constexpr int kMaxViewport = 2048;
 constexpr int kMaxBinsLog2 = 4;               // <= 16x16 bins
 constexpr int kMaxBins = 1 << (2 * kMaxBinsLog2);
@@ -73,6 +73,7 @@ struct Params {
   int32_t viewportWidth, viewportHeight;
   int32_t widthPixels, heightPixels;   // rounded to tiles
   int32_t widthTiles, heightTiles, numTiles;
+  int32_t binLog2;                 // bin = 2^binLog2 x 2^binLog2 tiles (<= kBinLog2; <= 256 bins)
   int32_t widthBins, heightBins, numBins;
   // Pipe.
   int32_t samplesLog2;
@@ -216,17 +217,19 @@ void launchFragmentSum(const Params& P, hipStream_t stream);
 constexpr int kSetupChunkTris = 1024;
 
 // Batched draws: `views` is a device array of per-view Params, each stage
-// one launch over the workgroups of all views.
+// one launch over the workgroups of all views.  A stage lists its views in
+// its own order: the k-th listed view is views[order[k]] and owns the
+// workgroups [blockStart[k], blockStart[k+1]).
 void launchResetBatch(const Params* views, int numViews, hipStream_t stream);
-void launchSetupBatch(const Params* views, const int32_t* blockStart, int numViews, int totalBlocks,
-                      hipStream_t stream);
-void launchCoarseBatch(const Params* views, const int32_t* blockStart, int numViews, int totalBlocks,
-                       hipStream_t stream);
+void launchSetupBatch(const Params* views, const int32_t* blockStart, const int32_t* order, int numViews,
+                      int totalBlocks, hipStream_t stream);
+void launchCoarseBatch(const Params* views, const int32_t* blockStart, const int32_t* order, int numViews,
+                       int totalBlocks, hipStream_t stream);
 // Fine-stage variant of a draw: (samplesLog2 * 2 + deferred shading) * 2 + depth test.
 constexpr int kFineVariants = 16;
 int fineVariant(const Params& P);
-void launchFineBatch(int variant, int tilesPerGroup, const Params* views, const int32_t* groupStart, int numViews,
-                     int totalGroups, hipStream_t stream);
+void launchFineBatch(int variant, int tilesPerGroup, int banded, const Params* views, const int32_t* groupStart,
+                     const int32_t* order, int numViews, int totalGroups, hipStream_t stream);
 constexpr int kFineBatchShapes[3] = {2, 4, 8};   // tiles (waves) per workgroup the batch path can use
 
 // Differentiable interpolation over a single-sample triangle-id surface.

@@ -22,6 +22,8 @@
 //    deferred: the loop records (subtriangle slot, centroid) per sample and
 //    the tile is shaded once per (pixel, distinct triangle) at the end.
 //    SrcOver and Additive blend in submission order inside the loop.
+#include <utility>
+
 #include "cr_device.h"
 
 namespace crhip {
@@ -43,6 +45,26 @@ struct Rec {
   static constexpr int SIZE_VIS = (SH + 3) & ~3;
   static constexpr int SIZE_FULL = (SH + 21 + 3) & ~3;
 };
+
+// x * K (mod 2^32) for a constant K < 16 as shifts and adds (no 32-bit
+// multiply).
+template <int K>
+__device__ __forceinline__ uint32_t mulSmall(uint32_t x) {
+  static_assert(K >= 0 && K < 16, "small constant");
+  uint32_t r = 0;
+  if (K & 1) r += x;
+  if (K & 2) r += x << 1;
+  if (K & 4) r += x << 2;
+  if (K & 8) r += x << 3;
+  return r;
+}
+
+// out[i] = base + zx * pattern(i) + zy * i for every sample i.
+template <int S, int... I>
+__device__ __forceinline__ void sampleDepths(uint32_t* out, uint32_t base, uint32_t zx, uint32_t zy,
+                                             std::integer_sequence<int, I...>) {
+  ((out[I] = base + mulSmall<msaaPattern(S, I)>(zx) + mulSmall<I>(zy)), ...);
+}
 
 // (hi << 16) + lo as one full-rate instruction.  Written out so that the
 // compiler does not fold the shift back into a quarter-rate 32-bit multiply
@@ -203,6 +225,8 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
 
   for (uint32_t c = 0; c < cnt; c += kChunk) {
     // ---- lane j < kChunk sets up triangle j of this chunk ------------------
+    // Hierarchical Z: a triangle whose conservative zmin is not nearer than
+    // the tile's farthest stored depth fails LESS at every sample.
     uint32_t tileZMax = 0xFFFFFFFFu;
     if (hiz) {
       uint32_t mx = dep[0];
@@ -246,7 +270,8 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
         for (int i = 0; i < N; i++) {
           const int32_t ox = (2 * msaaPattern(S, i) + 1) << (3 - S);
           const int32_t oy = (2 * i + 1) << (3 - S);
-          r[L::R + k * N + i] = (uint32_t)(ck - (ox * dy - oy * dx));
+          // |ox|, |oy| < 16 and |dx|, |dy| < 2^17: exact 24-bit products.
+          r[L::R + k * N + i] = (uint32_t)(ck - (__mul24(ox, dy) - __mul24(oy, dx)));
         }
         r[L::D + 2 * k] = (uint32_t)dy;
         r[L::D + 2 * k + 1] = (uint32_t)(-dx);
@@ -256,9 +281,10 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
         r[L::Z + 1] = z.x >> 16;
         r[L::Z + 2] = z.y & 0xFFFFu;
         r[L::Z + 3] = z.y >> 16;
-#pragma unroll
-        for (int i = 0; i < N; i++)
-          r[L::ZS + i] = z.x * (tileSx + (uint32_t)msaaPattern(S, i)) + z.y * (tileSy + (uint32_t)i) + z.z;
+        // Depth at each sample of the tile's first pixel: one full 32-bit
+        // product per axis, then the sample offsets (< 8) as shifts and adds.
+        const uint32_t base = z.x * tileSx + z.y * tileSy + z.z;
+        sampleDepths<S>(r + L::ZS, base, z.x, z.y, std::make_integer_sequence<int, N>{});
       }
       r[L::SLOT] = slot;
       if (!VIS) r[L::TRI] = (slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris];
@@ -427,18 +453,21 @@ __global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P) {
                                              P.subtriTri, P.color, P.depth, P.triid);
 }
 
-// Batched draws: the tile groups of every view of one pipe variant form one
-// grid (groupStart[v] = first group of view v, numViews + 1 entries; views of
-// other variants own no groups).
+// Batched draws: the tile groups of the views of one pipe variant form one
+// grid; the k-th listed view is views[order[k]] and owns groups
+// [groupStart[k], groupStart[k+1]).  banded: XCD-aware bands (as a plain
+// draw); otherwise groups are dealt round-robin over the XCDs in list order,
+// which spreads the first-listed (heaviest) groups over all of them.
 template <int S, bool VIS, int W, bool DEPTH>
 __global__ void __launch_bounds__(64 * W) fineBatchKernel(const Params* __restrict__ views,
-                                                          const int32_t* __restrict__ groupStart, int numViews,
-                                                          int totalGroups) {
-  const int g = xcdGroup(totalGroups);
+                                                          const int32_t* __restrict__ groupStart,
+                                                          const int32_t* __restrict__ order, int numViews,
+                                                          int totalGroups, int banded) {
+  const int g = banded ? xcdGroup(totalGroups) : (int)blockIdx.x;
   if (g >= totalGroups) return;
-  const int v = findView(groupStart, numViews, g);
-  const Params& P = views[v];
-  fineBody<S, VIS, false, W, DEPTH>(P, g - groupStart[v], reinterpret_cast<const uint4*>(P.triHeader),
+  const int k = findView(groupStart, numViews, g);
+  const Params& P = views[order[k]];
+  fineBody<S, VIS, false, W, DEPTH>(P, g - groupStart[k], reinterpret_cast<const uint4*>(P.triHeader),
                                     reinterpret_cast<const uint4*>(P.triData), P.tileList, P.vertices,
                                     P.subtriTri, P.color, P.depth, P.triid);
 }
@@ -461,16 +490,16 @@ static void launchFineV(const Params& P, hipStream_t stream) {
 }
 
 template <int V, int W>
-static void launchFineBatchVW(const Params* views, const int32_t* groupStart, int numViews, int totalGroups,
-                              hipStream_t stream) {
+static void launchFineBatchVW(const Params* views, const int32_t* groupStart, const int32_t* order, int numViews,
+                              int totalGroups, int banded, hipStream_t stream) {
   using FV = FineVariant<V>;
   const int per = (totalGroups + 7) >> 3;
-  hipLaunchKernelGGL((fineBatchKernel<FV::S, FV::VIS, W, FV::DEPTH>), dim3(per * 8), dim3(64 * W), 0, stream,
-                     views, groupStart, numViews, totalGroups);
+  hipLaunchKernelGGL((fineBatchKernel<FV::S, FV::VIS, W, FV::DEPTH>), dim3(banded ? per * 8 : totalGroups),
+                     dim3(64 * W), 0, stream, views, groupStart, order, numViews, totalGroups, banded);
 }
 
 using FineLaunch = void (*)(const Params&, hipStream_t);
-using FineBatchLaunch = void (*)(const Params*, const int32_t*, int, int, hipStream_t);
+using FineBatchLaunch = void (*)(const Params*, const int32_t*, const int32_t*, int, int, int, hipStream_t);
 
 template <int... V>
 struct FineTables {
@@ -488,12 +517,12 @@ int fineVariant(const Params& P) {
   return (P.samplesLog2 * 2 + (vis ? 1 : 0)) * 2 + (depth ? 1 : 0);
 }
 
-void launchFineBatch(int variant, int tilesPerGroup, const Params* views, const int32_t* groupStart, int numViews,
-                     int totalGroups, hipStream_t stream) {
-  if (totalGroups <= 0 || variant < 0 || variant >= kFineVariants) return;
+void launchFineBatch(int variant, int tilesPerGroup, int banded, const Params* views, const int32_t* groupStart,
+                     const int32_t* order, int numViews, int totalGroups, hipStream_t stream) {
+  if (totalGroups <= 0 || numViews <= 0 || variant < 0 || variant >= kFineVariants) return;
   const FineBatchLaunch* t = tilesPerGroup == 2 ? AllFine::batch2 : (tilesPerGroup == 8 ? AllFine::batch8
                                                                                          : AllFine::batch4);
-  t[variant](views, groupStart, numViews, totalGroups, stream);
+  t[variant](views, groupStart, order, numViews, totalGroups, banded, stream);
 }
 
 // Host-side launcher.  Deferred shading needs the visibility word

@@ -49,7 +49,7 @@ struct DevBuf {
 };
 
 struct Geometry {
-  int32_t vw, vh, wp, hp, wt, ht, nt, wb, hb, nb;
+  int32_t vw, vh, wp, hp, wt, ht, nt, bl, wb, hb, nb;
 };
 
 inline Geometry geometryOf(const cr_surface* s) {
@@ -61,9 +61,15 @@ inline Geometry geometryOf(const cr_surface* s) {
   g.wt = g.wp >> crhip::kTileLog2;
   g.ht = g.hp >> crhip::kTileLog2;
   g.nt = g.wt * g.ht;
-  g.wb = (g.wt + (1 << crhip::kBinLog2) - 1) >> crhip::kBinLog2;
-  g.hb = (g.ht + (1 << crhip::kBinLog2) - 1) >> crhip::kBinLog2;
-  g.nb = g.wb * g.hb;
+  // The smallest bins that keep the count within kMaxBins: small surfaces
+  // get small bins, so that coarse work spreads over as many workgroups as a
+  // large surface's (the image does not depend on the bin size).
+  for (g.bl = 0;; g.bl++) {
+    g.wb = (g.wt + (1 << g.bl) - 1) >> g.bl;
+    g.hb = (g.ht + (1 << g.bl) - 1) >> g.bl;
+    g.nb = g.wb * g.hb;
+    if (g.nb <= crhip::kMaxBins || g.bl == crhip::kBinLog2) break;
+  }
   return g;
 }
 

@@ -358,7 +358,7 @@ __device__ __forceinline__ BinBox binBoxOf(const Params& P, uint4 h) {
   const int32_t minx = min(min(x0, x1), x2) + ox, maxx = max(max(x0, x1), x2) + ox;
   const int32_t miny = min(min(y0, y1), y2) + oy, maxy = max(max(y0, y1), y2) + oy;
   BinBox b;
-  const int sh = kSubpixelLog2 + kTileLog2 + kBinLog2;
+  const int sh = kSubpixelLog2 + kTileLog2 + P.binLog2;
   if (maxx < 0 || maxy < 0 || minx >= (P.widthPixels << kSubpixelLog2) || miny >= (P.heightPixels << kSubpixelLog2)) {
     b.x0 = 1; b.x1 = 0; b.y0 = 1; b.y1 = 0;
     return b;
@@ -531,9 +531,10 @@ __global__ void __launch_bounds__(kSetupChunk) setupBinKernel(const Params P) { 
 // first workgroup of view v (numViews + 1 entries).
 __global__ void __launch_bounds__(kSetupChunk) setupBinBatchKernel(const Params* __restrict__ views,
                                                                    const int32_t* __restrict__ blockStart,
+                                                                   const int32_t* __restrict__ order,
                                                                    int numViews) {
-  const int v = findView(blockStart, numViews, (int)blockIdx.x);
-  setupBinBody(views[v], (int)blockIdx.x - blockStart[v]);
+  const int k = findView(blockStart, numViews, (int)blockIdx.x);
+  setupBinBody(views[order[k]], (int)blockIdx.x - blockStart[k]);
 }
 
 void launchSetup(const Params& P, hipStream_t stream) {
@@ -541,11 +542,11 @@ void launchSetup(const Params& P, hipStream_t stream) {
   hipLaunchKernelGGL(setupBinKernel, dim3(P.numChunks), dim3(kSetupChunk), 0, stream, P);
 }
 
-void launchSetupBatch(const Params* views, const int32_t* blockStart, int numViews, int totalBlocks,
-                      hipStream_t stream) {
+void launchSetupBatch(const Params* views, const int32_t* blockStart, const int32_t* order, int numViews,
+                      int totalBlocks, hipStream_t stream) {
   if (totalBlocks <= 0) return;
   hipLaunchKernelGGL(setupBinBatchKernel, dim3(totalBlocks), dim3(kSetupChunk), 0, stream, views, blockStart,
-                     numViews);
+                     order, numViews);
 }
 
 }  // namespace crhip

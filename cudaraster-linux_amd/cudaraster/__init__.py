@@ -84,7 +84,7 @@ class BatchInfo(ctypes.Structure):
     _fields_ = [("num_views", ctypes.c_int32), ("fine_tiles_per_group", ctypes.c_int32),
                 ("uses_graph", ctypes.c_int32), ("tuned", ctypes.c_int32), ("retries", ctypes.c_int64),
                 ("reserved_bytes", ctypes.c_int64), ("last_gpu_ms", ctypes.c_float),
-                ("tuned_gpu_ms", ctypes.c_float)]
+                ("tuned_gpu_ms", ctypes.c_float), ("fine_banded", ctypes.c_int32)]
 
     def as_dict(self):
         return {n: (float(getattr(self, n)) if t is ctypes.c_float else int(getattr(self, n)))

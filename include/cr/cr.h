@@ -228,6 +228,7 @@ typedef struct cr_batch_info {
     int64_t reserved_bytes;        /* device bytes of the batch's intermediate buffers */
     float last_gpu_ms;             /* GPU time of the last draw's launch (events around it) */
     float tuned_gpu_ms;            /* best GPU time seen by the autotuner */
+    int32_t fine_banded;           /* 1: fine tile groups in per-XCD bands; 0: dealt round-robin */
 } cr_batch_info;
 
 /* Copies the view list; validates each view as cr_set_surfaces /
@@ -238,8 +239,11 @@ cr_status cr_batch_destroy(cr_batch* batch);
  * re-draws the views that overflowed, as cr_draw_triangles). */
 cr_status cr_batch_draw(cr_batch* batch, void* stream);
 /* Project-written autotuner: draws the batch to measure each view's exact
- * queue demand, sets per-view queue depths to it plus a margin, times every
- * fine-stage workgroup shape and keeps the fastest, then captures the graph.
+ * queue demand, sets per-view queue depths to it plus a margin, orders every
+ * stage's views longest-first by the measured entries per bin and per tile,
+ * times every fine-stage workgroup shape with banded and round-robin tile
+ * groups and keeps the fastest, then captures the graph.  Bin size is chosen
+ * per view from its resolution (the smallest bins with at most 256 bins).
  * Surfaces of views without deferred clear are restored afterwards. */
 cr_status cr_batch_autotune(cr_batch* batch, void* stream);
 /* Sets one view's capacities exactly (subtriangle slots, bin-queue and
