@@ -50,22 +50,22 @@ def fine_algorithmic_bytes(scene, visible_subtris, triid):
 def setup_algorithmic_bytes(scene, c, visible_subtris):
     """Setup + binning: indices (12 B) and three clip positions (16 B each)
     read per triangle; per visible subtriangle a 16 B header, a 64 B data
-    record and its 4 B source index written; 4 B per bin-queue entry and
-    8 B per (bin, chunk) count/offset pair written."""
+    record and its 4 B source index written; 16 B per bin-queue entry (slot
+    and snapped vertices) and 8 B per (bin, chunk) count/offset pair written."""
     chunks = (scene.num_tris + 1023) // 1024
     bins = ((scene.width + 127) // 128) * ((scene.height + 127) // 128)
     return (scene.num_tris * (12 + 3 * 16) + visible_subtris * (16 + 64 + 4)
-            + c["bin_entries"] * 4 + bins * chunks * 8)
+            + c["bin_entries"] * 16 + bins * chunks * 8)
 
 
 def coarse_algorithmic_bytes(scene, c, visible_subtris):
-    """Coarse: every bin-queue entry (4 B) and visible header (16 B) read,
-    the per-(bin, chunk) counts/offsets read, every tile-list entry (4 B)
-    and per-tile start/count (8 B) written."""
+    """Coarse: every 16 B bin-queue entry and the per-(bin, chunk)
+    counts/offsets read, every tile-list entry (4 B) and per-tile start/count
+    (8 B) written."""
     chunks = (scene.num_tris + 1023) // 1024
     bins = ((scene.width + 127) // 128) * ((scene.height + 127) // 128)
     tiles = ((scene.width + 7) // 8) * ((scene.height + 7) // 8)
-    return c["bin_entries"] * 4 + visible_subtris * 16 + bins * chunks * 8 + c["tile_entries"] * 4 + tiles * 8
+    return c["bin_entries"] * 16 + bins * chunks * 8 + c["tile_entries"] * 4 + tiles * 8
 
 
 def measured_traffic(kernel, workload_key, triid):

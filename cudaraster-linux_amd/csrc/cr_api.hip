@@ -123,7 +123,7 @@ cr_status ensureBuffers(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   CR_HIP(ctx, ctx->chunkBase.ensure((size_t)std::max(numChunks, 1) * 4));
   CR_HIP(ctx, ctx->binCnt.ensure((size_t)g.nb * std::max(numChunks, 1) * 4));
   CR_HIP(ctx, ctx->binOff.ensure((size_t)g.nb * std::max(numChunks, 1) * 4));
-  CR_HIP(ctx, ctx->binList.ensure((size_t)ctx->maxBinEntries * 4));
+  CR_HIP(ctx, ctx->binList.ensure((size_t)ctx->maxBinEntries * 16));
   CR_HIP(ctx, ctx->tileStart.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileCount.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileList.ensure((size_t)ctx->maxTileEntries * 4));
@@ -174,7 +174,7 @@ Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   P.binCnt = ctx->binCnt.as<uint32_t>();
   P.binOff = ctx->binOff.as<uint32_t>();
   P.maxBinEntries = (int32_t)ctx->maxBinEntries;
-  P.binList = ctx->binList.as<uint32_t>();
+  P.binList = ctx->binList.as<uint4>();
   P.tileStart = ctx->tileStart.as<uint32_t>();
   P.tileCount = ctx->tileCount.as<uint32_t>();
   P.maxTileEntries = (int32_t)ctx->maxTileEntries;

@@ -113,7 +113,7 @@ cr_status layout(cr_batch* b) {
   CR_HIP(ctx, b->chunkBase.ensure(nChunk * 4));
   CR_HIP(ctx, b->binCnt.ensure(nBinCC * 4));
   CR_HIP(ctx, b->binOff.ensure(nBinCC * 4));
-  CR_HIP(ctx, b->binList.ensure(nBin * 4));
+  CR_HIP(ctx, b->binList.ensure(nBin * 16));
   CR_HIP(ctx, b->tileStart.ensure(nTile * 4));
   CR_HIP(ctx, b->tileCount.ensure(nTile * 4));
   CR_HIP(ctx, b->tileList.ensure(nTileList * 4));
@@ -166,7 +166,7 @@ Params viewParams(const cr_batch* b, int i) {
   P.binCnt = b->binCnt.as<uint32_t>() + v.oBinCC;
   P.binOff = b->binOff.as<uint32_t>() + v.oBinCC;
   P.maxBinEntries = (int32_t)v.maxBin;
-  P.binList = b->binList.as<uint32_t>() + v.oBin;
+  P.binList = b->binList.as<uint4>() + v.oBin;
   P.tileStart = b->tileStart.as<uint32_t>() + v.oTile;
   P.tileCount = b->tileCount.as<uint32_t>() + v.oTile;
   P.maxTileEntries = (int32_t)v.maxTile;

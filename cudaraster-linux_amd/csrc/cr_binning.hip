@@ -71,6 +71,36 @@ __global__ void __launch_bounds__(64) resetCountersKernel(Counters* c, uint32_t 
   if (threadIdx.x == 0) c->numSubtris = numTris;
 }
 
+// Inclusive prefix maximum over the NT threads of the block (in thread order).
+template <int NT>
+__device__ __forceinline__ int blockMaxScanC(int v, uint32_t* sums) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x = max(x, y);
+  }
+  if (lane == 63) sums[wave] = (uint32_t)x;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    int sc = (threadIdx.x < NW) ? (int)sums[threadIdx.x] : -1;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(sc, o, 64);
+      if ((int)threadIdx.x >= o) sc = max(sc, y);
+    }
+    int ex = __shfl_up(sc, 1, 64);
+    if (threadIdx.x == 0) ex = -1;
+    if (threadIdx.x < NW) sums[threadIdx.x] = (uint32_t)ex;
+  }
+  __syncthreads();
+  const int r = max(x, (int)sums[wave]);
+  __syncthreads();
+  return r;
+}
+
 // Per-lane tile test data of one triangle inside bin (bx, by).
 struct TileTest {
   int32_t x0, y0, x1, y1;   // bin-local tile box (inclusive); empty when x0 > x1
@@ -226,6 +256,8 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   __shared__ uint64_t emitMask[kCoarseWaves][kBinTiles];
   __shared__ uint32_t emitPre[kCoarseWaves][kBinTiles];
   __shared__ uint32_t sums[kCoarseWaves + 1];
+  __shared__ int32_t segOf[kCoarseThreads];
+  __shared__ int32_t sCarry;
   __shared__ uint32_t sBase;
 
   if (P.counters->overflow & 3u) return;
@@ -287,27 +319,35 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
       __syncthreads();
 
       for (uint32_t r0 = 0; r0 < segTotal; r0 += kCoarseThreads) {
+        // Segment of entry r0 + t: every non-empty segment starting in this
+        // round marks its first entry; an inclusive max-scan then gives each
+        // entry the last segment started at or before it (the carry: the
+        // segment of entry r0 - 1).
+        segOf[threadIdx.x] = -1;
         if (pass == 1) {
           for (int i = threadIdx.x; i < kCoarseWaves * kBinTiles; i += kCoarseThreads)
             emitMask[i / kBinTiles][i % kBinTiles] = 0ull;
-          __syncthreads();
         }
+        __syncthreads();
+        const int carry = (r0 == 0) ? -1 : sCarry;
+        if (n0 > 0 && s0 >= r0 && s0 - r0 < (uint32_t)kCoarseThreads) segOf[s0 - r0] = i0;
+        if (n1 > 0 && s0 + n0 >= r0 && s0 + n0 - r0 < (uint32_t)kCoarseThreads) segOf[s0 + n0 - r0] = i1;
+        __syncthreads();
+        const int seg = max(blockMaxScanC<kCoarseThreads>(segOf[threadIdx.x], sums), carry);
+        if (threadIdx.x == kCoarseThreads - 1) sCarry = seg;
         const uint32_t g = r0 + threadIdx.x;
         bool valid = g < segTotal;
         uint32_t slot = 0;
         TileTest r;
         r.x0 = 1; r.x1 = 0; r.y0 = 1; r.y1 = 0;
         if (valid) {
-          // Largest segment whose start is <= g.
-          int lo = 0, hi = nb - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (segStart[mid] <= g) lo = mid; else hi = mid - 1;
-          }
-          slot = P.binList[segAddr[lo] + (g - segStart[lo])];
+          // The entry carries the slot and the snapped vertices: one
+          // contiguous 16-byte read, no dependent header fetch.
+          const uint4 e = P.binList[segAddr[seg] + (g - segStart[seg])];
+          slot = e.x;
           valid = slot < (uint32_t)P.maxSubtris;
           if (valid) {
-            const uint4 h = *reinterpret_cast<const uint4*>(&P.triHeader[slot]);
+            const uint4 h = make_uint4(e.y, e.z, e.w, 0u);
             r = pass == 0 ? tileTestOf<true>(P, h, bx, by) : tileTestOf<false>(P, h, bx, by);
           }
         }

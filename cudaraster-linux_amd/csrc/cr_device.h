@@ -100,7 +100,7 @@ struct Params {
   uint32_t* binCnt;                // [numBins][numChunks]
   uint32_t* binOff;                // [numBins][numChunks]
   int32_t maxBinEntries;
-  uint32_t* binList;
+  uint4* binList;                  // per entry: slot, then the header's three snapped vertices
   // Tile lists (each tile contiguous and in submission order).
   uint32_t* tileStart;             // [numTiles]
   uint32_t* tileCount;             // [numTiles]

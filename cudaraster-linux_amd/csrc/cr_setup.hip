@@ -432,7 +432,7 @@ __device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
 
   // Emit masks of one window: bit `lane` of emitMask[wave][bin] is set when
   // the window entry held by this thread touches the bin.
-  auto buildMasks = [&](uint32_t w, uint32_t& slot, BinBox& bb) -> bool {
+  auto buildMasks = [&](uint32_t w, uint32_t& slot, BinBox& bb, uint4& hv) -> bool {
     for (int i = threadIdx.x; i < kSetupWaves * P.numBins; i += kSetupChunk)
       emitMask[i / P.numBins][i % P.numBins] = 0ull;
     if (!compact) {
@@ -440,7 +440,10 @@ __device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
       slot = (uint32_t)base;
       bb.x0 = 1; bb.x1 = 0; bb.y0 = 1; bb.y1 = 0;
       __syncthreads();
-      if (k == 1) bb = binBoxOf(P, *reinterpret_cast<const uint4*>(&P.triHeader[slot]));
+      if (k == 1) {
+        hv = *reinterpret_cast<const uint4*>(&P.triHeader[slot]);
+        bb = binBoxOf(P, hv);
+      }
     } else {
       for (int s = 0; s < k; s++) {
         const uint32_t e = pos + (uint32_t)s;
@@ -452,7 +455,8 @@ __device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
       slot = 0;
       if (e < total) {
         slot = list[threadIdx.x];
-        bb = binBoxOf(P, *reinterpret_cast<const uint4*>(&P.triHeader[slot]));
+        hv = *reinterpret_cast<const uint4*>(&P.triHeader[slot]);
+        bb = binBoxOf(P, hv);
       }
     }
     const uint64_t bit = 1ull << lane;
@@ -466,8 +470,9 @@ __device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
   // ---- pass 1: per-bin counts ---------------------------------------------
   uint32_t slot0 = 0;
   BinBox bb0;
+  uint4 hv0 = make_uint4(0, 0, 0, 0);
   for (uint32_t w = 0; w < numWin; w++) {
-    buildMasks(w, slot0, bb0);
+    buildMasks(w, slot0, bb0, hv0);
     for (int i = threadIdx.x; i < P.numBins; i += kSetupChunk) {
       uint32_t acc = cnt[i];
 #pragma unroll
@@ -503,7 +508,8 @@ __device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
   for (uint32_t w = 0; w < numWin; w++) {
     uint32_t slot = slot0;
     BinBox bb = bb0;
-    if (numWin > 1) buildMasks(w, slot, bb);
+    uint4 hv = hv0;
+    if (numWin > 1) buildMasks(w, slot, bb, hv);
     for (int i = threadIdx.x; i < P.numBins; i += kSetupChunk) {
       uint32_t acc = run[i];
 #pragma unroll
@@ -519,7 +525,7 @@ __device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
       for (int x = bb.x0; x <= bb.x1; x++) {
         const int bin = x + y * P.widthBins;
         const uint32_t p = regionBase + emitPre[wave][bin] + (uint32_t)__popcll(emitMask[wave][bin] & lt);
-        if (p < (uint32_t)P.maxBinEntries) P.binList[p] = slot;
+        if (p < (uint32_t)P.maxBinEntries) P.binList[p] = make_uint4(slot, hv.x, hv.y, hv.z);
       }
     __syncthreads();
   }
