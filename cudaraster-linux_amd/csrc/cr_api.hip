@@ -162,6 +162,7 @@ Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   P.clearColor = ctx->clearColor;
   P.clearDepth = ctx->clearDepth;
   P.hiz = (ctx->dbg.disable_hiz || ctx->dbg.count_fragments) ? 0 : 1;
+  P.coarseThreads = ctx->dbg.coarse_threads;
   P.tileFrags = ctx->dbg.count_fragments ? ctx->tileFrags.as<uint32_t>() : nullptr;
   P.chunkSubtris = ctx->dbg.count_fragments ? ctx->chunkSubtris.as<uint32_t>() : nullptr;
   P.maxSubtris = (int32_t)ctx->maxSubtris;
@@ -261,7 +262,7 @@ cr_status drawImpl(cr_context* ctx, hipStream_t stream, bool sync) {
 
 extern "C" {
 
-int32_t cr_abi_version(void) { return (1 << 16) | 2; }
+int32_t cr_abi_version(void) { return (1 << 16) | 3; }
 
 const char* cr_last_error(const cr_context* ctx) { return ctx ? ctx->err.c_str() : g_lastError.c_str(); }
 
@@ -582,6 +583,8 @@ cr_status cr_interpolate_backward(cr_context* ctx, const int32_t* triImage, int3
 
 cr_status cr_set_debug_params(cr_context* ctx, const cr_debug_params* p) {
   if (!ctx || !p) return fail(ctx, CR_ERR_INVALID, "cr_set_debug_params: NULL argument");
+  if (p->coarse_threads != 0 && p->coarse_threads != 256 && p->coarse_threads != 1024)
+    return fail(ctx, CR_ERR_INVALID, "cr_set_debug_params: coarse_threads must be 0, 256 or 1024");
   ctx->dbg = *p;
   return CR_OK;
 }

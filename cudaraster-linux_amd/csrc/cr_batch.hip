@@ -71,6 +71,7 @@ struct cr_batch {
   Tables full, subset;
   int tilesPerGroup = 4;
   int banded = 0;                     // fine tile groups in XCD bands (1) or round-robin (0)
+  int coarseThreads = 256;            // coarse workgroup size
   bool haveCounts = false;            // hostCounters hold every view's demand from a complete draw
   bool laidOut = false, useGraph = true, graphValid = false, tuned = false;
   hipStream_t capStream = nullptr;
@@ -262,7 +263,8 @@ cr_status launchTables(cr_batch* b, const Tables& T, hipStream_t stream) {
   const int32_t* s = T.starts.as<int32_t>();
   launchResetBatch(pv, T.n, stream);
   launchSetupBatch(pv, s + T.setup.ofs, s + T.setup.orderOfs, T.setup.n, T.setup.blocks, stream);
-  launchCoarseBatch(pv, s + T.coarse.ofs, s + T.coarse.orderOfs, T.coarse.n, T.coarse.blocks, stream);
+  launchCoarseBatch(b->coarseThreads, pv, s + T.coarse.ofs, s + T.coarse.orderOfs, T.coarse.n, T.coarse.blocks,
+                    stream);
   for (const Tables::Stage& f : T.fine)
     launchFineBatch(f.variant, b->tilesPerGroup, b->banded, pv, s + f.ofs, s + f.orderOfs, f.n, f.blocks, stream);
   CR_HIP(b->ctx, hipGetLastError());
@@ -492,12 +494,16 @@ cr_status cr_batch_autotune(cr_batch* b, void* stream_) {
   // 3. Fine-stage workgroup shape and placement: the fastest combination of
   //    the compiled shapes and banded / round-robin tile groups (the stages
   //    list views longest-first by the measured entries per bin and tile).
-  int bestW = b->tilesPerGroup, bestBanded = b->banded;
+  int bestW = b->tilesPerGroup, bestBanded = b->banded, bestCoarse = b->coarseThreads;
   float bestMs = 1e30f;
   for (int banded = 0; banded < 2; banded++) {
-    for (int W : kFineBatchShapes) {
+    for (int trial = 0; trial < 3 + 1; trial++) {
+      // three fine shapes with the current coarse size, then the other
+      // coarse size with the best fine shape so far
+      const int W = trial < 3 ? kFineBatchShapes[trial] : bestW;
       b->tilesPerGroup = W;
       b->banded = banded;
+      b->coarseThreads = trial < 3 ? bestCoarse : (bestCoarse == 256 ? 1024 : 256);
       st = buildTables(b, b->full, allViews(b));
       if (st != CR_OK) return st;
       float ms[3];
@@ -512,11 +518,14 @@ cr_status cr_batch_autotune(cr_batch* b, void* stream_) {
         bestMs = ms[1];
         bestW = W;
         bestBanded = banded;
+        bestCoarse = b->coarseThreads;
       }
+      b->coarseThreads = bestCoarse;
     }
   }
   b->tilesPerGroup = bestW;
   b->banded = bestBanded;
+  b->coarseThreads = bestCoarse;
   st = buildTables(b, b->full, allViews(b));
   if (st == CR_OK) st = saveSurfaces(b, stream, true);
   if (st != CR_OK) return st;
@@ -556,6 +565,7 @@ cr_status cr_batch_get_info(const cr_batch* b, cr_batch_info* out) {
   out->num_views = (int32_t)b->views.size();
   out->fine_tiles_per_group = b->tilesPerGroup;
   out->fine_banded = b->banded;
+  out->coarse_threads = b->coarseThreads;
   out->uses_graph = b->useGraph ? 1 : 0;
   out->tuned = b->tuned ? 1 : 0;
   out->retries = b->retries;

@@ -25,9 +25,10 @@
 
 namespace crhip {
 
-constexpr int kCoarseThreads = 1024;
-constexpr int kCoarseWaves = kCoarseThreads / 64;
-constexpr int kSegBatch = 2048;        // chunks per segment-table batch
+// Workgroup sizes the coarse stage is compiled for (kCoarseShapes in
+// cr_device.h): 1024 lanes suit bins with thousands of entries; 256 lanes
+// need a quarter of the LDS, so several workgroups share a CU and a block
+// barrier waits for 4 waves instead of 16.
 
 __device__ __forceinline__ uint64_t lanesBelow() {
   const uint32_t lane = threadIdx.x & 63;
@@ -247,8 +248,12 @@ __device__ __forceinline__ void coarseRecord(const TileTest& r, bool valid, cons
   }
 }
 
-// Coarse raster of bin `bin` of one draw (one 1024-lane workgroup).
+// Coarse raster of bin `bin` of one draw (one CT-lane workgroup).
+template <int CT>
 __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
+  constexpr int kCoarseThreads = CT;
+  constexpr int kCoarseWaves = CT / 64;
+  constexpr int kSegBatch = 2 * CT;     // chunks per segment-table batch (two per lane)
   __shared__ uint32_t segStart[kSegBatch + 1];
   __shared__ uint32_t segAddr[kSegBatch];
   __shared__ uint32_t cnt[kBinTiles];
@@ -418,14 +423,15 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   }
 }
 
-__global__ void __launch_bounds__(kCoarseThreads) coarseKernel(const Params P) { coarseBody(P, blockIdx.x); }
+template <int CT>
+__global__ void __launch_bounds__(CT) coarseKernel(const Params P) { coarseBody<CT>(P, blockIdx.x); }
 
-__global__ void __launch_bounds__(kCoarseThreads) coarseBatchKernel(const Params* __restrict__ views,
-                                                                    const int32_t* __restrict__ blockStart,
-                                                                    const int32_t* __restrict__ order,
-                                                                    int numViews) {
+template <int CT>
+__global__ void __launch_bounds__(CT) coarseBatchKernel(const Params* __restrict__ views,
+                                                        const int32_t* __restrict__ blockStart,
+                                                        const int32_t* __restrict__ order, int numViews) {
   const int k = findView(blockStart, numViews, (int)blockIdx.x);
-  coarseBody(views[order[k]], (int)blockIdx.x - blockStart[k]);
+  coarseBody<CT>(views[order[k]], (int)blockIdx.x - blockStart[k]);
 }
 
 // One workgroup per view: clears the view's counter block.
@@ -441,8 +447,19 @@ void launchReset(const Params& P, hipStream_t stream) {
   hipLaunchKernelGGL(resetCountersKernel, dim3(1), dim3(64), 0, stream, P.counters, (uint32_t)P.numTris);
 }
 
+int coarseShapeFor(const Params& P) {
+  if (P.coarseThreads == 256 || P.coarseThreads == 1024) return P.coarseThreads;
+  // A single draw leaves the GPU room for the large workgroups (measured
+  // faster from 32k triangles at 256^2 to 1M at 2048^2); the small ones pay
+  // off when many draws share the GPU (batches tune it).
+  return 1024;
+}
+
 void launchCoarse(const Params& P, hipStream_t stream) {
-  hipLaunchKernelGGL(coarseKernel, dim3(P.numBins), dim3(kCoarseThreads), 0, stream, P);
+  if (coarseShapeFor(P) == 1024)
+    hipLaunchKernelGGL(coarseKernel<1024>, dim3(P.numBins), dim3(1024), 0, stream, P);
+  else
+    hipLaunchKernelGGL(coarseKernel<256>, dim3(P.numBins), dim3(256), 0, stream, P);
 }
 
 void launchResetBatch(const Params* views, int numViews, hipStream_t stream) {
@@ -450,11 +467,15 @@ void launchResetBatch(const Params* views, int numViews, hipStream_t stream) {
   hipLaunchKernelGGL(resetBatchKernel, dim3(numViews), dim3(64), 0, stream, views);
 }
 
-void launchCoarseBatch(const Params* views, const int32_t* blockStart, const int32_t* order, int numViews,
-                       int totalBlocks, hipStream_t stream) {
+void launchCoarseBatch(int threads, const Params* views, const int32_t* blockStart, const int32_t* order,
+                       int numViews, int totalBlocks, hipStream_t stream) {
   if (totalBlocks <= 0) return;
-  hipLaunchKernelGGL(coarseBatchKernel, dim3(totalBlocks), dim3(kCoarseThreads), 0, stream, views, blockStart,
-                     order, numViews);
+  if (threads == 1024)
+    hipLaunchKernelGGL(coarseBatchKernel<1024>, dim3(totalBlocks), dim3(1024), 0, stream, views, blockStart, order,
+                       numViews);
+  else
+    hipLaunchKernelGGL(coarseBatchKernel<256>, dim3(totalBlocks), dim3(256), 0, stream, views, blockStart, order,
+                       numViews);
 }
 
 }  // namespace crhip

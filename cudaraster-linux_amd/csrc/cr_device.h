@@ -84,6 +84,7 @@ struct Params {
   int32_t deferredClear;
   uint32_t clearColor, clearDepth;
   int32_t hiz;                     // 1: tile-level early depth rejection
+  int32_t coarseThreads;           // coarse workgroup size (256 or 1024), 0: chosen from the draw
   uint32_t* tileFrags;             // per-tile fragment counts (counting draws only), else null
   uint32_t* chunkSubtris;          // per-setup-chunk visible subtriangles (counting draws only), else null
   // Setup output.
@@ -223,8 +224,10 @@ constexpr int kSetupChunkTris = 1024;
 void launchResetBatch(const Params* views, int numViews, hipStream_t stream);
 void launchSetupBatch(const Params* views, const int32_t* blockStart, const int32_t* order, int numViews,
                       int totalBlocks, hipStream_t stream);
-void launchCoarseBatch(const Params* views, const int32_t* blockStart, const int32_t* order, int numViews,
-                       int totalBlocks, hipStream_t stream);
+void launchCoarseBatch(int threads, const Params* views, const int32_t* blockStart, const int32_t* order,
+                       int numViews, int totalBlocks, hipStream_t stream);
+constexpr int kCoarseShapes[2] = {256, 1024};    // coarse workgroup sizes (lanes) compiled
+int coarseShapeFor(const Params& P);             // the plain draw's choice
 // Fine-stage variant of a draw: (samplesLog2 * 2 + deferred shading) * 2 + depth test.
 constexpr int kFineVariants = 16;
 int fineVariant(const Params& P);

@@ -69,7 +69,8 @@ class Counters(ctypes.Structure):
 
 
 class DebugParams(ctypes.Structure):
-    _fields_ = [("disable_hiz", ctypes.c_int32), ("count_fragments", ctypes.c_int32)]
+    _fields_ = [("disable_hiz", ctypes.c_int32), ("count_fragments", ctypes.c_int32),
+                ("coarse_threads", ctypes.c_int32)]
 
 
 class View(ctypes.Structure):
@@ -84,7 +85,8 @@ class BatchInfo(ctypes.Structure):
     _fields_ = [("num_views", ctypes.c_int32), ("fine_tiles_per_group", ctypes.c_int32),
                 ("uses_graph", ctypes.c_int32), ("tuned", ctypes.c_int32), ("retries", ctypes.c_int64),
                 ("reserved_bytes", ctypes.c_int64), ("last_gpu_ms", ctypes.c_float),
-                ("tuned_gpu_ms", ctypes.c_float), ("fine_banded", ctypes.c_int32)]
+                ("tuned_gpu_ms", ctypes.c_float), ("fine_banded", ctypes.c_int32),
+                ("coarse_threads", ctypes.c_int32)]
 
     def as_dict(self):
         return {n: (float(getattr(self, n)) if t is ctypes.c_float else int(getattr(self, n)))
@@ -342,8 +344,8 @@ class CudaRaster:
         _check(load().cr_get_profiling_info(self.ctx, buf, 4096), self.ctx)
         return buf.value.decode()
 
-    def setDebugParams(self, disable_hiz=False, count_fragments=False):
-        p = DebugParams(1 if disable_hiz else 0, 1 if count_fragments else 0)
+    def setDebugParams(self, disable_hiz=False, count_fragments=False, coarse_threads=0):
+        p = DebugParams(1 if disable_hiz else 0, 1 if count_fragments else 0, int(coarse_threads))
         _check(load().cr_set_debug_params(self.ctx, ctypes.byref(p)), self.ctx)
 
     def close(self):

@@ -87,7 +87,7 @@ class CudaSurface {
 class CudaRaster {
  public:
   struct Stats { float setupTime, binTime, coarseTime, fineTime; };   // CudaRaster.hpp:45-51
-  struct DebugParams { bool disableHiZ = false; bool countFragments = false; };
+  struct DebugParams { bool disableHiZ = false; bool countFragments = false; int coarseThreads = 0; };
 
   CudaRaster() { check(cr_context_create(&ctx_), nullptr); }
   ~CudaRaster() { if (ctx_) cr_context_destroy(ctx_); }
@@ -147,6 +147,7 @@ class CudaRaster {
     cr_debug_params d;
     d.disable_hiz = p.disableHiZ ? 1 : 0;
     d.count_fragments = p.countFragments ? 1 : 0;
+    d.coarse_threads = p.coarseThreads;
     check(cr_set_debug_params(ctx_, &d), ctx_);
   }
 

@@ -120,6 +120,7 @@ typedef struct cr_counters {
 typedef struct cr_debug_params {
     int32_t disable_hiz;         /* 1: no tile-level early depth rejection */
     int32_t count_fragments;     /* 1: count fragments exactly (implies disable_hiz; slower) */
+    int32_t coarse_threads;      /* coarse-stage workgroup size: 256, 1024, or 0 to choose from the draw */
 } cr_debug_params;
 
 /* ---- context (CudaRaster ctor + init, CudaRaster.cpp:98-128) ---- */
@@ -229,6 +230,7 @@ typedef struct cr_batch_info {
     float last_gpu_ms;             /* GPU time of the last draw's launch (events around it) */
     float tuned_gpu_ms;            /* best GPU time seen by the autotuner */
     int32_t fine_banded;           /* 1: fine tile groups in per-XCD bands; 0: dealt round-robin */
+    int32_t coarse_threads;        /* coarse-stage workgroup size in use */
 } cr_batch_info;
 
 /* Copies the view list; validates each view as cr_set_surfaces /
@@ -242,7 +244,8 @@ cr_status cr_batch_draw(cr_batch* batch, void* stream);
  * queue demand, sets per-view queue depths to it plus a margin, orders every
  * stage's views longest-first by the measured entries per bin and per tile,
  * times every fine-stage workgroup shape with banded and round-robin tile
- * groups and keeps the fastest, then captures the graph.  Bin size is chosen
+ * groups and both coarse workgroup sizes and keeps the fastest, then
+ * captures the graph.  Bin size is chosen
  * per view from its resolution (the smallest bins with at most 256 bins).
  * Surfaces of views without deferred clear are restored afterwards. */
 cr_status cr_batch_autotune(cr_batch* batch, void* stream);
