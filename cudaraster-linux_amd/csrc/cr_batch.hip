@@ -59,6 +59,8 @@ struct Tables {
 
 size_t align64(size_t x) { return (x + 63) & ~(size_t)63; }
 
+constexpr int kMaxGroups = 4;   // concurrent view groups (streams / graph branches)
+
 }  // namespace
 
 struct cr_batch {
@@ -68,7 +70,11 @@ struct cr_batch {
   DevBuf triSubtris, triHeader, triData, subtriTri, chunkBase, binCnt, binOff, binList;
   DevBuf tileStart, tileCount, tileList, counters, saved;
   Counters* hostCounters = nullptr;   // pinned, one block per view
-  Tables full, subset;
+  Tables groups[kMaxGroups];          // views split into cost-balanced groups, one launch chain each
+  int numGroups = 1;
+  Tables subset;                      // views re-drawn after an overflow
+  hipStream_t side[kMaxGroups] = {};  // groups 1.. run on these (group 0 on the draw's stream)
+  hipEvent_t evFork = nullptr, evJoin[kMaxGroups] = {};
   int tilesPerGroup = 4;
   int banded = 0;                     // fine tile groups in XCD bands (1) or round-robin (0)
   int coarseThreads = 256;            // coarse workgroup size
@@ -258,6 +264,37 @@ std::vector<int> allViews(const cr_batch* b) {
   return idx;
 }
 
+// Splits the views into numGroups groups of similar cost (longest first to
+// the least loaded group) and builds each group's tables.
+cr_status buildAll(cr_batch* b) {
+  const int n = (int)b->views.size();
+  const int K = std::max(1, std::min(b->numGroups, n));
+  std::vector<double> cost(n);
+  for (int i = 0; i < n; i++) {
+    const Counters& c = b->hostCounters[i];
+    const int ns = 1 << b->views[i].desc.pipe.samples_log2;
+    cost[i] = (b->haveCounts && !c.overflow) ? (double)c.binEntries + (double)c.tileEntries * ns
+                                             : (double)b->views[i].desc.num_tris * (1 + ns);
+  }
+  std::vector<int> order = allViews(b);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+  std::vector<std::vector<int>> members(K);
+  std::vector<double> load(K, 0.0);
+  for (int i : order) {
+    const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    members[g].push_back(i);
+    load[g] += cost[i];
+  }
+  for (int g = 0; g < K; g++) {
+    std::sort(members[g].begin(), members[g].end());
+    cr_status st = buildTables(b, b->groups[g], members[g]);
+    if (st != CR_OK) return st;
+  }
+  b->numGroups = K;
+  b->graphValid = false;
+  return CR_OK;
+}
+
 cr_status launchTables(cr_batch* b, const Tables& T, hipStream_t stream) {
   const Params* pv = T.params.as<Params>();
   const int32_t* s = T.starts.as<int32_t>();
@@ -271,6 +308,21 @@ cr_status launchTables(cr_batch* b, const Tables& T, hipStream_t stream) {
   return CR_OK;
 }
 
+// All groups: group 0 on `stream`, the others on side streams forked from
+// and joined back into it (parallel branches when captured in a graph).
+cr_status launchAllGroups(cr_batch* b, hipStream_t stream) {
+  if (b->numGroups <= 1) return launchTables(b, b->groups[0], stream);
+  CR_HIP(b->ctx, hipEventRecord(b->evFork, stream));
+  for (int g = 1; g < b->numGroups; g++) CR_HIP(b->ctx, hipStreamWaitEvent(b->side[g], b->evFork, 0));
+  cr_status st = launchTables(b, b->groups[0], stream);
+  for (int g = 1; g < b->numGroups && st == CR_OK; g++) st = launchTables(b, b->groups[g], b->side[g]);
+  for (int g = 1; g < b->numGroups; g++) {
+    CR_HIP(b->ctx, hipEventRecord(b->evJoin[g], b->side[g]));
+    CR_HIP(b->ctx, hipStreamWaitEvent(stream, b->evJoin[g], 0));
+  }
+  return st;
+}
+
 cr_status capture(cr_batch* b) {
   cr_context* ctx = b->ctx;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -278,7 +330,7 @@ cr_status capture(cr_batch* b) {
   b->exec = nullptr;
   b->graph = nullptr;
   CR_HIP(ctx, hipStreamBeginCapture(b->capStream, hipStreamCaptureModeThreadLocal));
-  cr_status st = launchTables(b, b->full, b->capStream);
+  cr_status st = launchAllGroups(b, b->capStream);
   hipError_t e = hipStreamEndCapture(b->capStream, &b->graph);
   if (st != CR_OK) return st;
   CR_HIP(ctx, e);
@@ -298,7 +350,7 @@ cr_status prepare(cr_batch* b) {
   if (b->laidOut) return CR_OK;
   cr_status st = layout(b);
   if (st != CR_OK) return st;
-  return buildTables(b, b->full, allViews(b));
+  return buildAll(b);
 }
 
 // Grows the capacities of every view whose counters overflowed and re-draws
@@ -323,7 +375,7 @@ cr_status settleOverflow(cr_batch* b, hipStream_t stream) {
     }
     b->retries++;
     cr_status st = layout(b);
-    if (st == CR_OK) st = buildTables(b, b->full, allViews(b));
+    if (st == CR_OK) st = buildAll(b);
     if (st == CR_OK) st = buildTables(b, b->subset, over);
     if (st == CR_OK) st = launchTables(b, b->subset, stream);
     if (st == CR_OK) st = readCounters(b, stream);
@@ -345,7 +397,7 @@ cr_status drawBatch(cr_batch* b, hipStream_t stream, bool graph) {
   if (graph) {
     CR_HIP(ctx, hipGraphLaunch(b->exec, stream));
   } else {
-    st = launchTables(b, b->full, stream);
+    st = launchAllGroups(b, stream);
     if (st != CR_OK) return st;
   }
   CR_HIP(ctx, hipEventRecord(b->ev1, stream));
@@ -436,6 +488,11 @@ cr_status cr_batch_create(cr_context* ctx, const cr_view* views, int32_t numView
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->capStream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evFork, hipEventDisableTiming);
+  for (int g = 1; g < kMaxGroups && e == hipSuccess; g++) {
+    e = hipStreamCreateWithFlags(&b->side[g], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evJoin[g], hipEventDisableTiming);
+  }
   if (e != hipSuccess) {
     cr_batch_destroy(b);
     return fail(ctx, CR_ERR_DEVICE, "cr_batch_create: %s", hipGetErrorString(e));
@@ -453,10 +510,19 @@ cr_status cr_batch_destroy(cr_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (DevBuf* d : {&b->triSubtris, &b->triHeader, &b->triData, &b->subtriTri, &b->chunkBase, &b->binCnt, &b->binOff,
                     &b->binList, &b->tileStart, &b->tileCount, &b->tileList, &b->counters, &b->saved,
-                    &b->full.params, &b->full.starts, &b->subset.params, &b->subset.starts})
+                    &b->subset.params, &b->subset.starts})
     d->release();
+  for (Tables& t : b->groups) {
+    t.params.release();
+    t.starts.release();
+  }
   if (b->hostCounters) (void)hipHostFree(b->hostCounters);
   if (b->capStream) (void)hipStreamDestroy(b->capStream);
+  for (int g = 1; g < kMaxGroups; g++) {
+    if (b->side[g]) (void)hipStreamDestroy(b->side[g]);
+    if (b->evJoin[g]) (void)hipEventDestroy(b->evJoin[g]);
+  }
+  if (b->evFork) (void)hipEventDestroy(b->evFork);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   delete b;
@@ -504,7 +570,7 @@ cr_status cr_batch_autotune(cr_batch* b, void* stream_) {
       b->tilesPerGroup = W;
       b->banded = banded;
       b->coarseThreads = trial < 3 ? bestCoarse : (bestCoarse == 256 ? 1024 : 256);
-      st = buildTables(b, b->full, allViews(b));
+      st = buildAll(b);
       if (st != CR_OK) return st;
       float ms[3];
       for (int rep = 0; rep < 4; rep++) {
@@ -526,7 +592,29 @@ cr_status cr_batch_autotune(cr_batch* b, void* stream_) {
   b->tilesPerGroup = bestW;
   b->banded = bestBanded;
   b->coarseThreads = bestCoarse;
-  st = buildTables(b, b->full, allViews(b));
+  // 4. Concurrent view groups: the views split into K cost-balanced groups
+  //    whose launch chains run side by side (K = 1, 2 or 4).
+  int bestK = 1;
+  for (int K : {2, 4}) {
+    if (K > (int)b->views.size()) break;
+    b->numGroups = K;
+    st = buildAll(b);
+    if (st != CR_OK) return st;
+    float ms[3];
+    for (int rep = 0; rep < 4; rep++) {
+      st = saveSurfaces(b, stream, true);
+      if (st == CR_OK) st = drawBatch(b, stream, false);
+      if (st != CR_OK) return st;
+      if (rep > 0) ms[rep - 1] = b->lastMs;
+    }
+    std::sort(ms, ms + 3);
+    if (ms[1] < bestMs) {
+      bestMs = ms[1];
+      bestK = K;
+    }
+  }
+  b->numGroups = bestK;
+  st = buildAll(b);
   if (st == CR_OK) st = saveSurfaces(b, stream, true);
   if (st != CR_OK) return st;
   CR_HIP(ctx, hipStreamSynchronize(stream));
@@ -553,6 +641,21 @@ cr_status cr_batch_reserve(cr_batch* b, int32_t view, int64_t maxSubtris, int64_
   return CR_OK;
 }
 
+cr_status cr_batch_configure(cr_batch* b, const cr_batch_config* c) {
+  if (!b || !c) return fail(nullptr, CR_ERR_INVALID, "cr_batch_configure: NULL argument");
+  if ((c->fine_tiles_per_group != 2 && c->fine_tiles_per_group != 4 && c->fine_tiles_per_group != 8) ||
+      (c->fine_banded != 0 && c->fine_banded != 1) || (c->coarse_threads != 256 && c->coarse_threads != 1024) ||
+      c->view_groups < 1 || c->view_groups > kMaxGroups)
+    return fail(b->ctx, CR_ERR_INVALID, "cr_batch_configure: fine tiles 2/4/8, banded 0/1, coarse 256/1024, groups 1..4");
+  b->tilesPerGroup = c->fine_tiles_per_group;
+  b->banded = c->fine_banded;
+  b->coarseThreads = c->coarse_threads;
+  b->numGroups = c->view_groups;
+  b->laidOut = false;                 // tables are rebuilt on the next draw
+  b->graphValid = false;
+  return CR_OK;
+}
+
 cr_status cr_batch_use_graph(cr_batch* b, int32_t enable) {
   if (!b) return fail(nullptr, CR_ERR_INVALID, "cr_batch_use_graph: batch is NULL");
   b->useGraph = enable != 0;
@@ -566,6 +669,7 @@ cr_status cr_batch_get_info(const cr_batch* b, cr_batch_info* out) {
   out->fine_tiles_per_group = b->tilesPerGroup;
   out->fine_banded = b->banded;
   out->coarse_threads = b->coarseThreads;
+  out->view_groups = b->numGroups;
   out->uses_graph = b->useGraph ? 1 : 0;
   out->tuned = b->tuned ? 1 : 0;
   out->retries = b->retries;

@@ -39,7 +39,7 @@ EXPORTS = [
     "cr_get_stats", "cr_get_counters", "cr_get_profiling_info", "cr_set_debug_params", "cr_abi_version",
     "cr_batch_create", "cr_batch_destroy", "cr_batch_draw", "cr_batch_autotune", "cr_batch_reserve",
     "cr_batch_use_graph", "cr_batch_get_info", "cr_batch_get_view_counters",
-    "cr_interpolate", "cr_interpolate_backward",
+    "cr_interpolate", "cr_interpolate_backward", "cr_batch_configure",
 ]
 
 
@@ -86,7 +86,7 @@ class BatchInfo(ctypes.Structure):
                 ("uses_graph", ctypes.c_int32), ("tuned", ctypes.c_int32), ("retries", ctypes.c_int64),
                 ("reserved_bytes", ctypes.c_int64), ("last_gpu_ms", ctypes.c_float),
                 ("tuned_gpu_ms", ctypes.c_float), ("fine_banded", ctypes.c_int32),
-                ("coarse_threads", ctypes.c_int32)]
+                ("coarse_threads", ctypes.c_int32), ("view_groups", ctypes.c_int32)]
 
     def as_dict(self):
         return {n: (float(getattr(self, n)) if t is ctypes.c_float else int(getattr(self, n)))
@@ -148,6 +148,7 @@ def load(path: str | None = None):
         "cr_batch_get_info": [vp, ctypes.POINTER(BatchInfo)],
         "cr_batch_get_view_counters": [vp, i32, ctypes.POINTER(Counters)],
         "cr_interpolate": [vp, vp, vp, i64, vp, i32, vp, i32, vp, vp, vp],
+        "cr_batch_configure": [vp, vp],
         "cr_interpolate_backward": [vp, vp, i32, i32, vp, i64, vp, i32, vp, i32, vp, vp, vp, vp],
     }
     for name, args in sig.items():
@@ -438,6 +439,10 @@ class Batch:
     def reserve(self, view, max_subtris, max_bin_entries, max_tile_entries):
         _check(load().cr_batch_reserve(self._h, int(view), int(max_subtris), int(max_bin_entries),
                                        int(max_tile_entries)), self._raster.ctx)
+
+    def configure(self, fine_tiles_per_group=4, fine_banded=0, coarse_threads=256, view_groups=1):
+        c = (ctypes.c_int32 * 4)(fine_tiles_per_group, fine_banded, coarse_threads, view_groups)
+        _check(load().cr_batch_configure(self._h, c), self._raster.ctx)
 
     def use_graph(self, enable=True):
         _check(load().cr_batch_use_graph(self._h, 1 if enable else 0), self._raster.ctx)

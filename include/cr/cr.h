@@ -231,6 +231,7 @@ typedef struct cr_batch_info {
     float tuned_gpu_ms;            /* best GPU time seen by the autotuner */
     int32_t fine_banded;           /* 1: fine tile groups in per-XCD bands; 0: dealt round-robin */
     int32_t coarse_threads;        /* coarse-stage workgroup size in use */
+    int32_t view_groups;           /* views split into this many concurrently launched groups */
 } cr_batch_info;
 
 /* Copies the view list; validates each view as cr_set_surfaces /
@@ -244,8 +245,9 @@ cr_status cr_batch_draw(cr_batch* batch, void* stream);
  * queue demand, sets per-view queue depths to it plus a margin, orders every
  * stage's views longest-first by the measured entries per bin and per tile,
  * times every fine-stage workgroup shape with banded and round-robin tile
- * groups and both coarse workgroup sizes and keeps the fastest, then
- * captures the graph.  Bin size is chosen
+ * groups and both coarse workgroup sizes, then 1, 2 or 4 cost-balanced view
+ * groups whose launch chains run concurrently (forked streams, parallel
+ * graph branches), keeps the fastest, then captures the graph.  Bin size is chosen
  * per view from its resolution (the smallest bins with at most 256 bins).
  * Surfaces of views without deferred clear are restored afterwards. */
 cr_status cr_batch_autotune(cr_batch* batch, void* stream);
@@ -256,6 +258,17 @@ cr_status cr_batch_reserve(cr_batch* batch, int32_t view, int64_t max_subtris, i
                            int64_t max_tile_entries);
 /* 0: launch the stages directly on each draw instead of replaying a graph. */
 cr_status cr_batch_use_graph(cr_batch* batch, int32_t enable);
+/* Sets the launch configuration the autotuner otherwise chooses (images do
+ * not depend on it): fine tiles per workgroup (2, 4 or 8), fine tile groups
+ * in XCD bands (1) or round-robin (0), coarse workgroup size (256 or 1024),
+ * concurrent view groups (1..4). */
+typedef struct cr_batch_config {
+    int32_t fine_tiles_per_group;
+    int32_t fine_banded;
+    int32_t coarse_threads;
+    int32_t view_groups;
+} cr_batch_config;
+cr_status cr_batch_configure(cr_batch* batch, const cr_batch_config* config);
 cr_status cr_batch_get_info(const cr_batch* batch, cr_batch_info* out);
 /* Counters of one view from the last draw (fragments and visible_subtris are 0). */
 cr_status cr_batch_get_view_counters(const cr_batch* batch, int32_t view, cr_counters* out);

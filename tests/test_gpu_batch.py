@@ -101,6 +101,33 @@ def test_batch_equals_plain_draws_on_every_replay():
         br.close()
 
 
+@pytest.mark.parametrize("cfg", [(2, 1, 1024, 1), (8, 0, 256, 2), (4, 1, 256, 4), (4, 0, 1024, 3)])
+def test_batch_configurations_give_identical_images(cfg):
+    """Every launch configuration (fine shape, placement, coarse size,
+    concurrent view groups) draws the same images, graph or direct."""
+    _torch()
+    import cudaraster
+    views = mixed_views(9, seed=12)
+    refs = [plain(v) if v.deferred_clear else None for v in views]
+    br = cudaraster.BatchRenderer(views)
+    try:
+        br.batch.configure(*cfg)
+        for graph in (True, False):
+            br.batch.use_graph(graph)
+            br.draw()
+            info = br.batch.info()
+            assert (info["fine_tiles_per_group"], info["fine_banded"], info["coarse_threads"],
+                    info["view_groups"]) == cfg
+            for i, v in enumerate(views):
+                if refs[i] is None:
+                    continue
+                got = br.read(i)
+                for k in ("color", "depth", "triid"):
+                    assert np.array_equal(got[k], refs[i][k]), f"{cfg} graph={graph}: view {i} {k}"
+    finally:
+        br.close()
+
+
 def test_batch_against_oracle(oracle_lib):
     _torch()
     import cudaraster
