@@ -152,12 +152,15 @@ __device__ __forceinline__ int64_t wmul64(int64_t a, int64_t b) { return (int64_
 __device__ __forceinline__ uint32_t wabs32(int32_t a) { return a < 0 ? 0u - (uint32_t)a : (uint32_t)a; }
 
 // toABGR (Util.inl:30-38): per channel floor(255*c + 1/2) of the exact
-// product, saturated to [0,255], NaN -> 0.  Evaluated exactly in F64.
+// product, saturated to [0,255], NaN -> 0.  Evaluated exactly in F64
+// (24x8-bit product + 1/2), then v_cvt_u32_f64, which truncates toward zero
+// (floor for x >= 0), saturates out-of-range values to [0, 2^32-1] and maps
+// NaN to 0, and a min with 255: no branches.
 __device__ __forceinline__ uint32_t abgrChannel(float c) {
-  double x = fma((double)c, 255.0, 0.5);      // exact: 24x8-bit product + 1/2
-  if (!(x >= 0.0)) return 0u;                 // negative or NaN
-  if (x >= 255.0) return 255u;
-  return (uint32_t)x;                         // truncation == floor for x >= 0
+  const double x = fma((double)c, 255.0, 0.5);
+  uint32_t u;
+  asm("v_cvt_u32_f64 %0, %1" : "=v"(u) : "v"(x));
+  return min(u, 255u);
 }
 __device__ __forceinline__ uint32_t toABGR(float r, float g, float b, float a) {
   return abgrChannel(r) | (abgrChannel(g) << 8) | (abgrChannel(b) << 16) | (abgrChannel(a) << 24);

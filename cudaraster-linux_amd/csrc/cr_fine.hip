@@ -37,8 +37,8 @@ struct Rec {
   static constexpr int N = 1 << S;
   static constexpr int R = 0;                  // 3*N coverage thresholds, R[k*N+i]
   static constexpr int D = 3 * N;              // per edge: dy, -dx (so that L = lx16*dy + ly16*(-dx))
-  static constexpr int Z = D + 6;              // depth plane: zx lo16, zx hi16, zy lo16, zy hi16
-  static constexpr int ZS = Z + 4;             // depth at the tile's first pixel, per sample [N]
+  static constexpr int Z = D + 6;              // depth plane: (zx lo16, zy lo16), (zx hi16, zy hi16)
+  static constexpr int ZS = Z + 2;             // depth at the tile's first pixel, per sample [N]
   static constexpr int SLOT = ZS + N;          // subtriangle slot
   static constexpr int TRI = SLOT + 1;         // source triangle index
   static constexpr int SH = TRI + 1;           // in-order shading: W/U/V planes (9) + colours (12)
@@ -73,6 +73,13 @@ __device__ __forceinline__ uint32_t shl16Add(uint32_t hi, uint32_t lo) {
   uint32_t r;
   asm("v_lshl_add_u32 %0, %1, 16, %2" : "=v"(r) : "v"(hi), "v"(lo));
   return r;
+}
+
+// a.lo*b.lo + a.hi*b.hi + c over unsigned 16-bit halves, modulo 2^32
+// (v_dot2_u32_u16 without clamping).
+typedef unsigned short crUShort2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t dot2u(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_udot2(__builtin_bit_cast(crUShort2, a), __builtin_bit_cast(crUShort2, b), c, false);
 }
 
 __device__ __forceinline__ uint32_t waveMax(uint32_t v) {
@@ -139,7 +146,8 @@ __device__ __forceinline__ uint32_t shadeGouraud(bool lerpOn, int code, int px, 
 }
 
 __device__ __forceinline__ float4 varying0(const uint8_t* verts, uint32_t vi, int stride) {
-  return *reinterpret_cast<const float4*>(verts + (size_t)vi * stride + 16);
+  // Unsigned 32x32 -> 64-bit product: one v_mad_u64_u32 with the base.
+  return *reinterpret_cast<const float4*>(verts + (uint64_t)vi * (uint32_t)stride + 16);
 }
 
 constexpr int kFineWaves = 4;     // tiles (waves) per workgroup
@@ -216,6 +224,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
   const int32_t Ty = (ty << (kTileLog2 + kSubpixelLog2)) - (P.viewportHeight << (kSubpixelLog2 - 1));
   const int32_t lx16 = lx << kSubpixelLog2, ly16 = ly << kSubpixelLog2;
   const uint32_t lxN = (uint32_t)(lx * N), lyN = (uint32_t)(ly * N);
+  const uint32_t pixN = lxN | (lyN << 16);
   const uint32_t tileSx = (uint32_t)(tx * 8 * N), tileSy = (uint32_t)(ty * 8 * N);
   constexpr CentroidOrder<S> co;
 
@@ -228,7 +237,9 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
     // Hierarchical Z: a triangle whose conservative zmin is not nearer than
     // the tile's farthest stored depth fails LESS at every sample.
     uint32_t tileZMax = 0xFFFFFFFFu;
-    if (hiz) {
+    if (hiz && c == 0 && P.deferredClear) {
+      tileZMax = P.clearDepth;               // every sample still holds the clear depth
+    } else if (hiz) {
       uint32_t mx = dep[0];
 #pragma unroll
       for (int i = 1; i < N; i++) mx = max(mx, dep[i]);
@@ -277,10 +288,8 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
         r[L::D + 2 * k + 1] = (uint32_t)(-dx);
       }
       if (DEPTH) {
-        r[L::Z] = z.x & 0xFFFFu;
-        r[L::Z + 1] = z.x >> 16;
-        r[L::Z + 2] = z.y & 0xFFFFu;
-        r[L::Z + 3] = z.y >> 16;
+        r[L::Z] = (z.x & 0xFFFFu) | (z.y << 16);
+        r[L::Z + 1] = (z.x >> 16) | (z.y & 0xFFFF0000u);
         // Depth at each sample of the tile's first pixel: one full 32-bit
         // product per axis, then the sample offsets (< 8) as shifts and adds.
         const uint32_t base = z.x * tileSx + z.y * tileSy + z.z;
@@ -329,11 +338,9 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
       uint32_t dz[N];
       bool anyWr = false;
       if (DEPTH) {
-        // zx*lxN + zy*lyN mod 2^32 from 16-bit halves: lxN, lyN < 64, so
-        // every product fits the full-rate 24-bit multiplier.
-        const uint32_t hi = __umul24(r[L::Z + 1], lxN) + __umul24(r[L::Z + 3], lyN);
-        const uint32_t lo = __umul24(r[L::Z], lxN) + __umul24(r[L::Z + 2], lyN);
-        const uint32_t pl = shl16Add(hi, lo);
+        // zx*lxN + zy*lyN mod 2^32 from the 16-bit halves of zx and zy, one
+        // dot product per half (lxN, lyN < 64: no product or sum overflows).
+        const uint32_t pl = shl16Add(dot2u(r[L::Z + 1], pixN, 0u), dot2u(r[L::Z], pixN, 0u));
 #pragma unroll
         for (int i = 0; i < N; i++) {
           dz[i] = r[L::ZS + i] + pl;
@@ -395,14 +402,20 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
   // ---- deferred shading: once per (pixel, distinct winning triangle) -------
   uint32_t tid[N];
   if (VIS) {
-    uint32_t pending = 0;
+    // Samples that no triangle reached keep the surface's id.
 #pragma unroll
-    for (int i = 0; i < N; i++) pending |= (aux[i] != 0xFFFFFFFFu) ? (1u << i) : 0u;
-    while (pending) {
-      const int first = __ffs(pending) - 1;
-      uint32_t v = aux[0];
+    for (int i = 0; i < N; i++) {
+      tid[i] = 0xFFFFFFFFu;
+      if (!P.deferredClear && idOn && aux[i] == 0xFFFFFFFFu) tid[i] = triid[texel + i * 8];
+    }
+    // Each pass shades the winner of this lane's first unresolved sample and
+    // resolves every sample it won (aux -> ~0); a lane leaves when none is
+    // left.
+    for (;;) {
+      uint32_t v = aux[N - 1];
 #pragma unroll
-      for (int i = 1; i < N; i++) v = (i == first) ? aux[i] : v;
+      for (int i = N - 2; i >= 0; i--) v = (aux[i] != 0xFFFFFFFFu) ? aux[i] : v;
+      if (v == 0xFFFFFFFFu) break;
       const uint32_t slot = v >> 4;
       const uint32_t tri = (slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris];
       uint32_t src = P.constColor;
@@ -414,17 +427,11 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
       }
 #pragma unroll
       for (int i = 0; i < N; i++) {
-        if ((pending & (1u << i)) && aux[i] == v) {
-          if (P.blend != kBlendDepthOnly) col[i] = src;
-          tid[i] = tri;
-          pending &= ~(1u << i);
-        }
+        const bool won = aux[i] == v;
+        if (P.blend != kBlendDepthOnly) col[i] = won ? src : col[i];
+        tid[i] = won ? tri : tid[i];
+        aux[i] = won ? 0xFFFFFFFFu : aux[i];
       }
-    }
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      if (aux[i] == 0xFFFFFFFFu)
-        tid[i] = P.deferredClear ? 0xFFFFFFFFu : (idOn ? triid[texel + i * 8] : 0xFFFFFFFFu);
     }
   } else {
 #pragma unroll

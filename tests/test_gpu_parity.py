@@ -70,6 +70,37 @@ def test_blend_modes(oracle_lib, blend):
     assert_parity(render_gpu(s), oracle_lib.render(s), f"blend {blend}")
 
 
+@pytest.mark.parametrize("blend,lerp", [(0, True), (0, False), (1, True), (1, False)])
+def test_extreme_colours(oracle_lib, blend, lerp):
+    """Vertex colours that exercise every branch of toABGR's saturation:
+    NaN, +-inf, negative, above 1, huge, and values a few ulp either side of
+    the k/255 rounding boundaries -- through deferred shading (replace) and
+    in-loop shading (src-over), interpolated or flat (without LERP the
+    fragment takes the third vertex's colour).  Colours must match the oracle
+    exactly."""
+    _torch()
+    from cudaraster import scenes
+    flags = scenes.FLAG_DEPTH | (scenes.FLAG_LERP if lerp else 0)
+    s = scenes.random_tris(5000, 192, 128, samples_log2=2, seed=91 + blend, area_range=(2.0, 1500.0),
+                           blend=blend, flags=flags)
+    rng = np.random.default_rng(5 + 2 * blend + lerp)
+    special = np.array([np.nan, np.inf, -np.inf, -0.5, -0.0, 1.5, 1e30, -1e30, 0.0, 1.0], np.float32)
+    k = rng.integers(0, 256, size=4096)
+    boundary = ((k + 0.5) / 255.0).astype(np.float32)
+    near = np.nextafter(boundary, np.where(rng.random(4096) < 0.5, np.float32(0), np.float32(2)))
+    cols = s.vertices[:, 4:8].copy()
+    u = rng.random(cols.shape)
+    cols = np.where(u < 0.1, rng.choice(special, size=cols.shape), cols)
+    cols = np.where((u >= 0.1) & (u < 0.3), rng.choice(boundary, size=cols.shape), cols)
+    cols = np.where((u >= 0.3) & (u < 0.5), rng.choice(near, size=cols.shape), cols)
+    s.vertices[:, 4:8] = cols.astype(np.float32)
+    got, ref = render_gpu(s), oracle_lib.render(s)
+    c = ref["color"].view(np.uint8)
+    assert (c == 0).sum() > 20 and (c == 255).sum() > 20, "workload must reach both saturation ends"
+    assert_parity(got, ref, f"extreme colours, blend {blend}, lerp {lerp}")
+    assert np.array_equal(got["color"], ref["color"])
+
+
 @pytest.mark.parametrize("flags", [0, 1, 2, 3])
 def test_render_mode_flags(oracle_lib, flags):
     _torch()
