@@ -172,18 +172,28 @@ __device__ __forceinline__ int xcdGroup(int groups) {
   return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
 }
 
+// LDS words of one wave's record array.
+template <int S, bool VIS>
+constexpr int recWords() { return kChunk * (VIS ? Rec<S>::SIZE_VIS : Rec<S>::SIZE_FULL); }
+
 // Fine raster of tile group `group` (W tiles, one wave each) of one draw.
 // COUNT: counting draw (fragment totals per tile into P.tileFrags).
-template <int S, bool VIS, bool COUNT, int W, bool DEPTH>
-__device__ __forceinline__ void fineBody(const Params& P, const int group, const uint4* __restrict__ hdr,
-                                         const uint4* __restrict__ tdata, const uint32_t* __restrict__ tileList,
-                                         const uint8_t* __restrict__ verts, const uint32_t* __restrict__ subtriTri,
-                                         uint32_t* __restrict__ color, uint32_t* __restrict__ depth,
-                                         uint32_t* __restrict__ triid) {
+// BL: blend mode known at compile time (in-order shading), or -1 for P.blend.
+// recAll: the workgroup's LDS record arrays, W * recWords<S, VIS>() words.
+template <int S, bool VIS, bool COUNT, int W, bool DEPTH, int BL>
+__device__ __forceinline__ void fineBody(const Params& P, const int group, uint32_t* __restrict__ recAll) {
   constexpr int N = 1 << S;
   using L = Rec<S>;
   constexpr int RS = VIS ? L::SIZE_VIS : L::SIZE_FULL;
-  __shared__ __attribute__((aligned(16))) uint32_t recAll[W * kChunk * RS];
+  const uint4* __restrict__ hdr = reinterpret_cast<const uint4*>(P.triHeader);
+  const uint4* __restrict__ tdata = reinterpret_cast<const uint4*>(P.triData);
+  const uint32_t* __restrict__ tileList = P.tileList;
+  const uint8_t* __restrict__ verts = P.vertices;
+  const uint32_t* __restrict__ subtriTri = P.subtriTri;
+  uint32_t* __restrict__ color = P.color;
+  uint32_t* __restrict__ depth = P.depth;
+  uint32_t* __restrict__ triid = P.triid;
+  const int32_t blend = BL >= 0 ? BL : P.blend;
 
   if (P.counters->overflow) return;
   const int lane = threadIdx.x & 63;
@@ -390,7 +400,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
         for (int i = 0; i < N; i++) {
           if (wr[i]) {
             if (DEPTH) dep[i] = dz[i];
-            col[i] = blendPixel(P.blend, src, col[i]);
+            col[i] = blendPixel(blend, src, col[i]);
             aux[i] = tri;
           }
         }
@@ -428,7 +438,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
 #pragma unroll
       for (int i = 0; i < N; i++) {
         const bool won = aux[i] == v;
-        if (P.blend != kBlendDepthOnly) col[i] = won ? src : col[i];
+        if (blend != kBlendDepthOnly) col[i] = won ? src : col[i];
         tid[i] = won ? tri : tid[i];
         aux[i] = won ? 0xFFFFFFFFu : aux[i];
       }
@@ -438,7 +448,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
     for (int i = 0; i < N; i++) tid[i] = aux[i];
   }
 
-  const bool storeColor = P.blend != kBlendDepthOnly || P.deferredClear;
+  const bool storeColor = blend != kBlendDepthOnly || P.deferredClear;
 #pragma unroll
   for (int i = 0; i < N; i++) {
     if (storeColor) color[texel + i * 8] = col[i];
@@ -450,14 +460,26 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, const
   if (COUNT && lane == 0) P.tileFrags[tile] = (uint32_t)frags;
 }
 
+// In-order shading blends in the per-triangle loop: SrcOver and Additive
+// get their own copy of the body with the blend mode folded in (the choice
+// is uniform per draw); counting draws and the rare in-order Replace /
+// DepthOnly case use the generic copy.
+template <int S, bool VIS, bool COUNT, int W, bool DEPTH>
+__device__ __forceinline__ void fineDispatch(const Params& P, const int group, uint32_t* recAll) {
+  if (!VIS && !COUNT) {
+    if (P.blend == kBlendSrcOver) return fineBody<S, VIS, COUNT, W, DEPTH, kBlendSrcOver>(P, group, recAll);
+    if (P.blend == kBlendAdditive) return fineBody<S, VIS, COUNT, W, DEPTH, kBlendAdditive>(P, group, recAll);
+  }
+  fineBody<S, VIS, COUNT, W, DEPTH, -1>(P, group, recAll);
+}
+
 template <int S, bool VIS, bool COUNT, bool DEPTH>
 __global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P) {
+  __shared__ __attribute__((aligned(16))) uint32_t recAll[kFineWaves * recWords<S, VIS>()];
   const int groups = (P.numTiles + kFineWaves - 1) / kFineWaves;
   const int group = xcdGroup(groups);
   if (group >= groups) return;
-  fineBody<S, VIS, COUNT, kFineWaves, DEPTH>(P, group, reinterpret_cast<const uint4*>(P.triHeader),
-                                             reinterpret_cast<const uint4*>(P.triData), P.tileList, P.vertices,
-                                             P.subtriTri, P.color, P.depth, P.triid);
+  fineDispatch<S, VIS, COUNT, kFineWaves, DEPTH>(P, group, recAll);
 }
 
 // Batched draws: the tile groups of the views of one pipe variant form one
@@ -470,13 +492,12 @@ __global__ void __launch_bounds__(64 * W) fineBatchKernel(const Params* __restri
                                                           const int32_t* __restrict__ groupStart,
                                                           const int32_t* __restrict__ order, int numViews,
                                                           int totalGroups, int banded) {
+  __shared__ __attribute__((aligned(16))) uint32_t recAll[W * recWords<S, VIS>()];
   const int g = banded ? xcdGroup(totalGroups) : (int)blockIdx.x;
   if (g >= totalGroups) return;
   const int k = findView(groupStart, numViews, g);
   const Params& P = views[order[k]];
-  fineBody<S, VIS, false, W, DEPTH>(P, g - groupStart[k], reinterpret_cast<const uint4*>(P.triHeader),
-                                    reinterpret_cast<const uint4*>(P.triData), P.tileList, P.vertices,
-                                    P.subtriTri, P.color, P.depth, P.triid);
+  fineDispatch<S, VIS, false, W, DEPTH>(P, g - groupStart[k], recAll);
 }
 
 // Variant v = (samplesLog2 * 2 + deferred shading) * 2 + depth test.
