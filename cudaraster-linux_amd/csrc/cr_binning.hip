@@ -203,17 +203,24 @@ __device__ __forceinline__ void coarseRecord(const TileTest& r, bool valid, cons
   bool coop = false;
   int ux0 = 0, ux1 = -1, uy0 = 0, uy1 = -1;
   if (!allSmall) {
-    // Union box of the wave's triangles and the largest single box.
+    // Walk the union box of the wave's triangles tile by tile (one ballot
+    // per tile) when that costs less than the scattered path below: its own
+    // box per lane for small boxes (as many steps as the largest, each an
+    // LDS atomic that lanes hitting the same tile serialise: weight 2), plus
+    // one wave step per 64 tiles for each large box.
+    const bool bigOne = has && w * h > kCoopArea;
     int a0 = has ? r.x0 : 1 << 20, a1 = has ? r.x1 : -1, b0 = has ? r.y0 : 1 << 20, b1 = has ? r.y1 : -1;
-    int area = has ? w * h : 0;
+    int smallArea = (has && !bigOne) ? w * h : 0;
+    int bigSteps = bigOne ? (h + 3) >> 2 : 0;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
       a0 = min(a0, __shfl_xor(a0, o, 64)); a1 = max(a1, __shfl_xor(a1, o, 64));
       b0 = min(b0, __shfl_xor(b0, o, 64)); b1 = max(b1, __shfl_xor(b1, o, 64));
-      area = max(area, __shfl_xor(area, o, 64));
+      smallArea = max(smallArea, __shfl_xor(smallArea, o, 64));
+      bigSteps += __shfl_xor(bigSteps, o, 64);
     }
     ux0 = a0; ux1 = a1; uy0 = b0; uy1 = b1;
-    coop = (ux1 - ux0 + 1) * (uy1 - uy0 + 1) <= 2 * area;
+    coop = (ux1 - ux0 + 1) * (uy1 - uy0 + 1) <= 2 * smallArea + bigSteps;
   }
   if (coop) {
     for (int y = uy0; y <= uy1; y++)
