@@ -279,8 +279,11 @@ __device__ __noinline__ int setupClipped(const Params P, int t, int3 vidx, float
 }
 
 // One triangle: returns its number of subtriangles and their first slot.
-__device__ __forceinline__ int setupOne(const Params& P, int t, int* base) {
+// When the triangle is set up without clipping, *hv receives the header's
+// vertex words (as written to P.triHeader[t]) and *hvOk is set.
+__device__ __forceinline__ int setupOne(const Params& P, int t, int* base, uint4* hv, bool* hvOk) {
   *base = t;
+  *hvOk = false;
   int3 vidx;
   const int32_t* ip = P.indices + (int64_t)t * 3;
   vidx.x = ip[0]; vidx.y = ip[1]; vidx.z = ip[2];
@@ -321,9 +324,14 @@ __device__ __forceinline__ int setupOne(const Params& P, int t, int* base) {
     if (loxy >= -32768 && hixy <= 32767 && hixy - loxy <= aabbLimit) {
       int res = prepare(P, p0, p1, p2, lo, hi, d1, d2, area);
       P.triSubtris[t] = (res == 0) ? 1 : 0;
-      if (res == 0)
+      if (res == 0) {
         setupTriangle(P, &P.triHeader[t], &P.triData[t], vidx, v0, v1, v2, make_float2(0.0f, 0.0f),
                       make_float2(1.0f, 0.0f), make_float2(0.0f, 1.0f), p0, p1, p2, rcpW, d1, d2, area);
+        *hv = make_uint4(((uint32_t)p0.x & 0xFFFFu) | ((uint32_t)p0.y << 16),
+                         ((uint32_t)p1.x & 0xFFFFu) | ((uint32_t)p1.y << 16),
+                         ((uint32_t)p2.x & 0xFFFFu) | ((uint32_t)p2.y << 16), 0u);
+        *hvOk = true;
+      }
       return res == 0 ? 1 : 0;
     }
   }
@@ -419,7 +427,9 @@ __device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
 
   // ---- setup ---------------------------------------------------------------
   int base = t, k = 0;
-  if (t < P.numTris) k = setupOne(P, t, &base);
+  uint4 hvOwn = make_uint4(0u, 0u, 0u, 0u);   // header vertex words kept from setup (no re-read)
+  bool hvOwnOk = false;
+  if (t < P.numTris) k = setupOne(P, t, &base, &hvOwn, &hvOwnOk);
 
   for (int i = threadIdx.x; i < P.numBins; i += kSetupChunk) cnt[i] = 0;
   // Window = up to kSetupChunk consecutive subtriangles of the chunk in
@@ -441,7 +451,7 @@ __device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
       bb.x0 = 1; bb.x1 = 0; bb.y0 = 1; bb.y1 = 0;
       __syncthreads();
       if (k == 1) {
-        hv = *reinterpret_cast<const uint4*>(&P.triHeader[slot]);
+        hv = hvOwnOk ? hvOwn : *reinterpret_cast<const uint4*>(&P.triHeader[slot]);
         bb = binBoxOf(P, hv);
       }
     } else {
@@ -531,11 +541,11 @@ __device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
   }
 }
 
-__global__ void __launch_bounds__(kSetupChunk) setupBinKernel(const Params P) { setupBinBody(P, blockIdx.x); }
+__global__ void __launch_bounds__(kSetupChunk, 8) setupBinKernel(const Params P) { setupBinBody(P, blockIdx.x); }
 
 // Batched draws: one workgroup per chunk of every view; blockStart[v] is the
 // first workgroup of view v (numViews + 1 entries).
-__global__ void __launch_bounds__(kSetupChunk) setupBinBatchKernel(const Params* __restrict__ views,
+__global__ void __launch_bounds__(kSetupChunk, 8) setupBinBatchKernel(const Params* __restrict__ views,
                                                                    const int32_t* __restrict__ blockStart,
                                                                    const int32_t* __restrict__ order,
                                                                    int numViews) {
