@@ -571,11 +571,9 @@ cr_status cr_interpolate_backward(cr_context* ctx, const int32_t* triImage, int3
       ((uintptr_t)tris & 3))
     return fail(ctx, CR_ERR_INVALID, "cr_interpolate_backward: bad sizes or buffers");
   CR_HIP(ctx, hipSetDevice(ctx->device));
-  const size_t nPos = (size_t)numVertices * 4, nAttr = (size_t)numVertices * (size_t)numChannels;
-  CR_HIP(ctx, ctx->interpScratch.ensure((nPos + nAttr) * sizeof(double)));
-  double* acc = ctx->interpScratch.as<double>();
   InterpGeom G{width, height, width, numChannels, numTris, numVertices};
-  launchInterpBackward(G, triImage, pos, attr, tris, gradOut, acc, acc + nPos, gradPos, gradAttr,
+  CR_HIP(ctx, ctx->interpScratch.ensure((size_t)interpBackwardScratchBytes(G)));
+  launchInterpBackward(G, triImage, pos, attr, tris, gradOut, ctx->interpScratch.as<void>(), gradPos, gradAttr,
                        static_cast<hipStream_t>(stream));
   CR_HIP(ctx, hipGetLastError());
   return CR_OK;

@@ -93,63 +93,205 @@ __global__ void __launch_bounds__(256) interpForwardKernel(const InterpGeom G, c
   if (triImage) triImage[p] = (int32_t)t;
 }
 
-// One lane per pixel.  dL/db_k = sum_c g_c a_k,c; dL/dc_j = (dL/db_j -
-// sum_k b_k dL/db_k) / (c_0 + c_1 + c_2); c_k = x'_{k+1} y'_{k+2} -
-// y'_{k+1} x'_{k+2}; x' = x - qx w, y' = y - qy w.
-__global__ void __launch_bounds__(256) interpBackwardKernel(const InterpGeom G, const int32_t* __restrict__ triImage,
-                                                            const float4* __restrict__ pos,
-                                                            const float* __restrict__ attr,
-                                                            const int3* __restrict__ tris,
-                                                            const float* __restrict__ gradOut,
-                                                            double* __restrict__ accPos,
-                                                            double* __restrict__ accAttr) {
+// Backward over the visible pixels taken in triangle order (order[]: pixel
+// indices grouped by triangle id, start[numTris] of them), 256 per
+// workgroup.  Phase 1, one lane per pixel: barycentrics, dL/db_k =
+// sum_c g_c a_k,c, dL/dc_j = (dL/db_j - sum_k b_k dL/db_k) / (c_0 + c_1 +
+// c_2), then through c_k = x'_{k+1} y'_{k+2} - y'_{k+1} x'_{k+2} and the
+// shear x' = x - qx w, y' = y - qy w to the clip x, y, w of each corner; the
+// results are staged in LDS.  Phases 2 and 3 issue the FP64 atomics
+// transposed, consecutive lanes on consecutive (pixel, corner, channel)
+// entries: with pixels grouped by triangle those land on consecutive
+// addresses, which device-scope atomics serve several times faster than
+// scattered ones.
+constexpr int kBwdThreads = 256;
+
+__global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const InterpGeom G,
+                                                                    const int32_t* __restrict__ triImage,
+                                                                    const uint32_t* __restrict__ order,
+                                                                    const uint32_t* __restrict__ start,
+                                                                    const float4* __restrict__ pos,
+                                                                    const float* __restrict__ attr,
+                                                                    const int3* __restrict__ tris,
+                                                                    const float* __restrict__ gradOut,
+                                                                    double* __restrict__ accPos,
+                                                                    double* __restrict__ accAttr) {
+  __shared__ double sB[kBwdThreads][3];        // barycentrics
+  __shared__ double sP[kBwdThreads][9];        // dL/d(x, y, w) per corner
+  __shared__ int32_t sV[kBwdThreads][3];       // vertex indices, sV[.][0] < 0: no pixel
+  __shared__ uint32_t sPix[kBwdThreads];
+  const int64_t total = start[G.numTris];
+  const int64_t j0 = (int64_t)blockIdx.x * kBwdThreads;
+  if (j0 >= total) return;                     // uniform: past the visible pixels
+  const int tid = threadIdx.x;
+  const int C = G.numChannels;
+
+  // ---- phase 1: one lane per pixel -----------------------------------------
+  sV[tid][0] = -1;
+  const int64_t j = j0 + tid;
+  if (j < total) {
+    const int64_t p = order[j];
+    const int32_t t = triImage[p];
+    const int3 v = tris[t];
+    const int32_t i[3] = {v.x, v.y, v.z};
+    bool ok = true;
+    for (int k = 0; k < 3; k++) ok = ok && i[k] >= 0 && (int64_t)i[k] < G.numVertices;
+    const int x = (int)(p % G.width), y = (int)(p / G.width);
+    const double qx = (2.0 * x + 1.0) / G.width - 1.0, qy = (2.0 * y + 1.0) / G.height - 1.0;
+    Bary B;
+    if (ok) {
+      baryAt(pos, i, qx, qy, B);
+      ok = B.ok;
+    }
+    if (ok) {
+      const float* g = gradOut + p * C;
+      double dB[3] = {0.0, 0.0, 0.0};
+      for (int k = 0; k < 3; k++) {
+        const float* a = attr + (int64_t)i[k] * C;
+        double s = 0.0;
+        for (int c = 0; c < C; c++) s += (double)g[c] * (double)a[c];
+        dB[k] = s;
+      }
+      const double mean = B.b[0] * dB[0] + B.b[1] * dB[1] + B.b[2] * dB[2];
+      const double rs = 1.0 / B.sum;
+      double dxs[3] = {0.0, 0.0, 0.0}, dys[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double dc = (dB[k] - mean) * rs;
+        const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+        dxs[k1] += dc * B.ys[k2];
+        dys[k1] -= dc * B.xs[k2];
+        dxs[k2] -= dc * B.ys[k1];
+        dys[k2] += dc * B.xs[k1];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        sB[tid][k] = B.b[k];
+        sV[tid][k] = i[k];
+        sP[tid][3 * k] = dxs[k];
+        sP[tid][3 * k + 1] = dys[k];
+        sP[tid][3 * k + 2] = -B.qx * dxs[k] - B.qy * dys[k];
+      }
+      sPix[tid] = (uint32_t)p;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 2: attribute gradients b_k g_c, entry e = (pixel, corner, channel)
+  const int per = 3 * C;
+  for (int e = tid; e < kBwdThreads * per; e += kBwdThreads) {
+    const int q = e / per, r = e - q * per, k = r / C, c = r - k * C;
+    if (sV[q][0] < 0) continue;
+    const double gc = (double)gradOut[(int64_t)sPix[q] * C + c];
+    if (gc != 0.0) unsafeAtomicAdd(&accAttr[(int64_t)sV[q][k] * C + c], sB[q][k] * gc);
+  }
+  // ---- phase 3: position gradients, entry e = (pixel, corner, x/y/w) ------
+  for (int e = tid; e < kBwdThreads * 9; e += kBwdThreads) {
+    const int q = e / 9, r = e - q * 9, k = r / 3, comp = r - k * 3;
+    if (sV[q][0] < 0) continue;
+    const double d = sP[q][r];
+    if (d != 0.0) unsafeAtomicAdd(&accPos[(int64_t)sV[q][k] * 4 + (comp == 2 ? 3 : comp)], d);
+  }
+}
+
+// ---- grouping the visible pixels by triangle (counting sort) --------------
+
+__global__ void __launch_bounds__(256) countTrianglePixelsKernel(const InterpGeom G,
+                                                                 const int32_t* __restrict__ triImage,
+                                                                 uint32_t* __restrict__ cnt) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (int64_t)G.width * G.height) return;
   const int32_t t = triImage[p];
-  if (t < 0 || t >= G.numTris) return;
-  const int3 v = tris[t];
-  const int32_t i[3] = {v.x, v.y, v.z};
-  for (int k = 0; k < 3; k++)
-    if (i[k] < 0 || (int64_t)i[k] >= G.numVertices) return;
-  const int x = (int)(p % G.width), y = (int)(p / G.width);
-  const int C = G.numChannels;
-  const double qx = (2.0 * x + 1.0) / G.width - 1.0, qy = (2.0 * y + 1.0) / G.height - 1.0;
-  Bary B;
-  baryAt(pos, i, qx, qy, B);
-  if (!B.ok) return;
-  const float* g = gradOut + p * C;
-  double dB[3] = {0.0, 0.0, 0.0};
-  for (int k = 0; k < 3; k++) {
-    const float* a = attr + (int64_t)i[k] * C;
-    double* acc = accAttr + (int64_t)i[k] * C;
-    double s = 0.0;
-    for (int c = 0; c < C; c++) {
-      const double gc = (double)g[c];
-      s += gc * (double)a[c];
-      if (gc != 0.0) unsafeAtomicAdd(&acc[c], B.b[k] * gc);
+  if (t >= 0 && t < G.numTris) atomicAdd(&cnt[t], 1u);
+}
+
+__global__ void __launch_bounds__(256) placeTrianglePixelsKernel(const InterpGeom G,
+                                                                 const int32_t* __restrict__ triImage,
+                                                                 const uint32_t* __restrict__ start,
+                                                                 uint32_t* __restrict__ fill,
+                                                                 uint32_t* __restrict__ order) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (int64_t)G.width * G.height) return;
+  const int32_t t = triImage[p];
+  if (t >= 0 && t < G.numTris) order[start[t] + atomicAdd(&fill[t], 1u)] = (uint32_t)p;
+}
+
+// Exclusive scan, kScanItems values per 1024-lane block; the block totals go
+// to sums[] (scanned by the next level, then added back).
+constexpr int kScanThreads = 1024, kScanPer = 4, kScanItems = kScanThreads * kScanPer;
+
+__global__ void __launch_bounds__(kScanThreads) scanBlocksKernel(uint32_t* __restrict__ d, int64_t n,
+                                                                 uint32_t* __restrict__ sums) {
+  __shared__ uint32_t waveSums[kScanThreads / 64];
+  const int64_t base = (int64_t)blockIdx.x * kScanItems + (int64_t)threadIdx.x * kScanPer;
+  uint32_t v[kScanPer], run = 0;
+#pragma unroll
+  for (int q = 0; q < kScanPer; q++) {
+    v[q] = (base + q < n) ? d[base + q] : 0u;
+    const uint32_t t = v[q];
+    v[q] = run;
+    run += t;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = run;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) waveSums[wave] = x;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const uint32_t w = (threadIdx.x < kScanThreads / 64) ? waveSums[threadIdx.x] : 0u;
+    uint32_t sc = w;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(sc, o, 64);
+      if ((int)threadIdx.x >= o) sc += y;
     }
-    dB[k] = s;
+    if (threadIdx.x < kScanThreads / 64) waveSums[threadIdx.x] = sc - w;
+    if (threadIdx.x == kScanThreads / 64 - 1) sums[blockIdx.x] = sc;
   }
-  const double mean = B.b[0] * dB[0] + B.b[1] * dB[1] + B.b[2] * dB[2];
-  const double rs = 1.0 / B.sum;
-  double dxs[3] = {0.0, 0.0, 0.0}, dys[3] = {0.0, 0.0, 0.0};
+  __syncthreads();
+  const uint32_t off = waveSums[wave] + x - run;
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    const double dc = (dB[k] - mean) * rs;
-    const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
-    dxs[k1] += dc * B.ys[k2];
-    dys[k1] -= dc * B.xs[k2];
-    dxs[k2] -= dc * B.ys[k1];
-    dys[k2] += dc * B.xs[k1];
-  }
+  for (int q = 0; q < kScanPer; q++)
+    if (base + q < n) d[base + q] = v[q] + off;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scanAddKernel(uint32_t* __restrict__ d, int64_t n,
+                                                              const uint32_t* __restrict__ sums) {
+  const int64_t base = (int64_t)blockIdx.x * kScanItems + (int64_t)threadIdx.x * kScanPer;
+  const uint32_t off = sums[blockIdx.x];
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    double* acc = accPos + (int64_t)i[k] * 4;
-    if (dxs[k] != 0.0) unsafeAtomicAdd(&acc[0], dxs[k]);
-    if (dys[k] != 0.0) unsafeAtomicAdd(&acc[1], dys[k]);
-    const double dw = -B.qx * dxs[k] - B.qy * dys[k];
-    if (dw != 0.0) unsafeAtomicAdd(&acc[3], dw);
+  for (int q = 0; q < kScanPer; q++)
+    if (base + q < n) d[base + q] += off;
+}
+
+// Words of scratch scanExclusive needs beyond the data for n values.
+static int64_t scanScratchWords(int64_t n) {
+  int64_t w = 0;
+  for (int64_t m = n; m > 1;) {
+    m = (m + kScanItems - 1) / kScanItems;
+    w += m;
   }
+  return w + 1;
+}
+
+static void scanExclusive(uint32_t* d, int64_t n, uint32_t* scratch, hipStream_t stream) {
+  if (n <= 0) return;
+  const int64_t nb = (n + kScanItems - 1) / kScanItems;
+  hipLaunchKernelGGL(scanBlocksKernel, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, d, n, scratch);
+  if (nb > 1) {
+    scanExclusive(scratch, nb, scratch + nb, stream);
+    hipLaunchKernelGGL(scanAddKernel, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, d, n,
+                       (const uint32_t*)scratch);
+  }
+}
+
+__global__ void __launch_bounds__(256) fillZeroU32Kernel(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    p[k] = 0u;
 }
 
 __global__ void __launch_bounds__(256) fillZeroKernel(double* __restrict__ p, int64_t n) {
@@ -173,17 +315,37 @@ void launchInterpForward(const InterpGeom& G, const uint32_t* ids, const float* 
                      reinterpret_cast<const float4*>(pos), attr, reinterpret_cast<const int3*>(tris), out, triImage);
 }
 
-void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const float* pos, const float* attr,
-                          const int32_t* tris, const float* gradOut, double* accPos, double* accAttr,
-                          float* gradPos, float* gradAttr, hipStream_t stream) {
+int64_t interpBackwardScratchBytes(const InterpGeom& G) {
   const int64_t nPos = G.numVertices * 4, nAttr = G.numVertices * G.numChannels;
+  const int64_t pixels = (int64_t)G.width * G.height, nt = (int64_t)G.numTris + 1;
+  return (nPos + nAttr) * 8 + (2 * nt + pixels + scanScratchWords(nt)) * 4;
+}
+
+void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const float* pos, const float* attr,
+                          const int32_t* tris, const float* gradOut, void* scratch, float* gradPos,
+                          float* gradAttr, hipStream_t stream) {
+  const int64_t nPos = G.numVertices * 4, nAttr = G.numVertices * G.numChannels;
+  const int64_t n = (int64_t)G.width * G.height, nt = (int64_t)G.numTris + 1;
+  double* accPos = static_cast<double*>(scratch);
+  double* accAttr = accPos + nPos;
+  uint32_t* start = reinterpret_cast<uint32_t*>(accAttr + nAttr);   // counts, then exclusive offsets [nt]
+  uint32_t* fill = start + nt;                                        // per-triangle placement cursor [nt]
+  uint32_t* order = fill + nt;                                        // pixel indices by triangle [n]
+  uint32_t* scanTmp = order + n;
   if (nPos > 0) hipLaunchKernelGGL(fillZeroKernel, dim3(gridFor(nPos)), dim3(256), 0, stream, accPos, nPos);
   if (nAttr > 0) hipLaunchKernelGGL(fillZeroKernel, dim3(gridFor(nAttr)), dim3(256), 0, stream, accAttr, nAttr);
-  const int64_t n = (int64_t)G.width * G.height;
-  if (n > 0)
-    hipLaunchKernelGGL(interpBackwardKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, G, triImage,
-                       reinterpret_cast<const float4*>(pos), attr, reinterpret_cast<const int3*>(tris), gradOut,
-                       accPos, accAttr);
+  hipLaunchKernelGGL(fillZeroU32Kernel, dim3(gridFor(2 * nt)), dim3(256), 0, stream, start, 2 * nt);
+  if (n > 0) {
+    hipLaunchKernelGGL(countTrianglePixelsKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, G,
+                       triImage, start);
+    scanExclusive(start, nt, scanTmp, stream);      // start[numTris] = number of visible pixels
+    hipLaunchKernelGGL(placeTrianglePixelsKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, G,
+                       triImage, (const uint32_t*)start, fill, order);
+    hipLaunchKernelGGL(interpBackwardKernel, dim3((unsigned)((n + kBwdThreads - 1) / kBwdThreads)), dim3(kBwdThreads),
+                       0, stream, G, triImage,
+                       (const uint32_t*)order, (const uint32_t*)start, reinterpret_cast<const float4*>(pos), attr,
+                       reinterpret_cast<const int3*>(tris), gradOut, accPos, accAttr);
+  }
   if (nPos > 0) hipLaunchKernelGGL(roundToFloatKernel, dim3(gridFor(nPos)), dim3(256), 0, stream, accPos, gradPos, nPos);
   if (nAttr > 0)
     hipLaunchKernelGGL(roundToFloatKernel, dim3(gridFor(nAttr)), dim3(256), 0, stream, accAttr, gradAttr, nAttr);
