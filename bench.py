@@ -87,6 +87,28 @@ def measured_traffic(kernel, workload_key, triid):
     return None
 
 
+def measured_issue(kernel, workload_key, triid):
+    """VALU instructions per launch of `kernel` (rocprofv3 SQ_INSTS_VALU) and
+    the VALU issue fraction they imply (x 4 cycles per wave64 instruction over
+    1024 SIMDs at 2.4 GHz and the kernel's traced duration), as recorded in a
+    profile summary under profiles/ (a line `VALU <kernel> <workload>
+    triid=<0|1> insts=<n> us=<t> issue_frac=<f>`), or None."""
+    pdir = os.path.join(REPO, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    want = f"VALU {kernel} {workload_key} triid={int(triid)} "
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith((".txt", ".md")):
+            continue
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                if line.startswith(want):
+                    kv = dict(t.split("=", 1) for t in line.split()[4:])
+                    return {"insts_per_launch": int(kv["insts"]), "frac": float(kv["issue_frac"]),
+                            "profiled_us": float(kv["us"]), "source": f"profiles/{name}"}
+    return None
+
+
 def copy_calibration():
     """A plain device-to-device copy of known size (256 MiB, three times):
     the reference point for scaling FETCH_SIZE / WRITE_SIZE readings."""
@@ -329,7 +351,10 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
                      "traffic": traffic, "algorithmic_bytes_per_launch": alg_bytes,
-                     "kernel_ms": kernel_s * 1e3, "visible_subtris": visible},
+                     "kernel_ms": kernel_s * 1e3, "visible_subtris": visible,
+                     # The fine stage is integer work limited by VALU issue, not by HBM: the
+                     # fraction of the VALU issue capacity its instruction count fills.
+                     "valu_issue": measured_issue(kname, args.workload, triid)},
         "kernels": kernels,
         "frame_algorithmic_GBps": frame_algorithmic_bytes(scene, triid) / (ms * 1e-3) / 1e9,
         "counters": counters,
