@@ -101,6 +101,55 @@ def test_extreme_colours(oracle_lib, blend, lerp):
     assert np.array_equal(got["color"], ref["color"])
 
 
+def pathological_scene(seed=77):
+    """Random triangles with hostile clip positions: NaN, +-inf and huge
+    components, w = 0, vertices behind the eye, vertices on the far plane,
+    and degenerate triangles (a repeated vertex, three collinear vertices)."""
+    from cudaraster import scenes
+    s = scenes.random_tris(4000, 160, 120, samples_log2=2, seed=seed, area_range=(1.0, 4000.0))
+    v = s.vertices.copy()
+    rng = np.random.default_rng(seed)
+    pick = rng.random(v.shape[0])
+    specials = np.array([np.nan, np.inf, -np.inf, 1e30, -1e30, 0.0], np.float32)
+    idx = np.where(pick < 0.03)[0]
+    v[idx, rng.integers(0, 4, len(idx))] = rng.choice(specials, len(idx))
+    idx = np.where((pick >= 0.03) & (pick < 0.05))[0]
+    v[idx, 3] = 0.0
+    idx = np.where((pick >= 0.05) & (pick < 0.07))[0]
+    v[idx, 3] *= -1
+    idx = np.where((pick >= 0.07) & (pick < 0.09))[0]
+    v[idx, 2] = v[idx, 3]
+    tri = v.reshape(-1, 3, 8)
+    k = rng.choice(len(tri), 200, replace=False)
+    tri[k, 2, :4] = tri[k, 1, :4]
+    k = rng.choice(len(tri), 200, replace=False)
+    tri[k, 2, :4] = 0.5 * (tri[k, 0, :4] + tri[k, 1, :4])
+    s.vertices[:] = tri.reshape(-1, 8)
+    return s
+
+
+def test_pathological_vertices(oracle_lib):
+    _torch()
+    s = pathological_scene()
+    ref = oracle_lib.render(s)
+    assert ref["stats"]["clipped"] > 100 and ref["stats"]["numSubtris"] > 1000
+    got = render_gpu(s, debug={"count_fragments": True})
+    assert_parity(got, ref, "pathological vertices")
+    assert got["counters"]["fragments"] == ref["stats"]["fragments"]
+
+
+def test_zero_triangles(oracle_lib):
+    """A draw without triangles still applies the deferred clear."""
+    _torch()
+    import dataclasses
+    from cudaraster import scenes
+    s = scenes.random_tris(10, 72, 40, samples_log2=1, seed=2)
+    s = dataclasses.replace(s, indices=s.indices[:0])
+    got = render_gpu(s)
+    assert_parity(got, oracle_lib.render(s), "zero triangles")
+    assert got["counters"]["num_tris"] == 0 and got["counters"]["tile_entries"] == 0
+
+
 @pytest.mark.parametrize("flags", [0, 1, 2, 3])
 def test_render_mode_flags(oracle_lib, flags):
     _torch()
