@@ -241,6 +241,9 @@ cr_status drawImpl(cr_context* ctx, hipStream_t stream, bool sync) {
     st = launchAll(ctx, P, stream);
     if (st != CR_OK) return st;
     if (!sync) {
+      // Waiting on this event (not on the stream) lets cr_draw_check return
+      // as soon as the draw is done, whatever was queued after it.
+      CR_HIP(ctx, hipEventRecord(ctx->evPending, stream));
       ctx->pending = true;
       ctx->pendingStream = stream;
       ctx->deferredClear = false;
@@ -277,6 +280,7 @@ cr_status cr_context_create(cr_context** out) {
   if (e == hipSuccess) e = ctx->counters.ensure(sizeof(Counters));
   if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hostCounters, sizeof(Counters), hipHostMallocDefault);
   for (int i = 0; i < 5 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->evPending, hipEventDisableTiming);
   if (e != hipSuccess) {
     cr_context_destroy(ctx);
     return fail(nullptr, CR_ERR_DEVICE, "cr_context_create: %s", hipGetErrorString(e));
@@ -297,6 +301,7 @@ cr_status cr_context_destroy(cr_context* ctx) {
   if (ctx->hostCounters) (void)hipHostFree(ctx->hostCounters);
   for (int i = 0; i < 5; i++)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+  if (ctx->evPending) (void)hipEventDestroy(ctx->evPending);
   delete ctx;
   return CR_OK;
 }
@@ -486,7 +491,7 @@ cr_status cr_draw_triangles_async(cr_context* ctx, void* stream) {
 cr_status cr_draw_check(cr_context* ctx) {
   if (!ctx) return fail(nullptr, CR_ERR_INVALID, "cr_draw_check: context is NULL");
   if (!ctx->pending) return CR_OK;
-  CR_HIP(ctx, hipStreamSynchronize(ctx->pendingStream));
+  CR_HIP(ctx, hipEventSynchronize(ctx->evPending));
   ctx->pending = false;
   bool overflow = false;
   cr_status st = finishDraw(ctx, &overflow);

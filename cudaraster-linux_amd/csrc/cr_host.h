@@ -133,6 +133,7 @@ struct cr_context {
   hipEvent_t ev[5] = {};
   bool evRecorded = false;
   hipStream_t pendingStream = nullptr;
+  hipEvent_t evPending = nullptr;     // after an asynchronous draw's counter readback
   bool pending = false;
   cr_counters last{};
 };

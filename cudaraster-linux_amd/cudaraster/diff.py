@@ -13,8 +13,8 @@ from __future__ import annotations
 
 import ctypes
 
-from . import (CrError, CudaRaster, CudaSurface, FORMAT_DEPTH32, FORMAT_RGBA8, FORMAT_TRIANGLE_ID, _check,
-               load, pipe_desc, SHADER_CONSTANT, BLEND_REPLACE, RENDER_DEPTH)
+from . import (CR_ERR_OVERFLOW, CR_OK, CrError, CudaRaster, CudaSurface, FORMAT_DEPTH32, FORMAT_RGBA8,
+               FORMAT_TRIANGLE_ID, _check, load, pipe_desc, SHADER_CONSTANT, BLEND_REPLACE, RENDER_DEPTH)
 
 
 def _stream():
@@ -61,6 +61,7 @@ class DiffRasterizer:
 
     def __init__(self, width, height):
         self.width, self.height = int(width), int(height)
+        self._sized = False       # capacities known from a synchronous draw
         self.cr = CudaRaster().init()
         self.color = CudaSurface((width, height), FORMAT_RGBA8, 1)
         self.depth = CudaSurface((width, height), FORMAT_DEPTH32, 1)
@@ -78,11 +79,22 @@ class DiffRasterizer:
             raise CrError(1, "pos must be a contiguous float32 [V, 4] device tensor")
         if tris.dtype != torch.int32 or tris.dim() != 2 or tris.shape[1] != 3 or not tris.is_contiguous():
             raise CrError(1, "tris must be a contiguous int32 [T, 3] device tensor")
-        torch.cuda.current_stream().synchronize()
+        stream = torch.cuda.current_stream()
+        self.sync()                       # the previous asynchronous draw fitted its capacities
         self.cr.deferredClear((0.0, 0.0, 0.0, 0.0), 1.0)
         self.cr.setVertexBuffer(pos.detach(), 0, pos.shape[0])
         self.cr.setIndexBuffer(tris, 0, tris.shape[0])
-        self.cr.drawTriangles()
+        if self._sized:
+            # Queued behind the previous view's work on torch's stream; the
+            # host does not wait for it (checked at the next call or sync()).
+            self.cr.drawTrianglesAsync(stream)
+        else:
+            # First draw: synchronous, sizes the queues (grown and re-run on
+            # overflow), then twice its demand is reserved for later views.
+            self.cr.drawTriangles(stream)
+            c = self.cr.getCounters()
+            self.cr.reserve(2 * c["num_subtris"] + 4096, 2 * c["bin_entries"] + 4096, 2 * c["tile_entries"] + 4096)
+            self._sized = True
         H, W = self.height, self.width
         tri_image = torch.empty((H, W), dtype=torch.int32, device=pos.device)
         dummy = torch.empty(1, dtype=torch.float32, device=pos.device)   # no channels: ids only
@@ -114,7 +126,19 @@ class DiffRasterizer:
                                               _stream()), self.cr.ctx)
         return gp, ga
 
+    def sync(self):
+        """Waits for the last asynchronous visibility draw; raises CrError if it
+        exceeded the reserved queue capacities (its triangle ids are then
+        incomplete; the capacities have been grown, so re-running the view
+        succeeds)."""
+        st = self.cr.drawCheck()
+        if st == CR_ERR_OVERFLOW:
+            raise CrError(st, "visibility draw exceeded its queue capacities; re-run the view")
+        if st != CR_OK:
+            _check(st, self.cr.ctx)
+
     def close(self):
+        self.sync()
         self.cr.close()
         for s in (self.color, self.depth, self.ids):
             s.close()
