@@ -275,6 +275,11 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
       ax[0] = (int32_t)(int16_t)(h.x & 0xFFFFu) - Tx; ay[0] = (int32_t)(int16_t)(h.x >> 16) - Ty;
       ax[1] = (int32_t)(int16_t)(h.y & 0xFFFFu) - Tx; ay[1] = (int32_t)(int16_t)(h.y >> 16) - Ty;
       ax[2] = (int32_t)(int16_t)(h.z & 0xFFFFu) - Tx; ay[2] = (int32_t)(int16_t)(h.z >> 16) - Ty;
+      // Vertices within 2^14 subpixels of the tile origin (all but large
+      // triangles): a x d fits 31 bits, so it is two 24-bit products and
+      // needs no clamp.  Uniform over the wave's set-up lanes.
+      const uint32_t span = (uint32_t)(abs(ax[0]) | abs(ay[0]) | abs(ax[1]) | abs(ay[1]) | abs(ax[2]) | abs(ay[2]));
+      const bool near = __ballot(span >= (1u << 14)) == 0;
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         const int kn = (k + 1) % 3;
@@ -283,10 +288,17 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
         // edges (Util.inl:100).  |a|,|d| <= 2^16.  Clamping to +-2^30
         // keeps the sign of a x d - s x d exact for every in-tile sample
         // (|s x d| < 2^25), and every threshold below fits in 32 bits.
-        int64_t cc = (int64_t)ax[k] * dy - (int64_t)ay[k] * dx;
-        if (dy > 0 || (dy == 0 && dx <= 0)) cc -= 1;
-        cc = cc > (1ll << 30) ? (1ll << 30) : (cc < -(1ll << 30) ? -(1ll << 30) : cc);
-        const int32_t ck = (int32_t)cc;
+        // With |a| < 2^14 (so |d| < 2^15), |a x d| < 2^30: the clamp never
+        // applies and 32-bit arithmetic gives the same value.
+        const int32_t excl = (dy > 0 || (dy == 0 && dx <= 0)) ? 1 : 0;
+        int32_t ck;
+        if (near) {
+          ck = __mul24(ax[k], dy) - __mul24(ay[k], dx) - excl;
+        } else {
+          int64_t cc = (int64_t)ax[k] * dy - (int64_t)ay[k] * dx - excl;
+          cc = cc > (1ll << 30) ? (1ll << 30) : (cc < -(1ll << 30) ? -(1ll << 30) : cc);
+          ck = (int32_t)cc;
+        }
 #pragma unroll
         for (int i = 0; i < N; i++) {
           const int32_t ox = (2 * msaaPattern(S, i) + 1) << (3 - S);
