@@ -294,15 +294,15 @@ __global__ void __launch_bounds__(256) fillZeroU32Kernel(uint32_t* __restrict__ 
     p[k] = 0u;
 }
 
-__global__ void __launch_bounds__(256) fillZeroKernel(double* __restrict__ p, int64_t n) {
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-    p[k] = 0.0;
-}
-
-__global__ void __launch_bounds__(256) roundToFloatKernel(const double* __restrict__ src, float* __restrict__ dst,
-                                                          int64_t n) {
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-    dst[k] = (float)src[k];
+// Both gradient arrays in one launch: [0, na) from a to da, then [na, na + nb)
+// from b to db.
+__global__ void __launch_bounds__(256) roundToFloatKernel(const double* __restrict__ a, float* __restrict__ da,
+                                                          int64_t na, const double* __restrict__ b,
+                                                          float* __restrict__ db, int64_t nb) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < na + nb; k += (int64_t)gridDim.x * blockDim.x) {
+    if (k < na) da[k] = (float)a[k];
+    else db[k - na] = (float)b[k - na];
+  }
 }
 
 static int gridFor(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 1 << 20); }
@@ -332,9 +332,10 @@ void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const fl
   uint32_t* fill = start + nt;                                        // per-triangle placement cursor [nt]
   uint32_t* order = fill + nt;                                        // pixel indices by triangle [n]
   uint32_t* scanTmp = order + n;
-  if (nPos > 0) hipLaunchKernelGGL(fillZeroKernel, dim3(gridFor(nPos)), dim3(256), 0, stream, accPos, nPos);
-  if (nAttr > 0) hipLaunchKernelGGL(fillZeroKernel, dim3(gridFor(nAttr)), dim3(256), 0, stream, accAttr, nAttr);
-  hipLaunchKernelGGL(fillZeroU32Kernel, dim3(gridFor(2 * nt)), dim3(256), 0, stream, start, 2 * nt);
+  // Accumulators, counts and cursors are contiguous: one clearing launch.
+  const int64_t clearWords = (nPos + nAttr) * 2 + 2 * nt;
+  hipLaunchKernelGGL(fillZeroU32Kernel, dim3(gridFor(clearWords)), dim3(256), 0, stream,
+                     reinterpret_cast<uint32_t*>(accPos), clearWords);
   if (n > 0) {
     hipLaunchKernelGGL(countTrianglePixelsKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, G,
                        triImage, start);
@@ -346,9 +347,9 @@ void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const fl
                        (const uint32_t*)order, (const uint32_t*)start, reinterpret_cast<const float4*>(pos), attr,
                        reinterpret_cast<const int3*>(tris), gradOut, accPos, accAttr);
   }
-  if (nPos > 0) hipLaunchKernelGGL(roundToFloatKernel, dim3(gridFor(nPos)), dim3(256), 0, stream, accPos, gradPos, nPos);
-  if (nAttr > 0)
-    hipLaunchKernelGGL(roundToFloatKernel, dim3(gridFor(nAttr)), dim3(256), 0, stream, accAttr, gradAttr, nAttr);
+  if (nPos + nAttr > 0)
+    hipLaunchKernelGGL(roundToFloatKernel, dim3(gridFor(nPos + nAttr)), dim3(256), 0, stream, (const double*)accPos,
+                       gradPos, nPos, (const double*)accAttr, gradAttr, nAttr);
 }
 
 }  // namespace crhip

@@ -71,9 +71,8 @@ class DiffRasterizer:
         # Positions only (16-byte vertices), depth test on, constant colour.
         self.cr.setPixelPipe(pipe_desc(16, SHADER_CONSTANT, BLEND_REPLACE, 0, RENDER_DEPTH))
 
-    def rasterize(self, pos, tris):
-        """Exact visibility: returns the triangle id image [H, W] (int32, -1
-        where empty) by drawing pos/tris and reading the id surface."""
+    def _draw(self, pos, tris):
+        """Exact visibility of pos/tris into the triangle-id surface."""
         import torch
         if pos.dtype != torch.float32 or pos.dim() != 2 or pos.shape[1] != 4 or not pos.is_contiguous():
             raise CrError(1, "pos must be a contiguous float32 [V, 4] device tensor")
@@ -95,15 +94,24 @@ class DiffRasterizer:
             c = self.cr.getCounters()
             self.cr.reserve(2 * c["num_subtris"] + 4096, 2 * c["bin_entries"] + 4096, 2 * c["tile_entries"] + 4096)
             self._sized = True
-        H, W = self.height, self.width
-        tri_image = torch.empty((H, W), dtype=torch.int32, device=pos.device)
+
+    def rasterize(self, pos, tris):
+        """Exact visibility: returns the triangle id image [H, W] (int32, -1
+        where empty) by drawing pos/tris and reading the id surface."""
+        import torch
+        self._draw(pos, tris)
+        tri_image = torch.empty((self.height, self.width), dtype=torch.int32, device=pos.device)
         dummy = torch.empty(1, dtype=torch.float32, device=pos.device)   # no channels: ids only
         self._interpolate(pos.detach(), None, tris, dummy, tri_image, channels=0)
         return tri_image
 
     def render(self, pos, attr, tris, tri_image=None):
+        """Interpolated attributes [H, W, C].  Without tri_image, draws pos/tris
+        first; the interpolation then also writes the triangle ids."""
+        import torch
         if tri_image is None:
-            tri_image = self.rasterize(pos, tris)
+            self._draw(pos, tris)
+            tri_image = torch.empty((self.height, self.width), dtype=torch.int32, device=pos.device)
         return Interpolate.function().apply(pos, attr, tris, tri_image, self)
 
     # -- raw calls ------------------------------------------------------------
