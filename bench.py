@@ -205,26 +205,25 @@ def multi_view_ratio(reps=5, triid=True):
 
 def differentiable_views(reps=3):
     """Differentiable workload: 32 views at 512x512, 200k triangles each, 8
-    FP32 channels; per view one exact visibility draw, the interpolation
-    forward, an L2 loss against the view's target and the backward pass to
-    positions and attributes (torch autograd over the HIP kernels)."""
+    FP32 channels; per view one exact visibility draw and the interpolation
+    forward, an L2 loss against the views' targets and the backward pass to
+    every view's positions and attributes (one torch autograd call over the
+    HIP kernels of all views: DiffRasterizer.render_views)."""
     import torch
     from cudaraster import scenes
     from cudaraster.diff import DiffRasterizer
     views = scenes.diff_views(32)
     dr = DiffRasterizer(512, 512)
-    dev = []
-    for v in views:
-        dev.append((torch.from_numpy(v.pos).cuda().requires_grad_(True),
-                    torch.from_numpy(v.attr).cuda().requires_grad_(True),
-                    torch.from_numpy(v.tris).cuda(), torch.from_numpy(v.target).cuda()))
+    pos = [torch.from_numpy(v.pos).cuda().requires_grad_(True) for v in views]
+    attr = [torch.from_numpy(v.attr).cuda().requires_grad_(True) for v in views]
+    tris = [torch.from_numpy(v.tris).cuda() for v in views]
+    target = torch.stack([torch.from_numpy(v.target).cuda() for v in views])
 
     def one_pass():
-        for pos, attr, tris, target in dev:
-            pos.grad = None
-            attr.grad = None
-            out = dr.render(pos, attr, tris)
-            ((out - target) ** 2).sum().backward()
+        for t in pos + attr:
+            t.grad = None
+        out, _ = dr.render_views(pos, attr, tris)
+        ((out - target) ** 2).sum().backward()
 
     one_pass()
     torch.cuda.synchronize()
@@ -236,7 +235,8 @@ def differentiable_views(reps=3):
     dr.close()
     return {"views": len(views), "triangles_per_view": int(views[0].tris.shape[0]), "resolution": 512,
             "channels": int(views[0].attr.shape[1]), "ms_per_pass": t * 1e3, "views_per_s": len(views) / t,
-            "steps": "visibility draw + interpolate forward + L2 loss + backward (positions, attributes)"}
+            "steps": "per view: visibility draw + interpolate forward; L2 loss; backward to positions and attributes",
+            "api": "DiffRasterizer.render_views (one autograd call for the 32 views)"}
 
 
 def cpu_baseline(scene, runs=5):

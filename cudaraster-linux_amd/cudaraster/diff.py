@@ -54,6 +54,50 @@ class Interpolate:
         return cls._fn
 
 
+class InterpolateViews:
+    """Autograd function over several views of one resolution: one forward
+    and one backward call for all of them (the per-view host work is a few
+    library calls, not a pass through the autograd engine)."""
+    _fn = None
+
+    @classmethod
+    def function(cls):
+        if cls._fn is not None:
+            return cls._fn
+        import torch
+
+        class _InterpolateViews(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, raster, n, *tensors):
+                pos, attr, tris = tensors[:n], tensors[n:2 * n], tensors[2 * n:3 * n]
+                H, W, C = raster.height, raster.width, attr[0].shape[1]
+                out = torch.empty((n, H, W, C), dtype=torch.float32, device=pos[0].device)
+                ids = torch.empty((n, H, W), dtype=torch.int32, device=pos[0].device)
+                for i in range(n):
+                    raster._draw(pos[i], tris[i])
+                    raster._interpolate(pos[i].detach(), attr[i].detach(), tris[i], out[i], ids[i])
+                ctx.save_for_backward(ids, *tensors)
+                ctx.raster, ctx.n = raster, n
+                ctx.mark_non_differentiable(ids)
+                return out, ids
+
+            @staticmethod
+            def backward(ctx, grad_out, _grad_ids):
+                n = ctx.n
+                ids, *tensors = ctx.saved_tensors
+                pos, attr, tris = tensors[:n], tensors[n:2 * n], tensors[2 * n:3 * n]
+                grad_out = grad_out.contiguous()
+                gp, ga = [], []
+                for i in range(n):
+                    a, b = ctx.raster._interpolate_backward(pos[i], attr[i], tris[i], ids[i], grad_out[i])
+                    gp.append(a)
+                    ga.append(b)
+                return (None, None, *gp, *ga, *([None] * n))
+
+        cls._fn = _InterpolateViews
+        return cls._fn
+
+
 class DiffRasterizer:
     """Single-sample rasterizer + differentiable interpolation for one
     resolution.  render(pos, attr, tris) -> image [H, W, C] (row 0 at the
@@ -113,6 +157,14 @@ class DiffRasterizer:
             self._draw(pos, tris)
             tri_image = torch.empty((self.height, self.width), dtype=torch.int32, device=pos.device)
         return Interpolate.function().apply(pos, attr, tris, tri_image, self)
+
+    def render_views(self, pos, attr, tris):
+        """Several views at once: lists of pos [V_i, 4], attr [V_i, C] and
+        tris [T_i, 3] (same C for all) -> (images [n, H, W, C], triangle ids
+        [n, H, W]); gradients flow to every pos and attr."""
+        if not (len(pos) == len(attr) == len(tris)) or len(pos) == 0:
+            raise CrError(1, "render_views needs equally long, non-empty lists")
+        return InterpolateViews.function().apply(self, len(pos), *pos, *attr, *tris)
 
     # -- raw calls ------------------------------------------------------------
     def _interpolate(self, pos, attr, tris, out, tri_image, channels=None):
