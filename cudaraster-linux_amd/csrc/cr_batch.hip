@@ -230,7 +230,7 @@ cr_status buildTables(cr_batch* b, Tables& T, const std::vector<int>& idx) {
                   std::max(P.numTiles, 1);
     chunks[k] = P.numTris > 0 ? P.numChunks : 0;
     bins[k] = P.numBins;
-    groups[k] = (P.numTiles + W - 1) / W;
+    groups[k] = ((P.numTiles + W - 1) / W + kFineRounds - 1) / kFineRounds;
   }
   std::vector<int> all(n);
   for (int k = 0; k < n; k++) all[k] = k;

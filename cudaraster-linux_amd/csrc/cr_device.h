@@ -233,6 +233,10 @@ constexpr int kCoarseShapes[2] = {256, 1024};    // coarse workgroup sizes (lane
 int coarseShapeFor(const Params& P);             // the plain draw's choice
 // Fine-stage variant of a draw: (samplesLog2 * 2 + deferred shading) * 2 + depth test.
 constexpr int kFineVariants = 16;
+// Tile groups per workgroup of a batched fine pass: the workgroup walks them
+// in turn, so its view lookup and parameter loads serve several groups and
+// fewer workgroups are dispatched.
+constexpr int kFineRounds = 4;
 int fineVariant(const Params& P);
 void launchFineBatch(int variant, int tilesPerGroup, int banded, const Params* views, const int32_t* groupStart,
                      const int32_t* order, int numViews, int totalGroups, hipStream_t stream);

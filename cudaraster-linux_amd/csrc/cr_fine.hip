@@ -509,7 +509,8 @@ __global__ void __launch_bounds__(64 * W) fineBatchKernel(const Params* __restri
   if (g >= totalGroups) return;
   const int k = findView(groupStart, numViews, g);
   const Params& P = views[order[k]];
-  fineDispatch<S, VIS, false, W, DEPTH>(P, g - groupStart[k], recAll);
+  const int first = (g - groupStart[k]) * kFineRounds;
+  for (int t = 0; t < kFineRounds; t++) fineDispatch<S, VIS, false, W, DEPTH>(P, first + t, recAll);
 }
 
 // Variant v = (samplesLog2 * 2 + deferred shading) * 2 + depth test.
