@@ -164,6 +164,10 @@ class DiffRasterizer:
         [n, H, W]); gradients flow to every pos and attr."""
         if not (len(pos) == len(attr) == len(tris)) or len(pos) == 0:
             raise CrError(1, "render_views needs equally long, non-empty lists")
+        C = attr[0].shape[1] if attr[0].dim() == 2 else -1
+        for a in attr:
+            if a.dim() != 2 or a.shape[1] != C or a.dtype.itemsize != 4 or not a.is_contiguous():
+                raise CrError(1, "render_views: every attr must be a contiguous float32 [V_i, C] tensor, same C")
         return InterpolateViews.function().apply(self, len(pos), *pos, *attr, *tris)
 
     # -- raw calls ------------------------------------------------------------
@@ -171,12 +175,18 @@ class DiffRasterizer:
         C = attr.shape[1] if channels is None else channels
         if attr is not None and (attr.dtype.itemsize != 4 or not attr.is_contiguous()):
             raise CrError(1, "attr must be a contiguous float32 [V, C] device tensor")
+        hw = self.height * self.width
+        if tri_image.numel() != hw or not tri_image.is_contiguous() or (C > 0 and (out.numel() != hw * C or
+                                                                                 not out.is_contiguous())):
+            raise CrError(1, "interpolation output buffers do not match the rasterizer's resolution")
         _check(load().cr_interpolate(self.cr.ctx, self.ids.handle, pos.data_ptr(), pos.shape[0],
                                      attr.data_ptr() if attr is not None else None, C, tris.data_ptr(),
                                      tris.shape[0], out.data_ptr(), tri_image.data_ptr(), _stream()), self.cr.ctx)
 
     def _interpolate_backward(self, pos, attr, tris, tri_image, grad_out):
         import torch
+        if grad_out.numel() != tri_image.numel() * attr.shape[1] or not grad_out.is_contiguous():
+            raise CrError(1, "gradient of the image does not match its shape")
         gp = torch.empty_like(pos)
         ga = torch.empty_like(attr)
         H, W = tri_image.shape
@@ -198,7 +208,9 @@ class DiffRasterizer:
             _check(st, self.cr.ctx)
 
     def close(self):
-        self.sync()
-        self.cr.close()
-        for s in (self.color, self.depth, self.ids):
-            s.close()
+        try:
+            self.sync()
+        finally:
+            self.cr.close()
+            for s in (self.color, self.depth, self.ids):
+                s.close()
