@@ -199,10 +199,12 @@ class DiffRasterizer:
     def sync(self):
         """Waits for the last asynchronous visibility draw; raises CrError if it
         exceeded the reserved queue capacities (its triangle ids are then
-        incomplete; the capacities have been grown, so re-running the view
-        succeeds)."""
+        incomplete).  The next draw is then synchronous again -- it grows the
+        queues and re-runs until they fit, and reserves for later views -- so
+        re-running the view succeeds."""
         st = self.cr.drawCheck()
         if st == CR_ERR_OVERFLOW:
+            self._sized = False
             raise CrError(st, "visibility draw exceeded its queue capacities; re-run the view")
         if st != CR_OK:
             _check(st, self.cr.ctx)

@@ -151,3 +151,33 @@ def test_render_views_matches_single_views():
                 assert np.abs(g - r).max() <= 1e-6 * np.abs(r).max() + 1e-30, f"view {k} {name} gradient"
     finally:
         dr.close()
+
+
+def test_async_visibility_overflow_is_reported_then_recovers():
+    """After a small sizing view, a much larger view overflows the reserved
+    queues in its asynchronous draw: sync() raises instead of returning
+    incomplete ids, and re-rendering the view (queues now grown) matches a
+    fresh rasterizer."""
+    torch = _torch()
+    from cudaraster import CrError, scenes
+    from cudaraster.diff import DiffRasterizer
+    small = scenes.diff_view(seed=61, num_tris=200, width=128, height=96)
+    big = scenes.diff_view(seed=62, num_tris=60000, width=128, height=96, area_range=(4.0, 2000.0))
+    expect = run_view(big)
+    dr = DiffRasterizer(128, 96)
+    try:
+        def render(v):
+            pos = torch.from_numpy(v.pos).cuda().requires_grad_(True)
+            attr = torch.from_numpy(v.attr).cuda().requires_grad_(True)
+            tris = torch.from_numpy(v.tris).cuda()
+            return dr.render(pos, attr, tris)
+        render(small)                      # synchronous sizing draw
+        render(big)                        # asynchronous, exceeds the reserve
+        with pytest.raises(CrError, match="capacities"):
+            dr.sync()
+        out = render(big)
+        dr.sync()
+        torch.cuda.synchronize()
+        assert np.array_equal(out.detach().cpu().numpy(), expect["out"])
+    finally:
+        dr.close()
