@@ -272,6 +272,8 @@ def main():
                     help="skip the batched/plain multi-view ratio (64 views)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB copy kernel (reference for HBM counter scaling)")
+    ap.add_argument("--only", choices=("multi-view", "diff"),
+                    help="run only that leg and print its JSON record (for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -281,6 +283,12 @@ def main():
     import torch
     if args.calibrate_copy:
         copy_calibration()
+        return
+    if args.only == "multi-view":
+        print(json.dumps({"multi_view": multi_view_ratio(triid=not args.no_triid)}), flush=True)
+        return
+    if args.only == "diff":
+        print(json.dumps({"differentiable": differentiable_views()}), flush=True)
         return
     import cudaraster
     assert torch.cuda.is_available(), "bench.py needs a GPU"

@@ -289,9 +289,13 @@ static void scanExclusive(uint32_t* d, int64_t n, uint32_t* scratch, hipStream_t
   }
 }
 
+// Zeroes n words at p (16-byte aligned): 16-byte stores, then the tail.
 __global__ void __launch_bounds__(256) fillZeroU32Kernel(uint32_t* __restrict__ p, int64_t n) {
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-    p[k] = 0u;
+  const int64_t n4 = n >> 2;
+  uint4* __restrict__ q = reinterpret_cast<uint4*>(p);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n4; k += (int64_t)gridDim.x * blockDim.x)
+    q[k] = make_uint4(0u, 0u, 0u, 0u);
+  if (blockIdx.x == 0 && threadIdx.x < (unsigned)(n & 3)) p[(n4 << 2) + threadIdx.x] = 0u;
 }
 
 // Both gradient arrays in one launch: [0, na) from a to da, then [na, na + nb)
@@ -334,7 +338,7 @@ void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const fl
   uint32_t* scanTmp = order + n;
   // Accumulators, counts and cursors are contiguous: one clearing launch.
   const int64_t clearWords = (nPos + nAttr) * 2 + 2 * nt;
-  hipLaunchKernelGGL(fillZeroU32Kernel, dim3(gridFor(clearWords)), dim3(256), 0, stream,
+  hipLaunchKernelGGL(fillZeroU32Kernel, dim3(gridFor((clearWords + 3) / 4)), dim3(256), 0, stream,
                      reinterpret_cast<uint32_t*>(accPos), clearWords);
   if (n > 0) {
     hipLaunchKernelGGL(countTrianglePixelsKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, G,
