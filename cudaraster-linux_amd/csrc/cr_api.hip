@@ -217,7 +217,9 @@ cr_status finishDraw(cr_context* ctx, bool* overflow) {
   if (c.overflow & 1u) ctx->maxSubtris = std::max<int64_t>(ctx->maxSubtris, (int64_t)c.numSubtris + kSubtrisSlack);
   if (c.overflow & 2u) ctx->maxBinEntries = std::max<int64_t>(ctx->maxBinEntries, (int64_t)c.binEntries + kBinSlack);
   if (c.overflow & 4u) ctx->maxTileEntries = std::max<int64_t>(ctx->maxTileEntries, (int64_t)c.tileEntries + kTileSlack);
-  if (ctx->maxSubtris > kMaxEntries || ctx->maxBinEntries > kMaxEntries || ctx->maxTileEntries > kMaxEntries)
+  if (ctx->maxSubtris > kMaxSubtris)
+    return fail(ctx, CR_ERR_OVERFLOW, "CudaRaster: more than 2^28 subtriangles in one draw");
+  if (ctx->maxBinEntries > kMaxEntries || ctx->maxTileEntries > kMaxEntries)
     return fail(ctx, CR_ERR_OVERFLOW, "CudaRaster: internal queue capacity exceeded (2^31 entries)");
   return CR_OK;
 }
@@ -502,8 +504,8 @@ cr_status cr_draw_check(cr_context* ctx) {
 
 cr_status cr_reserve(cr_context* ctx, int64_t maxSubtris, int64_t maxBin, int64_t maxTile) {
   if (!ctx) return fail(nullptr, CR_ERR_INVALID, "cr_reserve: context is NULL");
-  if (maxSubtris > kMaxEntries || maxBin > kMaxEntries || maxTile > kMaxEntries)
-    return fail(ctx, CR_ERR_INVALID, "cr_reserve: capacity above 2^31 entries");
+  if (maxSubtris > kMaxSubtris || maxBin > kMaxEntries || maxTile > kMaxEntries)
+    return fail(ctx, CR_ERR_INVALID, "cr_reserve: capacity above 2^28 subtriangles or 2^31 queue entries");
   ctx->maxSubtris = std::max(ctx->maxSubtris, maxSubtris);
   ctx->maxBinEntries = std::max(ctx->maxBinEntries, maxBin);
   ctx->maxTileEntries = std::max(ctx->maxTileEntries, maxTile);

@@ -365,7 +365,9 @@ cr_status settleOverflow(cr_batch* b, hipStream_t stream) {
       if (c.overflow & 1u) v.maxSubtris = std::max<int64_t>(v.maxSubtris, (int64_t)c.numSubtris + kSubtrisSlack);
       if (c.overflow & 2u) v.maxBin = std::max<int64_t>(v.maxBin, (int64_t)c.binEntries + kBinSlack);
       if (c.overflow & 4u) v.maxTile = std::max<int64_t>(v.maxTile, (int64_t)c.tileEntries + kTileSlack);
-      if (v.maxSubtris > kMaxEntries || v.maxBin > kMaxEntries || v.maxTile > kMaxEntries)
+      if (v.maxSubtris > kMaxSubtris)
+        return fail(b->ctx, CR_ERR_OVERFLOW, "batch view %zu: more than 2^28 subtriangles", i);
+      if (v.maxBin > kMaxEntries || v.maxTile > kMaxEntries)
         return fail(b->ctx, CR_ERR_OVERFLOW, "batch view %zu: internal queue capacity exceeded (2^31 entries)", i);
       over.push_back((int)i);
     }
@@ -551,7 +553,7 @@ cr_status cr_batch_autotune(cr_batch* b, void* stream_) {
     const Counters& c = b->hostCounters[i];
     ViewState& v = b->views[i];
     const int64_t sub = std::max<int64_t>(c.numSubtris, v.desc.num_tris);
-    v.maxSubtris = std::min<int64_t>(sub + sub / 16 + 1024, kMaxEntries);
+    v.maxSubtris = std::min<int64_t>(sub + sub / 16 + 1024, kMaxSubtris);
     v.maxBin = std::min<int64_t>((int64_t)c.binEntries + c.binEntries / 16 + 4096, kMaxEntries);
     v.maxTile = std::min<int64_t>((int64_t)c.tileEntries + c.tileEntries / 16 + 4096, kMaxEntries);
   }

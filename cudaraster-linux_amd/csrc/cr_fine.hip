@@ -39,7 +39,7 @@ struct Rec {
   static constexpr int D = 3 * N;              // per edge: dy, -dx (so that L = lx16*dy + ly16*(-dx))
   static constexpr int Z = D + 6;              // depth plane: (zx lo16, zy lo16), (zx hi16, zy hi16)
   static constexpr int ZS = Z + 2;             // depth at the tile's first pixel, per sample [N]
-  static constexpr int SLOT = ZS + N;          // subtriangle slot
+  static constexpr int SLOT = ZS + N;          // subtriangle slot (deferred shading: slot << 4)
   static constexpr int TRI = SLOT + 1;         // source triangle index
   static constexpr int SH = TRI + 1;           // in-order shading: W/U/V planes (9) + colours (12)
   static constexpr int SIZE_VIS = (SH + 3) & ~3;
@@ -171,6 +171,12 @@ __device__ __forceinline__ int xcdGroup(int groups) {
   const int per = (groups + 7) >> 3;
   return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
 }
+
+// Waves per SIMD the 1x-4x deferred-shading copies are held to (at most
+// 64 registers each): the batched copies' extra round loop would otherwise
+// cost them an eighth of their occupancy.
+template <int S, bool VIS>
+constexpr int fineMinWaves() { return (VIS && S <= 2) ? 8 : 1; }
 
 // LDS words of one wave's record array.
 template <int S, bool VIS>
@@ -317,7 +323,9 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
         const uint32_t base = z.x * tileSx + z.y * tileSy + z.z;
         sampleDepths<S>(r + L::ZS, base, z.x, z.y, std::make_integer_sequence<int, N>{});
       }
-      r[L::SLOT] = slot;
+      // Deferred shading records the (slot << 4 | centroid code) words the
+      // walk stores per sample; at 1x the code is always the pixel centre.
+      r[L::SLOT] = VIS ? ((slot << 4) | (N == 1 ? 15u : 0u)) : slot;
       if (!VIS) r[L::TRI] = (slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris];
       if (!VIS && gouraud) {
         const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
@@ -344,36 +352,45 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
 #pragma unroll
       for (int k = 0; k < 3; k++)
         Lk[k] = __mul24(lx16, (int32_t)cur[L::D + 2 * k]) + __mul24(ly16, (int32_t)cur[L::D + 2 * k + 1]);
-      bool cov[N];
-      bool any = false;
+      // Per-sample coverage and write masks are kept as explicit 64-bit lane
+      // masks (scalar registers): a per-lane bool carried from one block to
+      // the next would be rebuilt from a vector register before each ballot.
+      uint64_t covM[N];
+      uint64_t anyM = 0;
 #pragma unroll
       for (int i = 0; i < N; i++) {
-        cov[i] = (Lk[0] <= (int32_t)cur[L::R + i]) & (Lk[1] <= (int32_t)cur[L::R + N + i]) &
-                 (Lk[2] <= (int32_t)cur[L::R + 2 * N + i]);
-        any = any | cov[i];
+        covM[i] = __ballot(Lk[0] <= (int32_t)cur[L::R + i]) & __ballot(Lk[1] <= (int32_t)cur[L::R + N + i]) &
+                  __ballot(Lk[2] <= (int32_t)cur[L::R + 2 * N + i]);
+        anyM |= covM[i];
       }
-      const uint64_t covBallot = __ballot(any);
-      if (covBallot == 0) continue;
-      if (COUNT) frags += (uint64_t)__popcll(covBallot);
+      if (anyM == 0) continue;
+      if (COUNT) frags += (uint64_t)__popcll(anyM);
 
-      bool wr[N];
+      uint64_t wrM[N];
       uint32_t dz[N];
-      bool anyWr = false;
       if (DEPTH) {
         // zx*lxN + zy*lyN mod 2^32 from the 16-bit halves of zx and zy, one
         // dot product per half (lxN, lyN < 64: no product or sum overflows).
         const uint32_t pl = shl16Add(dot2u(r[L::Z + 1], pixN, 0u), dot2u(r[L::Z], pixN, 0u));
+        uint64_t anyWrM = 0;
 #pragma unroll
         for (int i = 0; i < N; i++) {
           dz[i] = r[L::ZS + i] + pl;
-          wr[i] = cov[i] & (dz[i] < dep[i]);               // LESS (FineRaster.inl:697)
-          anyWr = anyWr | wr[i];
+          wrM[i] = covM[i] & __ballot(dz[i] < dep[i]);     // LESS (FineRaster.inl:697)
+          anyWrM |= wrM[i];
         }
-        if (__ballot(anyWr) == 0) continue;
+        if (anyWrM == 0) continue;
       } else {
 #pragma unroll
-        for (int i = 0; i < N; i++) { wr[i] = cov[i]; dz[i] = 0; }
-        anyWr = any;
+        for (int i = 0; i < N; i++) { wrM[i] = covM[i]; dz[i] = 0; }
+      }
+      bool cov[N], wr[N];
+      bool anyWr = false;
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        cov[i] = __builtin_amdgcn_inverse_ballot_w64(covM[i]);
+        wr[i] = __builtin_amdgcn_inverse_ballot_w64(wrM[i]);
+        anyWr = anyWr | wr[i];
       }
       // Centroid (selectMSAACentroid): first covered sample in distance
       // order, the pixel centre (15) when none or all are covered.
@@ -386,7 +403,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
       if (all) code = 15;
 
       if (VIS) {
-        const uint32_t v = (r[L::SLOT] << 4) | (uint32_t)code;
+        const uint32_t v = N == 1 ? r[L::SLOT] : (r[L::SLOT] | (uint32_t)code);
 #pragma unroll
         for (int i = 0; i < N; i++) {
           if (DEPTH) dep[i] = wr[i] ? dz[i] : dep[i];
@@ -486,7 +503,7 @@ __device__ __forceinline__ void fineDispatch(const Params& P, const int group, u
 }
 
 template <int S, bool VIS, bool COUNT, bool DEPTH>
-__global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P) {
+__global__ void __launch_bounds__(64 * kFineWaves, (fineMinWaves<S, VIS>())) fineKernel(const Params P) {
   __shared__ __attribute__((aligned(16))) uint32_t recAll[kFineWaves * recWords<S, VIS>()];
   const int groups = (P.numTiles + kFineWaves - 1) / kFineWaves;
   const int group = xcdGroup(groups);
@@ -500,7 +517,7 @@ __global__ void __launch_bounds__(64 * kFineWaves) fineKernel(const Params P) {
 // draw); otherwise groups are dealt round-robin over the XCDs in list order,
 // which spreads the first-listed (heaviest) groups over all of them.
 template <int S, bool VIS, int W, bool DEPTH>
-__global__ void __launch_bounds__(64 * W) fineBatchKernel(const Params* __restrict__ views,
+__global__ void __launch_bounds__(64 * W, (fineMinWaves<S, VIS>())) fineBatchKernel(const Params* __restrict__ views,
                                                           const int32_t* __restrict__ groupStart,
                                                           const int32_t* __restrict__ order, int numViews,
                                                           int totalGroups, int banded) {

@@ -79,10 +79,13 @@ constexpr int64_t kBinSlack = 256 * 512;
 constexpr int64_t kTileSlack = 4096 * 32;
 constexpr int64_t kMaxEntries = (int64_t)INT32_MAX - 1;
 constexpr int32_t kMaxTris = 1 << 28;
+// Subtriangle slots: the fine stage packs slot << 4 | centroid code into one
+// 32-bit word per sample.
+constexpr int64_t kMaxSubtris = (int64_t)1 << 28;
 
 // First-draw capacities of a draw of nt triangles over geometry g.
 inline void initialCapacities(int64_t nt, const Geometry& g, int64_t& subtris, int64_t& bin, int64_t& tile) {
-  subtris = std::max(subtris, nt + kSubtrisSlack);
+  subtris = std::max(subtris, std::min(nt + kSubtrisSlack, kMaxSubtris));
   bin = std::max(bin, nt + (int64_t)g.nb * 64 + kBinSlack);
   tile = std::max(tile, 2 * nt + (int64_t)g.nt + kTileSlack);
 }

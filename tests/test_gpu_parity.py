@@ -215,6 +215,21 @@ def test_overflow_retry(oracle_lib):
     assert_parity(got, oracle_lib.render(s), "overflow")
 
 
+def test_subtriangle_capacity_limit():
+    """Subtriangle slots are capped at 2^28 (the fine stage packs slot << 4
+    with a centroid code into 32 bits): a larger reservation is refused
+    with an error instead of wrapping."""
+    _torch()
+    import cudaraster
+    cr = cudaraster.CudaRaster().init()
+    try:
+        cr.reserve(1 << 28, 1, 1)
+        with pytest.raises(cudaraster.CrError, match="2\\^28"):
+            cr.reserve((1 << 28) + 1, 1, 1)
+    finally:
+        cr.close()
+
+
 def test_hiz_does_not_change_image(oracle_lib):
     _torch()
     from cudaraster import scenes
