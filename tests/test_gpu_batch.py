@@ -222,3 +222,26 @@ def test_batch_rejects_shared_or_mismatched_surfaces():
         empty.close()
     finally:
         br.close()
+
+
+def test_full_64_view_workload_equals_plain_draws():
+    """The 64-view workload at full size (256^2-2048^2, 10k-500k triangles,
+    1x/4x, 8.6M triangles): after the autotuner, two graph replays give every
+    view exactly the images of a plain draw of it."""
+    torch = _torch()
+    import cudaraster
+    from cudaraster import scenes
+    views = scenes.multi_view(64, seed=3)
+    br = cudaraster.BatchRenderer(views)
+    try:
+        br.batch.autotune()
+        for rep in range(2):
+            br.draw()
+            torch.cuda.synchronize()
+            for i, v in enumerate(views):
+                ref = plain(v)
+                got = br.read(i)
+                for k in ("color", "depth", "triid"):
+                    assert np.array_equal(got[k], ref[k]), f"replay {rep}: view {i} ({v.name}) {k} differs"
+    finally:
+        br.close()

@@ -248,3 +248,15 @@ def test_full_c1_against_oracle(oracle_lib):
     got = render_gpu(s)
     ref = oracle_lib.render(s)
     assert_parity(got, ref, "c1 full")
+
+
+def test_full_c2_against_oracle(oracle_lib):
+    """4M sub-pixel triangles + 1024 near-full-screen ones, 2048^2, 8x MSAA,
+    SrcOver: full-size parity with the oracle (queue growth, ordered merges
+    of the large triangles' long tile lists, blending in submission order)."""
+    _torch()
+    from cudaraster import scenes
+    s = scenes.config(2)
+    got = render_gpu(s)
+    ref = oracle_lib.render(s)
+    assert_parity(got, ref, "c2 full")
