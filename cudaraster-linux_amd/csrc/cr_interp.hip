@@ -99,11 +99,12 @@ __global__ void __launch_bounds__(256) interpForwardKernel(const InterpGeom G, c
 // sum_c g_c a_k,c, dL/dc_j = (dL/db_j - sum_k b_k dL/db_k) / (c_0 + c_1 +
 // c_2), then through c_k = x'_{k+1} y'_{k+2} - y'_{k+1} x'_{k+2} and the
 // shear x' = x - qx w, y' = y - qy w to the clip x, y, w of each corner; the
-// results are staged in LDS.  Phases 2 and 3 issue the FP64 atomics
-// transposed, consecutive lanes on consecutive (pixel, corner, channel)
-// entries: with pixels grouped by triangle those land on consecutive
-// addresses, which device-scope atomics serve several times faster than
-// scattered ones.
+// results are staged in LDS.  The workgroup's pixels then form runs of one
+// triangle each (consecutive in the grouped order); phases 2 and 3 sum every
+// (run, corner, channel) and (run, corner, x/y/w) entry over the run's pixels
+// and issue one FP64 atomic per entry, consecutive lanes on consecutive
+// entries -- consecutive addresses, which device-scope atomics serve several
+// times faster than scattered ones.
 constexpr int kBwdThreads = 256;
 
 __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const InterpGeom G,
@@ -118,8 +119,11 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
                                                                     double* __restrict__ accAttr) {
   __shared__ double sB[kBwdThreads][3];        // barycentrics
   __shared__ double sP[kBwdThreads][9];        // dL/d(x, y, w) per corner
-  __shared__ int32_t sV[kBwdThreads][3];       // vertex indices, sV[.][0] < 0: no pixel
+  __shared__ int32_t sV[kBwdThreads][3];       // vertex indices
+  __shared__ int32_t sT[kBwdThreads];          // triangle id, -1: no pixel
   __shared__ uint32_t sPix[kBwdThreads];
+  __shared__ uint16_t sRun[kBwdThreads + 1];   // first pixel of each run, then the end
+  __shared__ uint32_t sWaveRuns[kBwdThreads / 64];
   const int64_t total = start[G.numTris];
   const int64_t j0 = (int64_t)blockIdx.x * kBwdThreads;
   if (j0 >= total) return;                     // uniform: past the visible pixels
@@ -127,7 +131,7 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
   const int C = G.numChannels;
 
   // ---- phase 1: one lane per pixel -----------------------------------------
-  sV[tid][0] = -1;
+  int32_t myTri = -1;
   const int64_t j = j0 + tid;
   if (j < total) {
     const int64_t p = order[j];
@@ -173,24 +177,48 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
         sP[tid][3 * k + 2] = -B.qx * dxs[k] - B.qy * dys[k];
       }
       sPix[tid] = (uint32_t)p;
+      myTri = t;
     }
   }
+  sT[tid] = myTri;
   __syncthreads();
 
-  // ---- phase 2: attribute gradients b_k g_c, entry e = (pixel, corner, channel)
-  const int per = 3 * C;
-  for (int e = tid; e < kBwdThreads * per; e += kBwdThreads) {
-    const int q = e / per, r = e - q * per, k = r / C, c = r - k * C;
-    if (sV[q][0] < 0) continue;
-    const double gc = (double)gradOut[(int64_t)sPix[q] * C + c];
-    if (gc != 0.0) unsafeAtomicAdd(&accAttr[(int64_t)sV[q][k] * C + c], sB[q][k] * gc);
+  // ---- runs: a pixel starts one where its triangle differs from the
+  // previous pixel's; run r covers [sRun[r], sRun[r + 1]) minus any trailing
+  // pixels without a triangle (sT < 0), which are skipped.
+  const bool head = myTri >= 0 && (tid == 0 || sT[tid - 1] != myTri);
+  const uint64_t hm = __ballot(head);
+  const int lane = tid & 63, wave = tid >> 6;
+  if (lane == 0) sWaveRuns[wave] = (uint32_t)__popcll(hm);
+  __syncthreads();
+  uint32_t before = 0, numRuns = 0;
+#pragma unroll
+  for (int w = 0; w < kBwdThreads / 64; w++) {
+    before += (w < wave) ? sWaveRuns[w] : 0u;
+    numRuns += sWaveRuns[w];
   }
-  // ---- phase 3: position gradients, entry e = (pixel, corner, x/y/w) ------
-  for (int e = tid; e < kBwdThreads * 9; e += kBwdThreads) {
-    const int q = e / 9, r = e - q * 9, k = r / 3, comp = r - k * 3;
-    if (sV[q][0] < 0) continue;
-    const double d = sP[q][r];
-    if (d != 0.0) unsafeAtomicAdd(&accPos[(int64_t)sV[q][k] * 4 + (comp == 2 ? 3 : comp)], d);
+  if (head) sRun[before + (uint32_t)__popcll(hm & ((1ull << lane) - 1ull))] = (uint16_t)tid;
+  if (tid == 0) sRun[numRuns] = (uint16_t)kBwdThreads;
+  __syncthreads();
+
+  // ---- phase 2: attribute gradients sum_q b_k g_c per (run, corner, channel)
+  const int per = 3 * C;
+  for (int e = tid; e < (int)numRuns * per; e += kBwdThreads) {
+    const int r = e / per, rem = e - r * per, k = rem / C, c = rem - k * C;
+    const int q0 = sRun[r], q1 = sRun[r + 1];
+    const int32_t t = sT[q0];
+    double s = 0.0;
+    for (int q = q0; q < q1 && sT[q] == t; q++) s += sB[q][k] * (double)gradOut[(int64_t)sPix[q] * C + c];
+    if (s != 0.0) unsafeAtomicAdd(&accAttr[(int64_t)sV[q0][k] * C + c], s);
+  }
+  // ---- phase 3: position gradients per (run, corner, x/y/w) ----------------
+  for (int e = tid; e < (int)numRuns * 9; e += kBwdThreads) {
+    const int r = e / 9, rem = e - r * 9, k = rem / 3, comp = rem - k * 3;
+    const int q0 = sRun[r], q1 = sRun[r + 1];
+    const int32_t t = sT[q0];
+    double s = 0.0;
+    for (int q = q0; q < q1 && sT[q] == t; q++) s += sP[q][rem];
+    if (s != 0.0) unsafeAtomicAdd(&accPos[(int64_t)sV[q0][k] * 4 + (comp == 2 ? 3 : comp)], s);
   }
 }
 
