@@ -254,8 +254,7 @@ void launchInterpForward(const InterpGeom& G, const uint32_t* ids, const float* 
                          const int32_t* tris, float* out, int32_t* triImage, hipStream_t stream);
 // accPos (4 per vertex) and accAttr (C per vertex) are FP64 scratch; the
 // results are rounded into gradPos / gradAttr.
-// Scratch of launchInterpBackward: FP64 accumulators (4 + C per vertex) and
-// the per-triangle pixel grouping (two words per triangle, one per pixel).
+// Scratch of launchInterpBackward: FP64 accumulators (4 + C per vertex).
 int64_t interpBackwardScratchBytes(const InterpGeom& G);
 void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const float* pos, const float* attr,
                           const int32_t* tris, const float* gradOut, void* scratch, float* gradPos,

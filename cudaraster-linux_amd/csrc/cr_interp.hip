@@ -93,24 +93,24 @@ __global__ void __launch_bounds__(256) interpForwardKernel(const InterpGeom G, c
   if (triImage) triImage[p] = (int32_t)t;
 }
 
-// Backward over the visible pixels taken in triangle order (order[]: pixel
-// indices grouped by triangle id, start[numTris] of them), 256 per
-// workgroup.  Phase 1, one lane per pixel: barycentrics, dL/db_k =
-// sum_c g_c a_k,c, dL/dc_j = (dL/db_j - sum_k b_k dL/db_k) / (c_0 + c_1 +
-// c_2), then through c_k = x'_{k+1} y'_{k+2} - y'_{k+1} x'_{k+2} and the
-// shear x' = x - qx w, y' = y - qy w to the clip x, y, w of each corner; the
-// results are staged in LDS.  The workgroup's pixels then form runs of one
-// triangle each (consecutive in the grouped order); phases 2 and 3 sum every
-// (run, corner, channel) and (run, corner, x/y/w) entry over the run's pixels
-// and issue one FP64 atomic per entry, consecutive lanes on consecutive
-// entries -- consecutive addresses, which device-scope atomics serve several
-// times faster than scattered ones.
-constexpr int kBwdThreads = 256;
+// Backward over 16x16-pixel tiles, one 256-lane workgroup each.  The tile's
+// pixels are first sorted by triangle id in LDS (bitonic, 64-bit keys: id,
+// then pixel), so every triangle's pixels of the tile are consecutive.
+// Phase 1, one lane per sorted pixel: barycentrics, dL/db_k = sum_c g_c
+// a_k,c, dL/dc_j = (dL/db_j - sum_k b_k dL/db_k) / (c_0 + c_1 + c_2), then
+// through c_k = x'_{k+1} y'_{k+2} - y'_{k+1} x'_{k+2} and the shear
+// x' = x - qx w, y' = y - qy w to the clip x, y, w of each corner; the
+// results are staged in LDS.  The pixels then form runs of one triangle
+// each; phases 2 and 3 sum every (run, corner, channel) and (run, corner,
+// x/y/w) entry over the run's pixels and issue one FP64 atomic per entry,
+// consecutive lanes on consecutive entries -- consecutive addresses within a
+// run, which device-scope atomics serve several times faster than scattered
+// ones.
+constexpr int kBwdTile = 16;
+constexpr int kBwdThreads = kBwdTile * kBwdTile;
 
 __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const InterpGeom G,
                                                                     const int32_t* __restrict__ triImage,
-                                                                    const uint32_t* __restrict__ order,
-                                                                    const uint32_t* __restrict__ start,
                                                                     const float4* __restrict__ pos,
                                                                     const float* __restrict__ attr,
                                                                     const int3* __restrict__ tris,
@@ -124,18 +124,42 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
   __shared__ uint32_t sPix[kBwdThreads];
   __shared__ uint16_t sRun[kBwdThreads + 1];   // first pixel of each run, then the end
   __shared__ uint32_t sWaveRuns[kBwdThreads / 64];
-  const int64_t total = start[G.numTris];
-  const int64_t j0 = (int64_t)blockIdx.x * kBwdThreads;
-  if (j0 >= total) return;                     // uniform: past the visible pixels
+  __shared__ uint64_t sKey[kBwdThreads];
   const int tid = threadIdx.x;
   const int C = G.numChannels;
 
-  // ---- phase 1: one lane per pixel -----------------------------------------
+  // ---- sort the tile's pixels by triangle id ---------------------------------
+  {
+    const int x = (int)blockIdx.x * kBwdTile + (tid & (kBwdTile - 1));
+    const int y = (int)blockIdx.y * kBwdTile + tid / kBwdTile;
+    uint32_t key = 0xFFFFFFFFu;                // no triangle: sorts last
+    if (x < G.width && y < G.height) {
+      const int32_t t = triImage[(int64_t)y * G.width + x];
+      if (t >= 0 && t < G.numTris) key = (uint32_t)t;
+    }
+    sKey[tid] = ((uint64_t)key << 32) | (uint32_t)((int64_t)y * G.width + x);
+  }
+  __syncthreads();
+  for (int k = 2; k <= kBwdThreads; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      const int o = tid ^ jj;
+      if (o > tid) {
+        const uint64_t a = sKey[tid], b = sKey[o];
+        if ((a > b) == ((tid & k) == 0)) {
+          sKey[tid] = b;
+          sKey[o] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- phase 1: one lane per sorted pixel ------------------------------------
   int32_t myTri = -1;
-  const int64_t j = j0 + tid;
-  if (j < total) {
-    const int64_t p = order[j];
-    const int32_t t = triImage[p];
+  const uint64_t myKey = sKey[tid];
+  if ((uint32_t)(myKey >> 32) != 0xFFFFFFFFu) {
+    const int64_t p = (int64_t)(uint32_t)myKey;
+    const int32_t t = (int32_t)(myKey >> 32);
     const int3 v = tris[t];
     const int32_t i[3] = {v.x, v.y, v.z};
     bool ok = true;
@@ -222,101 +246,6 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
   }
 }
 
-// ---- grouping the visible pixels by triangle (counting sort) --------------
-
-__global__ void __launch_bounds__(256) countTrianglePixelsKernel(const InterpGeom G,
-                                                                 const int32_t* __restrict__ triImage,
-                                                                 uint32_t* __restrict__ cnt) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (int64_t)G.width * G.height) return;
-  const int32_t t = triImage[p];
-  if (t >= 0 && t < G.numTris) atomicAdd(&cnt[t], 1u);
-}
-
-__global__ void __launch_bounds__(256) placeTrianglePixelsKernel(const InterpGeom G,
-                                                                 const int32_t* __restrict__ triImage,
-                                                                 const uint32_t* __restrict__ start,
-                                                                 uint32_t* __restrict__ fill,
-                                                                 uint32_t* __restrict__ order) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (int64_t)G.width * G.height) return;
-  const int32_t t = triImage[p];
-  if (t >= 0 && t < G.numTris) order[start[t] + atomicAdd(&fill[t], 1u)] = (uint32_t)p;
-}
-
-// Exclusive scan, kScanItems values per 1024-lane block; the block totals go
-// to sums[] (scanned by the next level, then added back).
-constexpr int kScanThreads = 1024, kScanPer = 4, kScanItems = kScanThreads * kScanPer;
-
-__global__ void __launch_bounds__(kScanThreads) scanBlocksKernel(uint32_t* __restrict__ d, int64_t n,
-                                                                 uint32_t* __restrict__ sums) {
-  __shared__ uint32_t waveSums[kScanThreads / 64];
-  const int64_t base = (int64_t)blockIdx.x * kScanItems + (int64_t)threadIdx.x * kScanPer;
-  uint32_t v[kScanPer], run = 0;
-#pragma unroll
-  for (int q = 0; q < kScanPer; q++) {
-    v[q] = (base + q < n) ? d[base + q] : 0u;
-    const uint32_t t = v[q];
-    v[q] = run;
-    run += t;
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t x = run;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) waveSums[wave] = x;
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    const uint32_t w = (threadIdx.x < kScanThreads / 64) ? waveSums[threadIdx.x] : 0u;
-    uint32_t sc = w;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(sc, o, 64);
-      if ((int)threadIdx.x >= o) sc += y;
-    }
-    if (threadIdx.x < kScanThreads / 64) waveSums[threadIdx.x] = sc - w;
-    if (threadIdx.x == kScanThreads / 64 - 1) sums[blockIdx.x] = sc;
-  }
-  __syncthreads();
-  const uint32_t off = waveSums[wave] + x - run;
-#pragma unroll
-  for (int q = 0; q < kScanPer; q++)
-    if (base + q < n) d[base + q] = v[q] + off;
-}
-
-__global__ void __launch_bounds__(kScanThreads) scanAddKernel(uint32_t* __restrict__ d, int64_t n,
-                                                              const uint32_t* __restrict__ sums) {
-  const int64_t base = (int64_t)blockIdx.x * kScanItems + (int64_t)threadIdx.x * kScanPer;
-  const uint32_t off = sums[blockIdx.x];
-#pragma unroll
-  for (int q = 0; q < kScanPer; q++)
-    if (base + q < n) d[base + q] += off;
-}
-
-// Words of scratch scanExclusive needs beyond the data for n values.
-static int64_t scanScratchWords(int64_t n) {
-  int64_t w = 0;
-  for (int64_t m = n; m > 1;) {
-    m = (m + kScanItems - 1) / kScanItems;
-    w += m;
-  }
-  return w + 1;
-}
-
-static void scanExclusive(uint32_t* d, int64_t n, uint32_t* scratch, hipStream_t stream) {
-  if (n <= 0) return;
-  const int64_t nb = (n + kScanItems - 1) / kScanItems;
-  hipLaunchKernelGGL(scanBlocksKernel, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, d, n, scratch);
-  if (nb > 1) {
-    scanExclusive(scratch, nb, scratch + nb, stream);
-    hipLaunchKernelGGL(scanAddKernel, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, d, n,
-                       (const uint32_t*)scratch);
-  }
-}
-
 // Zeroes n words at p (16-byte aligned): 16-byte stores, then the tail.
 __global__ void __launch_bounds__(256) fillZeroU32Kernel(uint32_t* __restrict__ p, int64_t n) {
   const int64_t n4 = n >> 2;
@@ -349,35 +278,25 @@ void launchInterpForward(const InterpGeom& G, const uint32_t* ids, const float* 
 
 int64_t interpBackwardScratchBytes(const InterpGeom& G) {
   const int64_t nPos = G.numVertices * 4, nAttr = G.numVertices * G.numChannels;
-  const int64_t pixels = (int64_t)G.width * G.height, nt = (int64_t)G.numTris + 1;
-  return (nPos + nAttr) * 8 + (2 * nt + pixels + scanScratchWords(nt)) * 4;
+  return (nPos + nAttr) * 8;
 }
 
 void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const float* pos, const float* attr,
                           const int32_t* tris, const float* gradOut, void* scratch, float* gradPos,
                           float* gradAttr, hipStream_t stream) {
   const int64_t nPos = G.numVertices * 4, nAttr = G.numVertices * G.numChannels;
-  const int64_t n = (int64_t)G.width * G.height, nt = (int64_t)G.numTris + 1;
   double* accPos = static_cast<double*>(scratch);
   double* accAttr = accPos + nPos;
-  uint32_t* start = reinterpret_cast<uint32_t*>(accAttr + nAttr);   // counts, then exclusive offsets [nt]
-  uint32_t* fill = start + nt;                                        // per-triangle placement cursor [nt]
-  uint32_t* order = fill + nt;                                        // pixel indices by triangle [n]
-  uint32_t* scanTmp = order + n;
-  // Accumulators, counts and cursors are contiguous: one clearing launch.
-  const int64_t clearWords = (nPos + nAttr) * 2 + 2 * nt;
-  hipLaunchKernelGGL(fillZeroU32Kernel, dim3(gridFor((clearWords + 3) / 4)), dim3(256), 0, stream,
-                     reinterpret_cast<uint32_t*>(accPos), clearWords);
-  if (n > 0) {
-    hipLaunchKernelGGL(countTrianglePixelsKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, G,
-                       triImage, start);
-    scanExclusive(start, nt, scanTmp, stream);      // start[numTris] = number of visible pixels
-    hipLaunchKernelGGL(placeTrianglePixelsKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, G,
-                       triImage, (const uint32_t*)start, fill, order);
-    hipLaunchKernelGGL(interpBackwardKernel, dim3((unsigned)((n + kBwdThreads - 1) / kBwdThreads)), dim3(kBwdThreads),
-                       0, stream, G, triImage,
-                       (const uint32_t*)order, (const uint32_t*)start, reinterpret_cast<const float4*>(pos), attr,
-                       reinterpret_cast<const int3*>(tris), gradOut, accPos, accAttr);
+  // Both FP64 accumulators are contiguous: one clearing launch.
+  const int64_t clearWords = (nPos + nAttr) * 2;
+  if (clearWords > 0)
+    hipLaunchKernelGGL(fillZeroU32Kernel, dim3(gridFor((clearWords + 3) / 4)), dim3(256), 0, stream,
+                       reinterpret_cast<uint32_t*>(accPos), clearWords);
+  if (G.width > 0 && G.height > 0 && G.numTris > 0) {
+    const dim3 grid((unsigned)((G.width + kBwdTile - 1) / kBwdTile), (unsigned)((G.height + kBwdTile - 1) / kBwdTile));
+    hipLaunchKernelGGL(interpBackwardKernel, grid, dim3(kBwdThreads), 0, stream, G, triImage,
+                       reinterpret_cast<const float4*>(pos), attr, reinterpret_cast<const int3*>(tris), gradOut,
+                       accPos, accAttr);
   }
   if (nPos + nAttr > 0)
     hipLaunchKernelGGL(roundToFloatKernel, dim3(gridFor(nPos + nAttr)), dim3(256), 0, stream, (const double*)accPos,
