@@ -298,7 +298,7 @@ cr_status cr_context_destroy(cr_context* ctx) {
   if (ctx->pending) (void)hipStreamSynchronize(ctx->pendingStream);
   for (DevBuf* b : {&ctx->triSubtris, &ctx->triHeader, &ctx->triData, &ctx->subtriTri, &ctx->chunkBase,
                     &ctx->binCnt, &ctx->binOff, &ctx->binList, &ctx->tileStart, &ctx->tileCount,
-                    &ctx->tileList, &ctx->tileFrags, &ctx->chunkSubtris, &ctx->counters, &ctx->interpScratch})
+                    &ctx->tileList, &ctx->tileFrags, &ctx->chunkSubtris, &ctx->counters})
     b->release();
   if (ctx->hostCounters) (void)hipHostFree(ctx->hostCounters);
   for (int i = 0; i < 5; i++)
@@ -583,9 +583,7 @@ cr_status cr_interpolate_backward(cr_context* ctx, const int32_t* triImage, int3
     return fail(ctx, CR_ERR_INVALID, "cr_interpolate_backward: bad sizes or buffers");
   CR_HIP(ctx, hipSetDevice(ctx->device));
   InterpGeom G{width, height, width, numChannels, numTris, numVertices};
-  CR_HIP(ctx, ctx->interpScratch.ensure((size_t)interpBackwardScratchBytes(G)));
-  launchInterpBackward(G, triImage, pos, attr, tris, gradOut, ctx->interpScratch.as<void>(), gradPos, gradAttr,
-                       static_cast<hipStream_t>(stream));
+  launchInterpBackward(G, triImage, pos, attr, tris, gradOut, gradPos, gradAttr, static_cast<hipStream_t>(stream));
   CR_HIP(ctx, hipGetLastError());
   return CR_OK;
 }

@@ -252,13 +252,11 @@ struct InterpGeom {
 };
 void launchInterpForward(const InterpGeom& G, const uint32_t* ids, const float* pos, const float* attr,
                          const int32_t* tris, float* out, int32_t* triImage, hipStream_t stream);
-// accPos (4 per vertex) and accAttr (C per vertex) are FP64 scratch; the
-// results are rounded into gradPos / gradAttr.
-// Scratch of launchInterpBackward: FP64 accumulators (4 + C per vertex).
-int64_t interpBackwardScratchBytes(const InterpGeom& G);
+// gradPos (4 per vertex) and gradAttr (C per vertex) are cleared, then
+// accumulated with FP32 atomics (per-pixel terms summed in FP64 first).
 void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const float* pos, const float* attr,
-                          const int32_t* tris, const float* gradOut, void* scratch, float* gradPos,
-                          float* gradAttr, hipStream_t stream);
+                          const int32_t* tris, const float* gradOut, float* gradPos, float* gradAttr,
+                          hipStream_t stream);
 
 // Largest v with start[v] <= b: the view owning workgroup (or group) b.
 // start has numViews + 1 nondecreasing entries; views without work have

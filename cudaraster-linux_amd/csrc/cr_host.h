@@ -131,7 +131,6 @@ struct cr_context {
   crhost::DevBuf chunkBase, binCnt, binOff, binList;
   crhost::DevBuf tileStart, tileCount, tileList, tileFrags, chunkSubtris;
   crhost::DevBuf counters;
-  crhost::DevBuf interpScratch;       // cr_interpolate_backward: FP64 accumulators
   crhip::Counters* hostCounters = nullptr;   // pinned
   hipEvent_t ev[5] = {};
   bool evRecorded = false;

@@ -115,8 +115,8 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
                                                                     const float* __restrict__ attr,
                                                                     const int3* __restrict__ tris,
                                                                     const float* __restrict__ gradOut,
-                                                                    double* __restrict__ accPos,
-                                                                    double* __restrict__ accAttr) {
+                                                                    float* __restrict__ gradPos,
+                                                                    float* __restrict__ gradAttr) {
   __shared__ double sB[kBwdThreads][3];        // barycentrics
   __shared__ double sP[kBwdThreads][9];        // dL/d(x, y, w) per corner
   __shared__ int32_t sV[kBwdThreads][3];       // vertex indices
@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
     const int32_t t = sT[q0];
     double s = 0.0;
     for (int q = q0; q < q1 && sT[q] == t; q++) s += sB[q][k] * (double)gradOut[(int64_t)sPix[q] * C + c];
-    if (s != 0.0) unsafeAtomicAdd(&accAttr[(int64_t)sV[q0][k] * C + c], s);
+    if (s != 0.0) unsafeAtomicAdd(&gradAttr[(int64_t)sV[q0][k] * C + c], (float)s);
   }
   // ---- phase 3: position gradients per (run, corner, x/y/w) ----------------
   for (int e = tid; e < (int)numRuns * 9; e += kBwdThreads) {
@@ -242,27 +242,16 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
     const int32_t t = sT[q0];
     double s = 0.0;
     for (int q = q0; q < q1 && sT[q] == t; q++) s += sP[q][rem];
-    if (s != 0.0) unsafeAtomicAdd(&accPos[(int64_t)sV[q0][k] * 4 + (comp == 2 ? 3 : comp)], s);
+    if (s != 0.0) unsafeAtomicAdd(&gradPos[(int64_t)sV[q0][k] * 4 + (comp == 2 ? 3 : comp)], (float)s);
   }
 }
 
-// Zeroes n words at p (16-byte aligned): 16-byte stores, then the tail.
-__global__ void __launch_bounds__(256) fillZeroU32Kernel(uint32_t* __restrict__ p, int64_t n) {
-  const int64_t n4 = n >> 2;
-  uint4* __restrict__ q = reinterpret_cast<uint4*>(p);
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n4; k += (int64_t)gridDim.x * blockDim.x)
-    q[k] = make_uint4(0u, 0u, 0u, 0u);
-  if (blockIdx.x == 0 && threadIdx.x < (unsigned)(n & 3)) p[(n4 << 2) + threadIdx.x] = 0u;
-}
-
-// Both gradient arrays in one launch: [0, na) from a to da, then [na, na + nb)
-// from b to db.
-__global__ void __launch_bounds__(256) roundToFloatKernel(const double* __restrict__ a, float* __restrict__ da,
-                                                          int64_t na, const double* __restrict__ b,
-                                                          float* __restrict__ db, int64_t nb) {
+// Zeroes [0, na) of a and [0, nb) of b (any 4-byte alignment).
+__global__ void __launch_bounds__(256) zeroTwoKernel(float* __restrict__ a, int64_t na, float* __restrict__ b,
+                                                     int64_t nb) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < na + nb; k += (int64_t)gridDim.x * blockDim.x) {
-    if (k < na) da[k] = (float)a[k];
-    else db[k - na] = (float)b[k - na];
+    if (k < na) a[k] = 0.0f;
+    else b[k - na] = 0.0f;
   }
 }
 
@@ -276,31 +265,19 @@ void launchInterpForward(const InterpGeom& G, const uint32_t* ids, const float* 
                      reinterpret_cast<const float4*>(pos), attr, reinterpret_cast<const int3*>(tris), out, triImage);
 }
 
-int64_t interpBackwardScratchBytes(const InterpGeom& G) {
-  const int64_t nPos = G.numVertices * 4, nAttr = G.numVertices * G.numChannels;
-  return (nPos + nAttr) * 8;
-}
-
 void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const float* pos, const float* attr,
-                          const int32_t* tris, const float* gradOut, void* scratch, float* gradPos,
-                          float* gradAttr, hipStream_t stream) {
+                          const int32_t* tris, const float* gradOut, float* gradPos, float* gradAttr,
+                          hipStream_t stream) {
   const int64_t nPos = G.numVertices * 4, nAttr = G.numVertices * G.numChannels;
-  double* accPos = static_cast<double*>(scratch);
-  double* accAttr = accPos + nPos;
-  // Both FP64 accumulators are contiguous: one clearing launch.
-  const int64_t clearWords = (nPos + nAttr) * 2;
-  if (clearWords > 0)
-    hipLaunchKernelGGL(fillZeroU32Kernel, dim3(gridFor((clearWords + 3) / 4)), dim3(256), 0, stream,
-                       reinterpret_cast<uint32_t*>(accPos), clearWords);
+  if (nPos + nAttr > 0)
+    hipLaunchKernelGGL(zeroTwoKernel, dim3(gridFor(nPos + nAttr)), dim3(256), 0, stream, gradPos, nPos, gradAttr,
+                       nAttr);
   if (G.width > 0 && G.height > 0 && G.numTris > 0) {
     const dim3 grid((unsigned)((G.width + kBwdTile - 1) / kBwdTile), (unsigned)((G.height + kBwdTile - 1) / kBwdTile));
     hipLaunchKernelGGL(interpBackwardKernel, grid, dim3(kBwdThreads), 0, stream, G, triImage,
                        reinterpret_cast<const float4*>(pos), attr, reinterpret_cast<const int3*>(tris), gradOut,
-                       accPos, accAttr);
+                       gradPos, gradAttr);
   }
-  if (nPos + nAttr > 0)
-    hipLaunchKernelGGL(roundToFloatKernel, dim3(gridFor(nPos + nAttr)), dim3(256), 0, stream, (const double*)accPos,
-                       gradPos, nPos, (const double*)accAttr, gradAttr, nAttr);
 }
 
 }  // namespace crhip

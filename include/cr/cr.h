@@ -281,8 +281,8 @@ cr_status cr_batch_get_view_counters(const cr_batch* batch, int32_t view, cr_cou
  * rasterizer's triangle ids (interior barycentric gradients; no edge or
  * occlusion terms).  Images are row-major, row 0 at the bottom (the surface
  * convention), pixel (x, y) at NDC ((2x+1)/W - 1, (2y+1)/H - 1).  Both calls
- * are asynchronous on `stream`; barycentrics and gradients are evaluated in
- * FP64 and rounded to FP32 once.
+ * are asynchronous on `stream`; barycentrics and per-pixel gradient terms
+ * are evaluated in FP64.
  *
  * pos:  float4 clip position per vertex (16-byte aligned); attr: C floats per
  * vertex; tris: int3 per triangle (the index buffer the frame was drawn with).
@@ -294,8 +294,10 @@ cr_status cr_interpolate(cr_context* ctx, const cr_surface* tri_id, const float*
                          const float* attr, int32_t num_channels, const int32_t* tris, int32_t num_tris,
                          float* out, int32_t* tri_image, void* stream);
 /* Backward: overwrites grad_pos[V][4] (z component 0) and grad_attr[V][C]
- * with dL/dpos and dL/dattr given grad_out[y][x][c] = dL/dout.  Uses FP64
- * scratch held by the context: calls on one context must not overlap. */
+ * with dL/dpos and dL/dattr given grad_out[y][x][c] = dL/dout.  The terms
+ * of each triangle's pixels within a 16x16 tile are summed in FP64 and added
+ * to the outputs with FP32 atomics (summation order varies between runs in
+ * the last bits).  Uses no context scratch. */
 cr_status cr_interpolate_backward(cr_context* ctx, const int32_t* tri_image, int32_t width, int32_t height,
                                   const float* pos, int64_t num_vertices, const float* attr,
                                   int32_t num_channels, const int32_t* tris, int32_t num_tris,
