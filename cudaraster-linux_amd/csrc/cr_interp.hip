@@ -108,6 +108,7 @@ __global__ void __launch_bounds__(256) interpForwardKernel(const InterpGeom G, c
 // ones.
 constexpr int kBwdTile = 16;
 constexpr int kBwdThreads = kBwdTile * kBwdTile;
+constexpr int kBwdStageC = 8;                  // channels of dL/dout staged in LDS per pixel
 
 __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const InterpGeom G,
                                                                     const int32_t* __restrict__ triImage,
@@ -125,6 +126,7 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
   __shared__ uint16_t sRun[kBwdThreads + 1];   // first pixel of each run, then the end
   __shared__ uint32_t sWaveRuns[kBwdThreads / 64];
   __shared__ uint64_t sKey[kBwdThreads];
+  __shared__ float sG[kBwdThreads][kBwdStageC];   // dL/dout of the first channels, per pixel
   const int tid = threadIdx.x;
   const int C = G.numChannels;
 
@@ -173,6 +175,7 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
     }
     if (ok) {
       const float* g = gradOut + p * C;
+      for (int c = 0; c < min(C, kBwdStageC); c++) sG[tid][c] = g[c];
       double dB[3] = {0.0, 0.0, 0.0};
       for (int k = 0; k < 3; k++) {
         const float* a = attr + (int64_t)i[k] * C;
@@ -232,7 +235,11 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
     const int q0 = sRun[r], q1 = sRun[r + 1];
     const int32_t t = sT[q0];
     double s = 0.0;
-    for (int q = q0; q < q1 && sT[q] == t; q++) s += sB[q][k] * (double)gradOut[(int64_t)sPix[q] * C + c];
+    if (c < kBwdStageC) {
+      for (int q = q0; q < q1 && sT[q] == t; q++) s += sB[q][k] * (double)sG[q][c];
+    } else {
+      for (int q = q0; q < q1 && sT[q] == t; q++) s += sB[q][k] * (double)gradOut[(int64_t)sPix[q] * C + c];
+    }
     if (s != 0.0) unsafeAtomicAdd(&gradAttr[(int64_t)sV[q0][k] * C + c], (float)s);
   }
   // ---- phase 3: position gradients per (run, corner, x/y/w) ----------------
