@@ -253,13 +253,24 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
   }
 }
 
-// Zeroes [0, na) of a and [0, nb) of b (any 4-byte alignment).
+// Zeroes [0, n) of p: up to three floats until a 16-byte boundary, then
+// 16-byte stores, then the tail; `lane` and `lanes` stride the grid.
+__device__ __forceinline__ void zeroFloats(float* __restrict__ p, int64_t n, int64_t lane, int64_t lanes) {
+  const int64_t head = min<int64_t>(n, (int64_t)(((16u - ((uintptr_t)p & 15u)) & 15u) >> 2));
+  if (lane < head) p[lane] = 0.0f;
+  float4* __restrict__ q = reinterpret_cast<float4*>(p + head);
+  const int64_t n4 = (n - head) >> 2;
+  for (int64_t k = lane; k < n4; k += lanes) q[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const int64_t tail = head + (n4 << 2);
+  if (lane < n - tail) p[tail + lane] = 0.0f;
+}
+
+// Zeroes [0, na) of a and [0, nb) of b (4-byte aligned).
 __global__ void __launch_bounds__(256) zeroTwoKernel(float* __restrict__ a, int64_t na, float* __restrict__ b,
                                                      int64_t nb) {
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < na + nb; k += (int64_t)gridDim.x * blockDim.x) {
-    if (k < na) a[k] = 0.0f;
-    else b[k - na] = 0.0f;
-  }
+  const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, lanes = (int64_t)gridDim.x * blockDim.x;
+  zeroFloats(a, na, lane, lanes);
+  zeroFloats(b, nb, lane, lanes);
 }
 
 static int gridFor(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 1 << 20); }
@@ -277,8 +288,8 @@ void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const fl
                           hipStream_t stream) {
   const int64_t nPos = G.numVertices * 4, nAttr = G.numVertices * G.numChannels;
   if (nPos + nAttr > 0)
-    hipLaunchKernelGGL(zeroTwoKernel, dim3(gridFor(nPos + nAttr)), dim3(256), 0, stream, gradPos, nPos, gradAttr,
-                       nAttr);
+    hipLaunchKernelGGL(zeroTwoKernel, dim3(gridFor((nPos + nAttr + 3) / 4)), dim3(256), 0, stream, gradPos, nPos,
+                       gradAttr, nAttr);
   if (G.width > 0 && G.height > 0 && G.numTris > 0) {
     const dim3 grid((unsigned)((G.width + kBwdTile - 1) / kBwdTile), (unsigned)((G.height + kBwdTile - 1) / kBwdTile));
     hipLaunchKernelGGL(interpBackwardKernel, grid, dim3(kBwdThreads), 0, stream, G, triImage,
