@@ -160,9 +160,13 @@ __device__ __forceinline__ void setupTriangle(const Params& P, Header* th, TriDa
   }
   uint4* tdv = reinterpret_cast<uint4*>(td);
   tdv[0] = make_uint4(zp[0], zp[1], zp[2], zslope);
-  tdv[1] = make_uint4(wp[0], wp[1], wp[2], up[0]);
-  tdv[2] = make_uint4(up[1], up[2], vp[0], vp[1]);
-  tdv[3] = make_uint4(vp[2], (uint32_t)vidx.x, (uint32_t)vidx.y, (uint32_t)vidx.z);
+  if (P.shader == kShaderGouraud) {
+    // Barycentric planes and vertex indices: read only by the Gouraud
+    // shader (a constant-colour draw writes the depth plane alone).
+    tdv[1] = make_uint4(wp[0], wp[1], wp[2], up[0]);
+    tdv[2] = make_uint4(up[1], up[2], vp[0], vp[1]);
+    tdv[3] = make_uint4(vp[2], (uint32_t)vidx.x, (uint32_t)vidx.y, (uint32_t)vidx.z);
+  }
 
   uint32_t f01 = selectFlips(d1.x, d1.y);
   uint32_t f12 = selectFlips(wsub(d2.x, d1.x), wsub(d2.y, d1.y));
