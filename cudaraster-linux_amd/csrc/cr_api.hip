@@ -266,7 +266,7 @@ cr_status drawImpl(cr_context* ctx, hipStream_t stream, bool sync) {
 
 extern "C" {
 
-int32_t cr_abi_version(void) { return (1 << 16) | 3; }
+int32_t cr_abi_version(void) { return (1 << 16) | 4; }
 
 const char* cr_last_error(const cr_context* ctx) { return ctx ? ctx->err.c_str() : g_lastError.c_str(); }
 
@@ -584,6 +584,20 @@ cr_status cr_interpolate_backward(cr_context* ctx, const int32_t* triImage, int3
   CR_HIP(ctx, hipSetDevice(ctx->device));
   InterpGeom G{width, height, width, numChannels, numTris, numVertices};
   launchInterpBackward(G, triImage, pos, attr, tris, gradOut, gradPos, gradAttr, static_cast<hipStream_t>(stream));
+  CR_HIP(ctx, hipGetLastError());
+  return CR_OK;
+}
+
+cr_status cr_vertex_shader_passthrough(cr_context* ctx, const float* modelPos, int64_t numVertices,
+                                       const float* posToClip, void* outVertices, int32_t outStride,
+                                       void* stream) {
+  if (!ctx || !posToClip || (numVertices > 0 && (!modelPos || !outVertices)))
+    return fail(ctx, CR_ERR_INVALID, "cr_vertex_shader_passthrough: NULL argument");
+  if (numVertices < 0 || outStride < 16 || (outStride & 15) || ((uintptr_t)outVertices & 15) ||
+      ((uintptr_t)modelPos & 3))
+    return fail(ctx, CR_ERR_INVALID, "cr_vertex_shader_passthrough: bad size, stride or alignment");
+  CR_HIP(ctx, hipSetDevice(ctx->device));
+  launchVertexPassthrough(modelPos, numVertices, posToClip, outVertices, outStride, static_cast<hipStream_t>(stream));
   CR_HIP(ctx, hipGetLastError());
   return CR_OK;
 }

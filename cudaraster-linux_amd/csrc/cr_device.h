@@ -250,6 +250,10 @@ struct InterpGeom {
   int32_t numTris;
   int64_t numVertices;
 };
+// Passthrough vertex shader: clip position posToClip * (p, 1) of each
+// model position into the first 16 bytes of each outStride-byte record.
+void launchVertexPassthrough(const float* modelPos, int64_t numVertices, const float posToClip[16], void* out,
+                             int64_t outStride, hipStream_t stream);
 void launchInterpForward(const InterpGeom& G, const uint32_t* ids, const float* pos, const float* attr,
                          const int32_t* tris, float* out, int32_t* triImage, hipStream_t stream);
 // gradPos (4 per vertex) and gradAttr (C per vertex) are cleared, then

@@ -39,7 +39,7 @@ EXPORTS = [
     "cr_get_stats", "cr_get_counters", "cr_get_profiling_info", "cr_set_debug_params", "cr_abi_version",
     "cr_batch_create", "cr_batch_destroy", "cr_batch_draw", "cr_batch_autotune", "cr_batch_reserve",
     "cr_batch_use_graph", "cr_batch_get_info", "cr_batch_get_view_counters",
-    "cr_interpolate", "cr_interpolate_backward", "cr_batch_configure",
+    "cr_interpolate", "cr_interpolate_backward", "cr_batch_configure", "cr_vertex_shader_passthrough",
 ]
 
 
@@ -150,6 +150,7 @@ def load(path: str | None = None):
         "cr_interpolate": [vp, vp, vp, i64, vp, i32, vp, i32, vp, vp, vp],
         "cr_batch_configure": [vp, vp],
         "cr_interpolate_backward": [vp, vp, i32, i32, vp, i64, vp, i32, vp, i32, vp, vp, vp, vp],
+        "cr_vertex_shader_passthrough": [vp, vp, i64, ctypes.POINTER(ctypes.c_float), vp, i32, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -325,6 +326,19 @@ class CudaRaster:
 
     def drawCheck(self):
         return load().cr_draw_check(self.ctx)
+
+    def vertexShaderPassthrough(self, model_pos, pos_to_clip, out, out_stride=16, num_vertices=None, stream=None):
+        """The demo's passthrough vertex shader (test/shader/PassThrough.cu:16-35)
+        on the GPU: clip position pos_to_clip * (p, 1) of each model position
+        (device buffer of Vec3f) into the first 16 bytes of each out_stride-byte
+        record of `out` (a vertex buffer for setVertexBuffer).  pos_to_clip
+        is a 4x4 matrix indexed [row][column] (numpy or nested lists)."""
+        m = np.ascontiguousarray(np.asarray(pos_to_clip, dtype=np.float32).reshape(4, 4).T).reshape(16)
+        n = (model_pos.numel() // 3) if num_vertices is None else int(num_vertices)
+        s = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
+        _check(load().cr_vertex_shader_passthrough(self.ctx, _ptr(model_pos), n,
+                                                   m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _ptr(out),
+                                                   int(out_stride), s), self.ctx)
 
     def reserve(self, max_subtris, max_bin_entries, max_tile_entries):
         _check(load().cr_reserve(self.ctx, int(max_subtris), int(max_bin_entries), int(max_tile_entries)), self.ctx)

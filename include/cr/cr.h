@@ -303,6 +303,22 @@ cr_status cr_interpolate_backward(cr_context* ctx, const int32_t* tri_image, int
                                   int32_t num_channels, const int32_t* tris, int32_t num_tris,
                                   const float* grad_out, float* grad_pos, float* grad_attr, void* stream);
 
+/* Passthrough vertex shader, the step before drawTriangles: replaces
+ * vertexShader_passthrough (test/shader/PassThrough.cu:16-35) as the demo
+ * launches it (test/SceneCR.cpp:267-282).  For each of num_vertices model
+ * positions (Vec3f, 12 bytes each) writes the clip position
+ * pos_to_clip * (p, 1) -- Mat4f * Vec4f of base/Math.hpp:990-1001, bit for
+ * bit -- as 4 floats at the start of the vertex's out_stride-byte output
+ * record, leaving the rest of the record (varyings) untouched; the output is
+ * a vertex buffer for cr_set_vertex_buffer.  pos_to_clip is 16 floats in
+ * Mat4f memory order (column-major, the reference's Mat4f::getPtr()), in
+ * host memory.
+ * out_vertices must be 16-byte aligned and out_stride a positive multiple of
+ * 16.  Asynchronous on `stream`. */
+cr_status cr_vertex_shader_passthrough(cr_context* ctx, const float* model_pos, int64_t num_vertices,
+                                       const float* pos_to_clip, void* out_vertices, int32_t out_stride,
+                                       void* stream);
+
 /* Version of the ABI (major << 16 | minor). */
 int32_t cr_abi_version(void);
 

@@ -941,3 +941,19 @@ int32_t cro_render(const cro_pipe* P, const void* vertices, int64_t numVertices,
     free(bandStart); free(fill); free(bandList);
     return 0;
 }
+
+/* ---- passthrough vertex shader (test/shader/PassThrough.cu:16-35) -------- */
+
+void cro_vertex_passthrough(const float* modelPos, int64_t numVertices, const float posToClip[16],
+                            float* out, int64_t outStrideFloats) {
+    for (int64_t v = 0; v < numVertices; v++) {
+        const float in[4] = {modelPos[3 * v + 0], modelPos[3 * v + 1], modelPos[3 * v + 2], 1.0f};
+        for (int i = 0; i < 4; i++) {
+            /* MatrixBase::operator*(Vector) (base/Math.hpp:990-1001): rr = 0;
+             * rr += get(i, j) * v[j], contracted to fmaf; get(i, j) = m[i + 4j]. */
+            float rr = 0.0f;
+            for (int j = 0; j < 4; j++) rr = fmaf(posToClip[i + 4 * j], in[j], rr);
+            out[outStrideFloats * v + i] = rr;
+        }
+    }
+}

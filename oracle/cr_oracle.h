@@ -140,6 +140,17 @@ int32_t  cro_render(const cro_pipe* pipe, const void* vertices, int64_t numVerti
                     uint32_t* color, uint32_t* depth, uint32_t* triid,
                     int32_t nthreads, cro_stats* stats);
 
+/* Passthrough vertex shader (test/shader/PassThrough.cu:16-35): the clip
+ * position of model position p (Vec3f) is posToClip * (p, 1) with
+ * Mat4f * Vec4f of base/Math.hpp:990-1001 -- per row i, rr = 0, then
+ * rr += m(i, j) * v[j] for j = 0..3, each step one fused multiply-add (nvcc's
+ * default contraction).  posToClip is in Mat4f memory order (column-major:
+ * m(r, c) = m[r + 4c]).  Clip positions are written as 4 floats at the
+ * start of each outStrideFloats-float output record; the rest of the record
+ * is left as it is. */
+void     cro_vertex_passthrough(const float* modelPos, int64_t numVertices, const float posToClip[16],
+                                float* out, int64_t outStrideFloats);
+
 #ifdef __cplusplus
 }
 #endif

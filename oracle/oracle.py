@@ -108,6 +108,8 @@ def lib():
         L.cro_render.argtypes = [ctypes.POINTER(Pipe), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                  ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_int32, ctypes.POINTER(Stats)]
+        L.cro_vertex_passthrough.restype = None
+        L.cro_vertex_passthrough.argtypes = [f4, ctypes.c_int64, f4, f4, ctypes.c_int64]
         _lib = L
     return _lib
 
@@ -234,3 +236,15 @@ def render(scene, color=None, depth=None, triid=True, nthreads=None, deferred_cl
     if rc != 0:
         raise ValueError("oracle rejected the draw arguments")
     return {"color": color, "depth": depth, "triid": tid, "stats": st.as_dict()}
+
+
+def vertex_passthrough(model_pos, pos_to_clip, out=None, stride_floats=4):
+    """Clip positions of model positions [V, 3] under the 4x4 matrix
+    pos_to_clip (row i, column j as a numpy array): records of stride_floats
+    floats, clip position first (out's other floats are kept when given)."""
+    p = np.ascontiguousarray(model_pos, dtype=np.float32).reshape(-1, 3)
+    m = np.ascontiguousarray(np.asarray(pos_to_clip, dtype=np.float32).T).reshape(16)   # Mat4f order
+    n = p.shape[0]
+    o = np.zeros((n, stride_floats), np.float32) if out is None else np.array(out, dtype=np.float32).reshape(n, stride_floats)
+    lib().cro_vertex_passthrough(_fptr(p), n, _fptr(m), _fptr(o), stride_floats)
+    return o
