@@ -239,6 +239,36 @@ def differentiable_views(reps=3):
             "api": "DiffRasterizer.render_views (one autograd call for the 32 views)"}
 
 
+def vertex_shader(num_vertices=3_000_000, reps=20):
+    """The passthrough vertex shader over the 1M-triangle workload's vertex
+    count (3 vertices per triangle): model positions (12 B) in, the clip
+    position part of 32-byte vertex records (16 B) out, timed with HIP events
+    on the stream it is launched on.  HBM-bound: 28 algorithmic bytes per
+    vertex."""
+    import numpy as np
+    import torch
+    import cudaraster
+    rng = np.random.default_rng(21)
+    p = torch.from_numpy((rng.standard_normal((num_vertices, 3)) * 20.0).astype(np.float32)).cuda()
+    out = torch.zeros((num_vertices, 8), dtype=torch.float32, device="cuda")
+    m = rng.standard_normal((4, 4)).astype(np.float32)
+    cr = cudaraster.CudaRaster().init()
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        cr.vertexShaderPassthrough(p, m, out, out_stride=32, stream=stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        cr.vertexShaderPassthrough(p, m, out, out_stride=32, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    cr.close()
+    us = e0.elapsed_time(e1) / reps * 1e3
+    bytes_ = 28 * num_vertices
+    return {"vertices": num_vertices, "us": us, "algorithmic_bytes": bytes_,
+            "GBps": bytes_ / (us * 1e-6) / 1e9, "peak_GBps": 8000.0}
+
+
 def cpu_baseline(scene, runs=5):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
@@ -272,7 +302,7 @@ def main():
                     help="skip the batched/plain multi-view ratio (64 views)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB copy kernel (reference for HBM counter scaling)")
-    ap.add_argument("--only", choices=("multi-view", "diff"),
+    ap.add_argument("--only", choices=("multi-view", "diff", "vertex"),
                     help="run only that leg and print its JSON record (for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -289,6 +319,9 @@ def main():
         return
     if args.only == "diff":
         print(json.dumps({"differentiable": differentiable_views()}), flush=True)
+        return
+    if args.only == "vertex":
+        print(json.dumps({"vertex_shader": vertex_shader()}), flush=True)
         return
     import cudaraster
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -374,6 +407,7 @@ def main():
         line["multi_view"] = mv
     if not args.no_diff:
         line["differentiable"] = differentiable_views()
+    line["vertex_shader"] = vertex_shader()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene)
     print(json.dumps(line), flush=True)
