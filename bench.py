@@ -269,6 +269,32 @@ def vertex_shader(num_vertices=3_000_000, reps=20):
             "GBps": bytes_ / (us * 1e-6) / 1e9, "peak_GBps": 8000.0}
 
 
+def resolve_leg(reps=20):
+    """Box MSAA resolve of a 2048x2048 4x RGBA8 surface (the 1M-triangle
+    workload's colour surface) into 2048x2048 pixels, HIP events on the
+    stream it runs on.  HBM-bound: 16 bytes read and 4 written per pixel."""
+    import numpy as np
+    import torch
+    import cudaraster
+    surf = cudaraster.CudaSurface((2048, 2048), cudaraster.FORMAT_RGBA8, 4)
+    surf.upload(np.random.default_rng(22).integers(0, 2 ** 32, size=surf.shape, dtype=np.uint64).astype(np.uint32))
+    out = torch.empty((2048, 2048), dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        surf.resolve(out, stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        surf.resolve(out, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    surf.close()
+    us = e0.elapsed_time(e1) / reps * 1e3
+    bytes_ = 20 * 2048 * 2048
+    return {"pixels": 2048 * 2048, "samples": 4, "us": us, "algorithmic_bytes": bytes_,
+            "GBps": bytes_ / (us * 1e-6) / 1e9, "peak_GBps": 8000.0}
+
+
 def cpu_baseline(scene, runs=5):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
@@ -302,7 +328,7 @@ def main():
                     help="skip the batched/plain multi-view ratio (64 views)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB copy kernel (reference for HBM counter scaling)")
-    ap.add_argument("--only", choices=("multi-view", "diff", "vertex"),
+    ap.add_argument("--only", choices=("multi-view", "diff", "vertex", "resolve"),
                     help="run only that leg and print its JSON record (for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -322,6 +348,9 @@ def main():
         return
     if args.only == "vertex":
         print(json.dumps({"vertex_shader": vertex_shader()}), flush=True)
+        return
+    if args.only == "resolve":
+        print(json.dumps({"resolve": resolve_leg()}), flush=True)
         return
     import cudaraster
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -408,6 +437,7 @@ def main():
     if not args.no_diff:
         line["differentiable"] = differentiable_views()
     line["vertex_shader"] = vertex_shader()
+    line["resolve"] = resolve_leg()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene)
     print(json.dumps(line), flush=True)

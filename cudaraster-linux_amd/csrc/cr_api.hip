@@ -266,7 +266,7 @@ cr_status drawImpl(cr_context* ctx, hipStream_t stream, bool sync) {
 
 extern "C" {
 
-int32_t cr_abi_version(void) { return (1 << 16) | 4; }
+int32_t cr_abi_version(void) { return (1 << 16) | 5; }
 
 const char* cr_last_error(const cr_context* ctx) { return ctx ? ctx->err.c_str() : g_lastError.c_str(); }
 
@@ -585,6 +585,17 @@ cr_status cr_interpolate_backward(cr_context* ctx, const int32_t* triImage, int3
   InterpGeom G{width, height, width, numChannels, numTris, numVertices};
   launchInterpBackward(G, triImage, pos, attr, tris, gradOut, gradPos, gradAttr, static_cast<hipStream_t>(stream));
   CR_HIP(ctx, hipGetLastError());
+  return CR_OK;
+}
+
+cr_status cr_surface_resolve(const cr_surface* color, void* out, void* stream) {
+  if (!color || !out) return fail(nullptr, CR_ERR_INVALID, "cr_surface_resolve: NULL argument");
+  if (color->format != CR_FORMAT_RGBA8 || ((uintptr_t)out & 3))
+    return fail(nullptr, CR_ERR_INVALID, "cr_surface_resolve: not an RGBA8 surface, or a misaligned output");
+  CR_HIP(nullptr, hipSetDevice(color->device));
+  launchResolve(color->dev, color->textureW, color->width, color->height, color->samplesLog2,
+                static_cast<uint32_t*>(out), static_cast<hipStream_t>(stream));
+  CR_HIP(nullptr, hipGetLastError());
   return CR_OK;
 }
 

@@ -250,6 +250,9 @@ struct InterpGeom {
   int32_t numTris;
   int64_t numVertices;
 };
+// Box MSAA resolve of an RGBA8 surface into width x height pixels.
+void launchResolve(const uint32_t* tex, int32_t textureW, int32_t width, int32_t height, int32_t samplesLog2,
+                   uint32_t* out, hipStream_t stream);
 // Passthrough vertex shader: clip position posToClip * (p, 1) of each
 // model position into the first 16 bytes of each outStride-byte record.
 void launchVertexPassthrough(const float* modelPos, int64_t numVertices, const float posToClip[16], void* out,

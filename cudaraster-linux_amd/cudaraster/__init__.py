@@ -40,6 +40,7 @@ EXPORTS = [
     "cr_batch_create", "cr_batch_destroy", "cr_batch_draw", "cr_batch_autotune", "cr_batch_reserve",
     "cr_batch_use_graph", "cr_batch_get_info", "cr_batch_get_view_counters",
     "cr_interpolate", "cr_interpolate_backward", "cr_batch_configure", "cr_vertex_shader_passthrough",
+    "cr_surface_resolve",
 ]
 
 
@@ -151,6 +152,7 @@ def load(path: str | None = None):
         "cr_batch_configure": [vp, vp],
         "cr_interpolate_backward": [vp, vp, i32, i32, vp, i64, vp, i32, vp, i32, vp, vp, vp, vp],
         "cr_vertex_shader_passthrough": [vp, vp, i64, ctypes.POINTER(ctypes.c_float), vp, i32, vp],
+        "cr_surface_resolve": [vp, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -228,6 +230,17 @@ class CudaSurface:
     def download(self) -> np.ndarray:
         out = np.empty(self.shape, dtype=np.uint32)
         _check(load().cr_surface_download(self._h, out.ctypes.data, out.nbytes))
+        return out
+
+    def resolve(self, out=None, stream=None):
+        """Box MSAA resolve on the GPU (cr_surface_resolve): returns a device
+        int32 tensor [height, width] of RGBA8 words (row 0 at the bottom)."""
+        import torch
+        w, h = self.width, self.height
+        if out is None:
+            out = torch.empty((h, w), dtype=torch.int32, device="cuda")
+        s = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
+        _check(load().cr_surface_resolve(self._h, _ptr(out), s))
         return out
 
     def upload(self, arr):
@@ -525,3 +538,23 @@ class BatchRenderer:
             for s in trio:
                 if s is not None:
                     s.close()
+
+
+def rgba8_to_image(words) -> np.ndarray:
+    """RGBA8 words [H, W] (the surfaces' ABGR packing, row 0 at the bottom)
+    -> uint8 [H, W, 4] RGBA with row 0 at the top (image order)."""
+    a = np.ascontiguousarray(np.asarray(words).astype(np.uint32))
+    return a.view(np.uint8).reshape(a.shape[0], a.shape[1], 4)[::-1].copy()
+
+
+def save_png(path, words):
+    """Writes RGBA8 words [H, W] as an RGBA PNG (PIL)."""
+    from PIL import Image
+    Image.fromarray(rgba8_to_image(words), "RGBA").save(path)
+
+
+def load_png(path) -> np.ndarray:
+    """Reads an RGBA PNG back as RGBA8 words [H, W], row 0 at the bottom."""
+    from PIL import Image
+    img = np.ascontiguousarray(np.asarray(Image.open(path).convert("RGBA"))[::-1])
+    return img.view(np.uint32).reshape(img.shape[0], img.shape[1])

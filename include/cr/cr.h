@@ -319,6 +319,14 @@ cr_status cr_vertex_shader_passthrough(cr_context* ctx, const float* model_pos, 
                                        const float* pos_to_clip, void* out_vertices, int32_t out_stride,
                                        void* stream);
 
+/* MSAA resolve, the step after drawTriangles (the reference declares one
+ * but leaves it commented out, CudaSurface.hpp:73): writes width x height
+ * RGBA8 words (row-major, row 0 at the bottom like the surface) to the
+ * device buffer out_rgba8, each channel the mean of the pixel's samples
+ * rounded half up, (sum + N/2) / N; a single-sample surface is copied.
+ * `color` must be an RGBA8 surface.  Asynchronous on `stream`. */
+cr_status cr_surface_resolve(const cr_surface* color, void* out_rgba8, void* stream);
+
 /* Version of the ABI (major << 16 | minor). */
 int32_t cr_abi_version(void);
 

@@ -248,3 +248,21 @@ def vertex_passthrough(model_pos, pos_to_clip, out=None, stride_floats=4):
     o = np.zeros((n, stride_floats), np.float32) if out is None else np.array(out, dtype=np.float32).reshape(n, stride_floats)
     lib().cro_vertex_passthrough(_fptr(p), n, _fptr(m), _fptr(o), stride_floats)
     return o
+
+
+def resolve(color, width, height, samples_log2):
+    """Box MSAA resolve of an RGBA8 surface (texture layout of
+    CudaSurface.cpp:86-91: per 8-pixel tile column, sample 0 of its 8
+    pixels, then sample 1, ...).  The reference's own resolve is commented
+    out (CudaSurface.hpp:73); per channel this is the mean of the samples
+    rounded half up, (sum + N/2) >> log2(N).  Returns uint32 [height, width]."""
+    n = 1 << samples_log2
+    c = np.asarray(color, dtype=np.uint32)
+    rw = c.shape[1] // n                                   # rounded width
+    t = c[:height].reshape(height, rw // 8, n, 8)          # [y, tile column, sample, x in tile]
+    t = t.transpose(0, 1, 3, 2).reshape(height, rw, n)[:, :width]   # [y, x, sample]
+    out = np.zeros((height, width), np.uint32)
+    for ch in range(4):
+        v = ((t >> np.uint32(8 * ch)) & np.uint32(0xFF)).astype(np.uint32).sum(axis=2)
+        out |= ((v + np.uint32(n >> 1)) >> np.uint32(samples_log2)) << np.uint32(8 * ch)
+    return out

@@ -1,0 +1,47 @@
+"""Writes the golden images under tests/golden/ from the CPU oracle (test
+fixtures: expected outputs of the drop-in path).  Store format: RGBA PNG,
+upright (the surface's row 0, the bottom row, is the file's last row),
+written and read by cudaraster.save_png / load_png.
+
+    python tests/golden/make_golden.py
+
+cube_256x192_4x_resolved.png: the demo cube (seed 0, demo camera) at
+256x192, 4x MSAA, clip positions from the passthrough vertex shader
+(float32 Mat4f * Vec4f), drawn by the oracle and box-resolved."""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+for _p in (os.path.join(REPO, "oracle"), os.path.join(REPO, "cudaraster-linux_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+
+def cube_scene(width=256, height=192, samples_log2=2):
+    import dataclasses
+    from cudaraster import scenes
+    import oracle
+    m = scenes.demo_camera(width, height).astype(np.float32)
+    base = scenes.cube(width, height, samples_log2=samples_log2)
+    verts = base.vertices.copy()
+    verts[:, :4] = oracle.vertex_passthrough(scenes.CUBE_POSITIONS.astype(np.float32), m)
+    return dataclasses.replace(base, vertices=np.ascontiguousarray(verts)), m
+
+
+def cube_golden():
+    import oracle
+    scene, _ = cube_scene()
+    r = oracle.render(scene)
+    return oracle.resolve(r["color"], scene.width, scene.height, scene.samples_log2)
+
+
+def main():
+    import cudaraster
+    cudaraster.save_png(os.path.join(HERE, "cube_256x192_4x_resolved.png"), cube_golden())
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,63 @@
+"""MSAA resolve on the GPU (cr_surface_resolve) against the oracle, and the
+whole demo frame -- model positions, vertex shader, 4x draw, resolve --
+against the golden image."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.mark.parametrize("w,h,sl2", [(64, 64, 0), (203, 77, 1), (256, 192, 2), (2048, 2048, 2), (129, 33, 3)])
+def test_resolve_matches_oracle(oracle_lib, w, h, sl2):
+    _torch()
+    import cudaraster
+    surf = cudaraster.CudaSurface((w, h), cudaraster.FORMAT_RGBA8, 1 << sl2)
+    try:
+        c = np.random.default_rng(w + sl2).integers(0, 2 ** 32, size=surf.shape, dtype=np.uint64).astype(np.uint32)
+        surf.upload(c)
+        got = surf.resolve().cpu().numpy().view(np.uint32)
+    finally:
+        surf.close()
+    assert np.array_equal(got, oracle_lib.resolve(c, w, h, sl2))
+
+
+def test_resolve_rejects_non_colour_surfaces():
+    _torch()
+    import cudaraster
+    surf = cudaraster.CudaSurface((16, 16), cudaraster.FORMAT_DEPTH32, 4)
+    try:
+        with pytest.raises(cudaraster.CrError):
+            surf.resolve()
+    finally:
+        surf.close()
+
+
+def test_cube_frame_matches_golden():
+    torch = _torch()
+    import sys
+    import cudaraster
+    from cudaraster import scenes
+    sys.path.insert(0, GOLDEN)
+    import make_golden
+    scene, m = make_golden.cube_scene()
+    base = scenes.cube(scene.width, scene.height, samples_log2=scene.samples_log2)
+    r = cudaraster.SceneRenderer(base)              # colours in place; positions from the vertex shader
+    try:
+        dp = torch.from_numpy(scenes.CUBE_POSITIONS.astype(np.float32)).cuda()
+        r.cr.vertexShaderPassthrough(dp, m, r.vbuf, out_stride=32)
+        r.draw()
+        got = r.color.resolve().cpu().numpy().view(np.uint32)
+    finally:
+        r.close()
+    assert np.array_equal(got, cudaraster.load_png(os.path.join(GOLDEN, "cube_256x192_4x_resolved.png")))
