@@ -73,6 +73,11 @@ class CudaSurface {
   void upload(const std::vector<uint32_t>& texels) {
     check(cr_surface_upload(s_, texels.data(), texels.size() * 4), nullptr);
   }
+  // Box MSAA resolve into width x height RGBA8 words at the device address
+  // devOut (the reference's commented-out resolve, CudaSurface.hpp:73).
+  void resolveToDevice(void* devOut, void* hipStream = nullptr) const {
+    check(cr_surface_resolve(s_, devOut, hipStream), nullptr);
+  }
   cr_surface* handle() const { return s_; }
 
  private:
@@ -126,6 +131,17 @@ class CudaRaster {
   }
 
   void drawTriangles(void* hipStream = nullptr) { check(cr_draw_triangles(ctx_, hipStream), ctx_); }
+
+  // The demo's passthrough vertex shader (test/shader/PassThrough.cu:16-35):
+  // clip positions posToClip * (modelPos, 1) into the first 16 bytes of each
+  // outStride-byte record of the device vertex buffer devOut.  posToClip is
+  // a Mat4f's getPtr() (column-major, host memory).
+  void vertexShaderPassthrough(const void* devModelPos, int64_t numVertices, const float* posToClip, void* devOut,
+                               int outStride, void* hipStream = nullptr) {
+    check(cr_vertex_shader_passthrough(ctx_, static_cast<const float*>(devModelPos), numVertices, posToClip, devOut,
+                                       outStride, hipStream),
+          ctx_);
+  }
 
   Stats getStats() {
     cr_stats s;

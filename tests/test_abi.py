@@ -87,7 +87,10 @@ int run(const void* vb, const void* ib, int nv, int nt) {
   raster.deferredClear(clear, 1.0f);
   raster.setVertexBuffer(vb, 0, nv);
   raster.setIndexBuffer(ib, 0, nt);
+  const float mvp[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  raster.vertexShaderPassthrough(nullptr, 0, mvp, nullptr, 32);
   raster.drawTriangles();
+  color.resolveToDevice(nullptr);
   return (int)raster.getStats().fineTime + (int)color.download().size();
 }
 int main(int argc, char**) { return argc > 5 ? run(nullptr, nullptr, 0, 0) : 0; }
