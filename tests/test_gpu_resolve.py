@@ -61,3 +61,20 @@ def test_cube_frame_matches_golden():
     finally:
         r.close()
     assert np.array_equal(got, cudaraster.load_png(os.path.join(GOLDEN, "cube_256x192_4x_resolved.png")))
+
+
+def test_resolve_into_an_unaligned_buffer(oracle_lib):
+    """Output rows that are not 16-byte aligned take the per-pixel stores."""
+    torch = _torch()
+    import cudaraster
+    surf = cudaraster.CudaSurface((64, 8), cudaraster.FORMAT_RGBA8, 4)
+    try:
+        c = np.random.default_rng(5).integers(0, 2 ** 32, size=surf.shape, dtype=np.uint64).astype(np.uint32)
+        surf.upload(c)
+        buf = torch.zeros(64 * 8 + 1, dtype=torch.int32, device="cuda")
+        surf.resolve(buf[1:].view(8, 64))
+        got = buf.cpu().numpy().view(np.uint32)
+    finally:
+        surf.close()
+    assert got[0] == 0
+    assert np.array_equal(got[1:].reshape(8, 64), oracle_lib.resolve(c, 64, 8, 2))
