@@ -517,10 +517,9 @@ __global__ void __launch_bounds__(64 * kFineWaves, (fineMinWaves<S, VIS>())) fin
 // draw); otherwise groups are dealt round-robin over the XCDs in list order,
 // which spreads the first-listed (heaviest) groups over all of them.
 template <int S, bool VIS, int W, bool DEPTH>
-__global__ void __launch_bounds__(64 * W, (fineMinWaves<S, VIS>())) fineBatchKernel(const Params* __restrict__ views,
-                                                          const int32_t* __restrict__ groupStart,
-                                                          const int32_t* __restrict__ order, int numViews,
-                                                          int totalGroups, int banded) {
+__global__ void __launch_bounds__(64 * W, (fineMinWaves<S, VIS>()))
+    fineBatchKernel(const Params* __restrict__ views, const int32_t* __restrict__ groupStart,
+                    const int32_t* __restrict__ order, int numViews, int totalGroups, int banded) {
   __shared__ __attribute__((aligned(16))) uint32_t recAll[W * recWords<S, VIS>()];
   const int g = banded ? xcdGroup(totalGroups) : (int)blockIdx.x;
   if (g >= totalGroups) return;
