@@ -7,7 +7,10 @@ written and read by cudaraster.save_png / load_png.
 
 cube_256x192_4x_resolved.png: the demo cube (seed 0, demo camera) at
 256x192, 4x MSAA, clip positions from the passthrough vertex shader
-(float32 Mat4f * Vec4f), drawn by the oracle and box-resolved."""
+(float32 Mat4f * Vec4f), drawn by the oracle and box-resolved.
+random_96x64_{1,2,4,8}x_resolved.png: 1500 seeded random triangles (areas
+0.5-64 px^2, depth test, Gouraud with perspective-correct interpolation) at
+each sample count, drawn by the oracle and box-resolved."""
 import os
 import sys
 
@@ -38,9 +41,34 @@ def cube_golden():
     return oracle.resolve(r["color"], scene.width, scene.height, scene.samples_log2)
 
 
+# Small random scenes (the 1M-triangle workload's distribution) at every
+# sample count: name -> (triangles, width, height, samples_log2, seed).
+RANDOM_GOLDENS = {
+    "random_96x64_1x_resolved.png": (1500, 96, 64, 0, 31),
+    "random_96x64_2x_resolved.png": (1500, 96, 64, 1, 32),
+    "random_96x64_4x_resolved.png": (1500, 96, 64, 2, 33),
+    "random_96x64_8x_resolved.png": (1500, 96, 64, 3, 34),
+}
+
+
+def random_scene(name):
+    from cudaraster import scenes
+    n, w, h, sl2, seed = RANDOM_GOLDENS[name]
+    return scenes.random_tris(n, w, h, samples_log2=sl2, seed=seed, area_range=(0.5, 64.0))
+
+
+def random_golden(name):
+    import oracle
+    scene = random_scene(name)
+    r = oracle.render(scene)
+    return oracle.resolve(r["color"], scene.width, scene.height, scene.samples_log2)
+
+
 def main():
     import cudaraster
     cudaraster.save_png(os.path.join(HERE, "cube_256x192_4x_resolved.png"), cube_golden())
+    for name in RANDOM_GOLDENS:
+        cudaraster.save_png(os.path.join(HERE, name), random_golden(name))
 
 
 if __name__ == "__main__":

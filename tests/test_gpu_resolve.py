@@ -78,3 +78,20 @@ def test_resolve_into_an_unaligned_buffer(oracle_lib):
         surf.close()
     assert got[0] == 0
     assert np.array_equal(got[1:].reshape(8, 64), oracle_lib.resolve(c, 64, 8, 2))
+
+
+@pytest.mark.parametrize("name", ["random_96x64_1x_resolved.png", "random_96x64_2x_resolved.png",
+                                  "random_96x64_4x_resolved.png", "random_96x64_8x_resolved.png"])
+def test_random_frames_match_goldens(name):
+    _torch()
+    import sys
+    import cudaraster
+    sys.path.insert(0, GOLDEN)
+    import make_golden
+    r = cudaraster.SceneRenderer(make_golden.random_scene(name))
+    try:
+        r.draw()
+        got = r.color.resolve().cpu().numpy().view(np.uint32)
+    finally:
+        r.close()
+    assert np.array_equal(got, cudaraster.load_png(os.path.join(GOLDEN, name)))

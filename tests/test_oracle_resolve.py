@@ -75,3 +75,13 @@ def test_oracle_reproduces_golden_cube():
     import make_golden
     got = make_golden.cube_golden()
     assert np.array_equal(got, cudaraster.load_png(os.path.join(GOLDEN, "cube_256x192_4x_resolved.png")))
+
+
+@pytest.mark.parametrize("name", ["random_96x64_1x_resolved.png", "random_96x64_2x_resolved.png",
+                                  "random_96x64_4x_resolved.png", "random_96x64_8x_resolved.png"])
+def test_oracle_reproduces_random_goldens(name):
+    import sys
+    import cudaraster
+    sys.path.insert(0, GOLDEN)
+    import make_golden
+    assert np.array_equal(make_golden.random_golden(name), cudaraster.load_png(os.path.join(GOLDEN, name)))
