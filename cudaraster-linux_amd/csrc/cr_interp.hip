@@ -15,7 +15,8 @@
 // (determinant 1) moves q to (0, 0, 1) and leaves
 // c_k = x'_{k+1} y'_{k+2} - y'_{k+1} x'_{k+2}.  All of it is evaluated in
 // FP64 (cheap on CDNA4 and well conditioned for sub-pixel triangles); the
-// gradients accumulate with FP64 atomics and are rounded to FP32 once.
+// backward sums each triangle's per-pixel gradient terms in FP64 and adds
+// the sums to the FP32 outputs with atomics.
 #include <algorithm>
 
 #include "cr_device.h"
