@@ -67,20 +67,27 @@ __device__ __forceinline__ bool pixelTriangle(const InterpGeom& G, const uint32_
   return true;
 }
 
+// One lane per pixel, one grid row per image row.  vec4: C % 4 == 0 and
+// attr / out 16-byte aligned, so channels move as float4.
 __global__ void __launch_bounds__(256) interpForwardKernel(const InterpGeom G, const uint32_t* __restrict__ ids,
                                                            const float4* __restrict__ pos,
                                                            const float* __restrict__ attr,
                                                            const int3* __restrict__ tris, float* __restrict__ out,
-                                                           int32_t* __restrict__ triImage) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (int64_t)G.width * G.height) return;
-  const int x = (int)(p % G.width), y = (int)(p / G.width);
+                                                           int32_t* __restrict__ triImage, int vec4) {
+  const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int y = (int)blockIdx.y;
+  if (x >= G.width) return;
+  const int64_t p = (int64_t)y * G.width + x;
   const int C = G.numChannels;
   float* o = out + p * C;
   uint32_t t;
   int32_t i[3];
   if (!pixelTriangle(G, ids, tris, x, y, t, i)) {
-    for (int c = 0; c < C; c++) o[c] = 0.0f;
+    if (vec4) {
+      for (int c = 0; c < C; c += 4) *reinterpret_cast<float4*>(o + c) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+      for (int c = 0; c < C; c++) o[c] = 0.0f;
+    }
     if (triImage) triImage[p] = -1;
     return;
   }
@@ -90,7 +97,18 @@ __global__ void __launch_bounds__(256) interpForwardKernel(const InterpGeom G, c
   const float* a0 = attr + (int64_t)i[0] * C;
   const float* a1 = attr + (int64_t)i[1] * C;
   const float* a2 = attr + (int64_t)i[2] * C;
-  for (int c = 0; c < C; c++) o[c] = (float)(B.b[0] * a0[c] + B.b[1] * a1[c] + B.b[2] * a2[c]);
+  if (vec4) {
+    for (int c = 0; c < C; c += 4) {
+      const float4 u = *reinterpret_cast<const float4*>(a0 + c), v = *reinterpret_cast<const float4*>(a1 + c),
+                   w = *reinterpret_cast<const float4*>(a2 + c);
+      *reinterpret_cast<float4*>(o + c) = make_float4((float)(B.b[0] * u.x + B.b[1] * v.x + B.b[2] * w.x),
+                                                      (float)(B.b[0] * u.y + B.b[1] * v.y + B.b[2] * w.y),
+                                                      (float)(B.b[0] * u.z + B.b[1] * v.z + B.b[2] * w.z),
+                                                      (float)(B.b[0] * u.w + B.b[1] * v.w + B.b[2] * w.w));
+    }
+  } else {
+    for (int c = 0; c < C; c++) o[c] = (float)(B.b[0] * a0[c] + B.b[1] * a1[c] + B.b[2] * a2[c]);
+  }
   if (triImage) triImage[p] = (int32_t)t;
 }
 
@@ -278,10 +296,11 @@ static int gridFor(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 1
 
 void launchInterpForward(const InterpGeom& G, const uint32_t* ids, const float* pos, const float* attr,
                          const int32_t* tris, float* out, int32_t* triImage, hipStream_t stream) {
-  const int64_t n = (int64_t)G.width * G.height;
-  if (n == 0) return;
-  hipLaunchKernelGGL(interpForwardKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, G, ids,
-                     reinterpret_cast<const float4*>(pos), attr, reinterpret_cast<const int3*>(tris), out, triImage);
+  if (G.width <= 0 || G.height <= 0) return;
+  const int vec4 = G.numChannels % 4 == 0 && ((uintptr_t)attr & 15) == 0 && ((uintptr_t)out & 15) == 0;
+  hipLaunchKernelGGL(interpForwardKernel, dim3((unsigned)((G.width + 255) / 256), (unsigned)G.height), dim3(256), 0,
+                     stream, G, ids, reinterpret_cast<const float4*>(pos), attr, reinterpret_cast<const int3*>(tris),
+                     out, triImage, vec4);
 }
 
 void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const float* pos, const float* attr,
