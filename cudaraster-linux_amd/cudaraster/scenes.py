@@ -255,6 +255,74 @@ def clip_mix(num_tris, width=256, height=256, samples_log2=2, seed=3) -> Scene:
                  SHADER_GOURAUD, verts, idx)
 
 
+def snap_ties(num_tris=300, width=64, height=64, samples_log2=0, seed=12):
+    """Triangles whose every vertex lies exactly half-way between two
+    subpixel positions: p = k + 1/2 subpixels with integer k of both signs
+    and parities, and w in {0.5, 1, 2} so that x * (1/w) * (W * 8) is exact.
+    The snap's rounding rule (cvt.rni: nearest, ties to even,
+    TriangleSetup.inl:19-31) alone decides every snapped coordinate.
+    Returns (scene, k) with k int64 [T, 3, 2] the integer parts."""
+    rng = np.random.default_rng(seed)
+    sx, sy = width * 8, height * 8                  # subpixels from the centre to an edge
+    ks = np.empty((num_tris, 3, 2), np.int64)
+    clip = np.empty((num_tris, 3, 4), np.float64)
+    for t in range(num_tris):
+        while True:
+            a = rng.integers(-sx + 8, sx - 8, size=3)
+            b = rng.integers(-sy + 8, sy - 8, size=3)
+            area = (a[1] - a[0]) * (b[2] - b[0]) - (b[1] - b[0]) * (a[2] - a[0])
+            if abs(area) > 256:
+                break
+        if area < 0:                                 # counter-clockwise = front-facing
+            a[[1, 2]] = a[[2, 1]]
+            b[[1, 2]] = b[[2, 1]]
+        w = float(rng.choice([0.5, 1.0, 2.0]))
+        ks[t, :, 0], ks[t, :, 1] = a, b
+        clip[t, :, 0] = (a + 0.5) / sx * w
+        clip[t, :, 1] = (b + 0.5) / sy * w
+        clip[t, :, 2] = (rng.random(3) - 0.5) * w
+        clip[t, :, 3] = w
+    col = rng.random((num_tris, 3, 4))
+    verts = np.concatenate([clip, col], axis=-1).reshape(num_tris * 3, 8).astype(np.float32)
+    assert np.array_equal(verts.reshape(num_tris, 3, 8)[..., :2].astype(np.float64), clip[..., :2])
+    idx = np.arange(num_tris * 3, dtype=np.int32).reshape(num_tris, 3)
+    return (Scene("snap_ties", width, height, samples_log2, FLAG_DEPTH | FLAG_LERP, BLEND_REPLACE,
+                  SHADER_GOURAUD, np.ascontiguousarray(verts), idx), ks)
+
+
+# Snapped boxes (x0, x1, y0, y1) in subpixels relative to the viewport centre
+# at 2048^2 (one NDC unit = 16384 subpixels), each drawn as the triangle
+# (x0,y0) (x1,y0) (x0,y1), and whether the fast path takes it: the snapped
+# box must lie in [-32768, 32767] with max(x1, y1) - min(x0, y0) <= 2^15 - 1
+# (TriangleSetup.inl:285-296).  The extent mixes the axes.
+GUARD_BAND_CASES = [
+    ((-16384, 16383, -100, 100), True),     # extent 32767
+    ((-16384, 16384, -100, 100), False),    # extent 32768
+    ((-16384, 0, 0, 16383), True),          # mixed axes: 16383 - (-16384)
+    ((-16384, 0, 0, 16384), False),         # mixed axes: 32768, each axis 16384
+    ((-32768, -1, -200, -1), True),         # corner at the lower limit
+    ((-32769, -2, -200, -2), False),        # one subpixel beyond it
+    ((1, 32767, 1, 200), True),             # corner at the upper limit
+    ((1, 32768, 1, 200), False),            # one subpixel beyond it
+]
+
+
+def guard_band(width=2048, height=2048, samples_log2=0) -> Scene:
+    """One triangle per GUARD_BAND_CASES entry, w = 1 (snapped coordinates
+    are the given integers exactly), front-facing, inside the depth range."""
+    n = len(GUARD_BAND_CASES)
+    clip = np.zeros((n, 3, 4), np.float64)
+    for t, ((x0, x1, y0, y1), _) in enumerate(GUARD_BAND_CASES):
+        pts = [(x0, y0), (x1, y0), (x0, y1)]
+        for k, (x, y) in enumerate(pts):
+            clip[t, k] = (x / (width * 8), y / (height * 8), 0.25 + 0.05 * t, 1.0)
+    col = np.random.default_rng(5).random((n, 3, 4))
+    verts = np.concatenate([clip, col], axis=-1).reshape(n * 3, 8).astype(np.float32)
+    idx = np.arange(n * 3, dtype=np.int32).reshape(n, 3)
+    return Scene("guard_band", width, height, samples_log2, FLAG_DEPTH | FLAG_LERP, BLEND_REPLACE,
+                 SHADER_GOURAUD, np.ascontiguousarray(verts), idx)
+
+
 def config(i: int, scale: float = 1.0) -> Scene:
     """The plain-method workloads, optionally scaled down (triangle counts and
     resolution) for oracle-sized parity runs."""

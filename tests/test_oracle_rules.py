@@ -499,6 +499,55 @@ def test_setup_culls(oracle_lib):
     assert code == 5 and len(h) >= 1
 
 
+def test_snap_rounds_ties_to_even(oracle_lib):
+    """R4: cvt.rni.sat.s32 (TriangleSetup.inl:19-31, Util.hpp:88) rounds a
+    subpixel position that lies exactly half-way to the even neighbour.
+    Every vertex here is such a tie (k + 1/2, both signs and parities); the
+    expected header coordinates come from Python's round(), which is
+    round-half-even on exact binary fractions.  The host emulation's
+    floor(x + 0.5) (CudaRaster.cpp:701-703) and round-half-away both
+    disagree on half of these vertices."""
+    from cudaraster import scenes
+    for s in (0, 2):
+        sc, ks = scenes.snap_ties(samples_log2=s, seed=12 + s)
+        seen = {"even+": 0, "odd+": 0, "even-": 0, "odd-": 0}
+        visible = 0
+        for t in range(sc.num_tris):
+            h, _, code = oracle_lib.setup_triangle(sc, t)
+            if code != 0:
+                continue
+            visible += 1
+            got = [(h[0].v0x, h[0].v0y), (h[0].v1x, h[0].v1y), (h[0].v2x, h[0].v2y)]
+            for k in range(3):
+                for axis in range(2):
+                    kk = int(ks[t, k, axis])
+                    want = round(Fraction(2 * kk + 1, 2))      # exact tie, half to even
+                    assert want == (kk if kk % 2 == 0 else kk + 1)
+                    assert got[k][axis] == want, (t, k, axis, kk)
+                    seen[("even" if kk % 2 == 0 else "odd") + ("+" if kk >= 0 else "-")] += 1
+        assert visible > 200 and min(seen.values()) > 100, seen
+
+
+def test_guard_band_extent_decides_fast_path(oracle_lib):
+    """R3: the fast path needs the snapped box inside [-32768, 32767] and
+    max(hi.x, hi.y) - min(lo.x, lo.y) <= 2^15 - 1 (TriangleSetup.inl:285-296);
+    anything else goes through the clip path (cull code 5) even when no
+    vertex is outside the frustum.  Cases sit exactly on and one subpixel
+    past each limit, including an extent that exceeds the limit only when
+    the axes are mixed."""
+    from cudaraster import scenes
+    sc = scenes.guard_band()
+    for t, (box, fast) in enumerate(scenes.GUARD_BAND_CASES):
+        h, _, code = oracle_lib.setup_triangle(sc, t)
+        assert code == (0 if fast else 5), (box, code)
+        assert len(h) >= 1, box
+        if fast:   # the snapped vertices are the box corners themselves
+            x0, x1, y0, y1 = box
+            assert (h[0].v0x, h[0].v0y, h[0].v1x, h[0].v1y, h[0].v2x, h[0].v2y) == (x0, y0, x1, y0, x0, y1)
+    st = oracle_lib.render(sc)["stats"]
+    assert st["clipped"] == sum(1 for _, fast in scenes.GUARD_BAND_CASES if not fast)
+
+
 # ---------------------------------------------------------------------------
 # Whole draw: an independent pure-Python sequential rasterizer
 # ---------------------------------------------------------------------------
