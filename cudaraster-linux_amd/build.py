@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libcr.so")
-SOURCES = ["cr_setup.hip", "cr_binning.hip", "cr_fine.hip", "cr_api.hip", "cr_batch.hip", "cr_interp.hip", "cr_vertex.hip", "cr_resolve.hip"]
+SOURCES = ["cr_setup.hip", "cr_binning.hip", "cr_fine.hip", "cr_api.hip", "cr_batch.hip", "cr_interp.hip", "cr_vertex.hip", "cr_resolve.hip", "cr_noalloc.hip"]
 ARCH = os.environ.get("CR_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: the only fused multiply-adds are explicit fmaf() calls.

@@ -99,7 +99,7 @@ cr_status validatePipe(cr_context* ctx, const cr_pipe_desc* p) {
 
 using namespace crhost;
 
-namespace {
+namespace crhost {
 
 cr_status validateDraw(cr_context* ctx) {
   if (!ctx->color) return fail(ctx, CR_ERR_STATE, "CudaRaster: Surfaces not set!");
@@ -113,6 +113,10 @@ cr_status validateDraw(cr_context* ctx) {
     return fail(ctx, CR_ERR_STATE, "triangle-id surface does not match the colour surface");
   return CR_OK;
 }
+
+}  // namespace crhost
+
+namespace {
 
 cr_status ensureBuffers(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   int64_t nt = ctx->numTris;
@@ -134,7 +138,33 @@ cr_status ensureBuffers(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   return CR_OK;
 }
 
-Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks) {
+}  // namespace
+
+namespace crhost {
+
+DrawBuffers contextBuffers(cr_context* ctx) {
+  DrawBuffers B;
+  B.maxSubtris = ctx->maxSubtris;
+  B.maxBinEntries = ctx->maxBinEntries;
+  B.maxTileEntries = ctx->maxTileEntries;
+  B.triSubtris = ctx->triSubtris.as<uint8_t>();
+  B.triHeader = ctx->triHeader.as<Header>();
+  B.triData = ctx->triData.as<TriData>();
+  B.subtriTri = ctx->subtriTri.as<uint32_t>();
+  B.chunkBase = ctx->chunkBase.as<uint32_t>();
+  B.binCnt = ctx->binCnt.as<uint32_t>();
+  B.binOff = ctx->binOff.as<uint32_t>();
+  B.binList = ctx->binList.as<uint4>();
+  B.tileStart = ctx->tileStart.as<uint32_t>();
+  B.tileCount = ctx->tileCount.as<uint32_t>();
+  B.tileList = ctx->tileList.as<uint32_t>();
+  B.tileFrags = ctx->dbg.count_fragments ? ctx->tileFrags.as<uint32_t>() : nullptr;
+  B.chunkSubtris = ctx->dbg.count_fragments ? ctx->chunkSubtris.as<uint32_t>() : nullptr;
+  B.counters = ctx->counters.as<Counters>();
+  return B;
+}
+
+Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks, const DrawBuffers& B) {
   Params P;
   memset(&P, 0, sizeof(P));
   P.numTris = ctx->numTris;
@@ -163,27 +193,27 @@ Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   P.clearDepth = ctx->clearDepth;
   P.hiz = (ctx->dbg.disable_hiz || ctx->dbg.count_fragments) ? 0 : 1;
   P.coarseThreads = ctx->dbg.coarse_threads;
-  P.tileFrags = ctx->dbg.count_fragments ? ctx->tileFrags.as<uint32_t>() : nullptr;
-  P.chunkSubtris = ctx->dbg.count_fragments ? ctx->chunkSubtris.as<uint32_t>() : nullptr;
-  P.maxSubtris = (int32_t)ctx->maxSubtris;
-  P.triSubtris = ctx->triSubtris.as<uint8_t>();
-  P.triHeader = ctx->triHeader.as<Header>();
-  P.triData = ctx->triData.as<TriData>();
-  P.subtriTri = ctx->subtriTri.as<uint32_t>();
+  P.tileFrags = ctx->dbg.count_fragments ? B.tileFrags : nullptr;
+  P.chunkSubtris = ctx->dbg.count_fragments ? B.chunkSubtris : nullptr;
+  P.maxSubtris = (int32_t)B.maxSubtris;
+  P.triSubtris = B.triSubtris;
+  P.triHeader = B.triHeader;
+  P.triData = B.triData;
+  P.subtriTri = B.subtriTri;
   P.numChunks = numChunks;
-  P.chunkBase = ctx->chunkBase.as<uint32_t>();
-  P.binCnt = ctx->binCnt.as<uint32_t>();
-  P.binOff = ctx->binOff.as<uint32_t>();
-  P.maxBinEntries = (int32_t)ctx->maxBinEntries;
-  P.binList = ctx->binList.as<uint4>();
-  P.tileStart = ctx->tileStart.as<uint32_t>();
-  P.tileCount = ctx->tileCount.as<uint32_t>();
-  P.maxTileEntries = (int32_t)ctx->maxTileEntries;
-  P.tileList = ctx->tileList.as<uint32_t>();
+  P.chunkBase = B.chunkBase;
+  P.binCnt = B.binCnt;
+  P.binOff = B.binOff;
+  P.maxBinEntries = (int32_t)B.maxBinEntries;
+  P.binList = B.binList;
+  P.tileStart = B.tileStart;
+  P.tileCount = B.tileCount;
+  P.maxTileEntries = (int32_t)B.maxTileEntries;
+  P.tileList = B.tileList;
   P.color = ctx->color->dev;
   P.depth = ctx->depth->dev;
   P.triid = ctx->triid ? ctx->triid->dev : nullptr;
-  P.counters = ctx->counters.as<Counters>();
+  P.counters = B.counters;
   return P;
 }
 
@@ -198,13 +228,13 @@ cr_status launchAll(cr_context* ctx, const Params& P, hipStream_t stream) {
   CR_HIP(ctx, hipEventRecord(ctx->ev[4], stream));
   launchFragmentSum(P, stream);
   CR_HIP(ctx, hipGetLastError());
-  CR_HIP(ctx, hipMemcpyAsync(ctx->hostCounters, ctx->counters.ptr, sizeof(Counters), hipMemcpyDeviceToHost, stream));
+  CR_HIP(ctx, hipMemcpyAsync(ctx->hostCounters, P.counters, sizeof(Counters), hipMemcpyDeviceToHost, stream));
   ctx->evRecorded = true;
   return CR_OK;
 }
 
 
-cr_status finishDraw(cr_context* ctx, bool* overflow) {
+uint32_t recordCounters(cr_context* ctx) {
   const Counters& c = *ctx->hostCounters;
   ctx->last.num_tris = ctx->numTris;
   ctx->last.num_subtris = c.numSubtris;
@@ -212,7 +242,16 @@ cr_status finishDraw(cr_context* ctx, bool* overflow) {
   ctx->last.tile_entries = c.tileEntries;
   ctx->last.fragments = (int64_t)c.fragments;
   ctx->last.visible_subtris = (int64_t)c.visibleSubtris;
-  *overflow = c.overflow != 0;
+  return c.overflow;
+}
+
+}  // namespace crhost
+
+namespace {
+
+cr_status finishDraw(cr_context* ctx, bool* overflow) {
+  const Counters& c = *ctx->hostCounters;
+  *overflow = recordCounters(ctx) != 0;
   if (!*overflow) return CR_OK;
   if (c.overflow & 1u) ctx->maxSubtris = std::max<int64_t>(ctx->maxSubtris, (int64_t)c.numSubtris + kSubtrisSlack);
   if (c.overflow & 2u) ctx->maxBinEntries = std::max<int64_t>(ctx->maxBinEntries, (int64_t)c.binEntries + kBinSlack);
@@ -239,7 +278,7 @@ cr_status drawImpl(cr_context* ctx, hipStream_t stream, bool sync) {
   for (;;) {
     st = ensureBuffers(ctx, g, numChunks);
     if (st != CR_OK) return st;
-    Params P = makeParams(ctx, g, numChunks);
+    Params P = makeParams(ctx, g, numChunks, contextBuffers(ctx));
     st = launchAll(ctx, P, stream);
     if (st != CR_OK) return st;
     if (!sync) {
@@ -279,6 +318,9 @@ cr_status cr_context_create(cr_context** out) {
   cr_context* ctx = new (std::nothrow) cr_context();
   if (!ctx) return fail(nullptr, CR_ERR_NOMEM, "out of host memory");
   hipError_t e = hipGetDevice(&ctx->device);
+  // The no-allocation draw's scratch array on this device (looked up once;
+  // a failed lookup is reported by that entry point, not here).
+  if (e == hipSuccess) scratchInit(ctx->device);
   if (e == hipSuccess) e = ctx->counters.ensure(sizeof(Counters));
   if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hostCounters, sizeof(Counters), hipHostMallocDefault);
   for (int i = 0; i < 5 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);

@@ -100,6 +100,41 @@ void clearValues(const float rgba[4], float depth, uint32_t* color, uint32_t* en
 // setPixelPipe's checks of a pipe descriptor (CudaRaster.cpp:183-216).
 cr_status validatePipe(cr_context* ctx, const cr_pipe_desc* p);
 
+// Where one draw's intermediates live and how many entries they hold.
+struct DrawBuffers {
+  int64_t maxSubtris = 0, maxBinEntries = 0, maxTileEntries = 0;
+  uint8_t* triSubtris = nullptr;
+  crhip::Header* triHeader = nullptr;
+  crhip::TriData* triData = nullptr;
+  uint32_t* subtriTri = nullptr;
+  uint32_t* chunkBase = nullptr;
+  uint32_t* binCnt = nullptr;
+  uint32_t* binOff = nullptr;
+  uint4* binList = nullptr;
+  uint32_t* tileStart = nullptr;
+  uint32_t* tileCount = nullptr;
+  uint32_t* tileList = nullptr;
+  uint32_t* tileFrags = nullptr;      // counting draws only
+  uint32_t* chunkSubtris = nullptr;   // counting draws only
+  crhip::Counters* counters = nullptr;
+};
+
+
+// drawTriangles' state checks (CudaRaster.cpp:243-260).
+cr_status validateDraw(cr_context* ctx);
+// The context's own (growable) intermediate buffers.
+DrawBuffers contextBuffers(cr_context* ctx);
+// Kernel parameters of the context's current state over buffers B.
+crhip::Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks, const DrawBuffers& B);
+// Reset, setup+bin, coarse and fine launches with the stage events, then the
+// counter block's copy into ctx->hostCounters, all on `stream`.
+cr_status launchAll(cr_context* ctx, const crhip::Params& P, hipStream_t stream);
+// Copies the read-back counters into ctx->last; returns their overflow bits.
+uint32_t recordCounters(cr_context* ctx);
+// Looks up this device's copy of the no-allocation draw's fixed scratch
+// array (once per device; the device must be current).
+void scratchInit(int device);
+
 }  // namespace crhost
 
 #define CR_HIP(ctx, call)                                                                      \

@@ -40,7 +40,7 @@ EXPORTS = [
     "cr_batch_create", "cr_batch_destroy", "cr_batch_draw", "cr_batch_autotune", "cr_batch_reserve",
     "cr_batch_use_graph", "cr_batch_get_info", "cr_batch_get_view_counters",
     "cr_interpolate", "cr_interpolate_backward", "cr_batch_configure", "cr_vertex_shader_passthrough",
-    "cr_surface_resolve",
+    "cr_surface_resolve", "cr_draw_triangles_noalloc",
 ]
 
 
@@ -153,6 +153,7 @@ def load(path: str | None = None):
         "cr_interpolate_backward": [vp, vp, i32, i32, vp, i64, vp, i32, vp, i32, vp, vp, vp, vp],
         "cr_vertex_shader_passthrough": [vp, vp, i64, ctypes.POINTER(ctypes.c_float), vp, i32, vp],
         "cr_surface_resolve": [vp, vp, vp],
+        "cr_draw_triangles_noalloc": [vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -332,6 +333,12 @@ class CudaRaster:
     def drawTriangles(self, stream=None):
         s = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
         _check(load().cr_draw_triangles(self.ctx, s), self.ctx)
+
+    def drawTrianglesNoAlloc(self, stream=None):
+        """drawTriangles with the intermediates in the library's fixed
+        scratch array (cr_draw_triangles_noalloc): no device allocation."""
+        s = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
+        _check(load().cr_draw_triangles_noalloc(self.ctx, s), self.ctx)
 
     def drawTrianglesAsync(self, stream=None):
         s = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)

@@ -131,6 +131,8 @@ class CudaRaster {
   }
 
   void drawTriangles(void* hipStream = nullptr) { check(cr_draw_triangles(ctx_, hipStream), ctx_); }
+  // drawTriangles without device allocation (cr_draw_triangles_noalloc).
+  void drawTrianglesNoAlloc(void* hipStream = nullptr) { check(cr_draw_triangles_noalloc(ctx_, hipStream), ctx_); }
 
   // The demo's passthrough vertex shader (test/shader/PassThrough.cu:16-35):
   // clip positions posToClip * (modelPos, 1) into the first 16 bytes of each

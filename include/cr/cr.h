@@ -184,6 +184,21 @@ cr_status cr_draw_check(cr_context* ctx);
 cr_status cr_reserve(cr_context* ctx, int64_t max_subtris, int64_t max_bin_entries,
                      int64_t max_tile_entries);
 
+/* drawTriangles that allocates nothing.  No counterpart in the reference:
+ * for callers that cannot call hipMalloc (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  Same state, same
+ * synchronous semantics and the same images as cr_draw_triangles, but the
+ * draw's intermediate buffers live in one fixed device array of
+ * CR_NOALLOC_SCRATCH_BYTES bytes that every GPU the library runs on holds
+ * until the process ends (looked up per device by cr_context_create).
+ * A draw whose intermediates do not fit is rejected with CR_ERR_INVALID
+ * (its per-draw tables alone are too large) or CR_ERR_OVERFLOW (its queues
+ * are), leaving the surfaces as cr_draw_triangles leaves them after an
+ * overflowed attempt.  All calls on one GPU share the array: calls that
+ * overlap on one GPU (from several threads or on several streams) race. */
+#define CR_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+cr_status cr_draw_triangles_noalloc(cr_context* ctx, void* stream);
+
 /* CudaRaster::getStats (CudaRaster.cpp:346-363). */
 cr_status cr_get_stats(cr_context* ctx, cr_stats* out);
 cr_status cr_get_counters(cr_context* ctx, cr_counters* out);
