@@ -165,6 +165,21 @@ def multi_view_ratio(reps=5, triid=True):
             cr.drawTriangles()
 
     plain_pass()
+    # GPU time of the plain draws: per view, the stage events of its draw
+    # (reset, setup+bin, coarse, fine) summed over the 64 views.
+    plain_gpu_ms = 0.0
+    for k, sc in enumerate(views):
+        col, dep, tid = plain_surf[k]
+        vb, ib = br.buffers[k]
+        cr.setSurfaces(col, dep)
+        cr.setTriangleIdSurface(tid)
+        cr.setPixelPipe(pipes[k])
+        cr.deferredClear(sc.clear_color, sc.clear_depth)
+        cr.setVertexBuffer(vb, 0, sc.vertices.shape[0])
+        cr.setIndexBuffer(ib, 0, sc.num_tris)
+        cr.drawTriangles()
+        st = cr.getStats()
+        plain_gpu_ms += (st["setupTime"] + st["binTime"] + st["coarseTime"] + st["fineTime"]) * 1e3
     br.batch.autotune()
     br.draw()
     identical = True
@@ -179,7 +194,30 @@ def multi_view_ratio(reps=5, triid=True):
     for _ in range(reps):
         plain_pass()
     torch.cuda.synchronize()
-    t_plain = (time.perf_counter() - t0) / reps
+    t_plain_py = (time.perf_counter() - t0) / reps
+    # The same sequential draws from a native loop over the C ABI (no
+    # interpreter between the calls): the plain leg of the ratio.
+    import ctypes
+    bench_lib = ctypes.CDLL(os.path.join(REPO, "cudaraster-linux_amd", "lib", "libcrbench.so"))
+    bench_lib.crbench_plain_draws.restype = ctypes.c_double
+    bench_lib.crbench_plain_draws.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    cviews = (cudaraster.View * len(views))()
+    for k, sc in enumerate(views):
+        col, dep, tid = plain_surf[k]
+        vb, ib = br.buffers[k]
+        v = cviews[k]
+        v.color, v.depth, v.tri_id = col.handle.value, dep.handle.value, (tid.handle.value if tid else None)
+        v.vertices, v.num_vertices = vb.data_ptr(), sc.vertices.shape[0]
+        v.indices, v.num_tris = ib.data_ptr(), sc.num_tris
+        v.deferred_clear = 1 if sc.deferred_clear else 0
+        v.pipe = pipes[k]
+        v.clear_color[:] = [float(x) for x in sc.clear_color]
+        v.clear_depth = float(sc.clear_depth)
+    st, bad = ctypes.c_int32(), ctypes.c_int32()
+    t_plain = bench_lib.crbench_plain_draws(cr.ctx, cviews, len(views), reps, ctypes.byref(st), ctypes.byref(bad))
+    if t_plain < 0:
+        raise RuntimeError(f"native plain draws failed at view {bad.value}: status {st.value}")
     gpu_ms = []
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -191,7 +229,10 @@ def multi_view_ratio(reps=5, triid=True):
     out = {"views": len(views), "triangles": int(sum(s.num_tris for s in views)),
            "plain_views_per_s": len(views) / t_plain, "batched_views_per_s": len(views) / t_batch,
            "ratio": t_plain / t_batch, "plain_ms_per_pass": t_plain * 1e3, "batched_ms_per_pass": t_batch * 1e3,
-           "batched_gpu_ms": statistics.median(gpu_ms), "identical_images": identical,
+           "plain_loop": "native C loop over the C ABI (cudaraster-linux_amd/bench_native/plain_draws.cpp)",
+           "plain_ms_per_pass_python": t_plain_py * 1e3, "ratio_python_plain_loop": t_plain_py / t_batch,
+           "plain_gpu_ms": plain_gpu_ms, "batched_gpu_ms": statistics.median(gpu_ms),
+           "gpu_only_ratio": plain_gpu_ms / statistics.median(gpu_ms), "identical_images": identical,
            "fine_tiles_per_group": info["fine_tiles_per_group"], "uses_graph": bool(info["uses_graph"]),
            "reserved_bytes": info["reserved_bytes"]}
     br.close()
@@ -295,11 +336,31 @@ def resolve_leg(reps=20):
             "GBps": bytes_ / (us * 1e-6) / 1e9, "peak_GBps": 8000.0}
 
 
-def cpu_baseline(scene, runs=5):
+def host_cpu():
+    """The host this process runs on: hardware threads, the threads it may
+    use, and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"nproc": os.cpu_count(), "affinity": allowed, "model": model}
+
+
+def cpu_baseline(scene, runs=5, single_runs=3, max_threads=16):
+    """The CPU oracle (plain C + pthreads, band-parallel) drawing the same
+    frame on the host cores: all the threads this process may use, capped at
+    the one-GPU host share of 16, median of `runs` after a warm-up; and one
+    thread, median of `single_runs`."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    threads = max(1, min(threads, 16))
+    host = host_cpu()
+    threads = max(1, min(host["affinity"], max_threads))
     oracle.render(scene, nthreads=threads)   # warm-up
     times = []
     for _ in range(runs):
@@ -307,9 +368,70 @@ def cpu_baseline(scene, runs=5):
         oracle.render(scene, nthreads=threads)
         times.append(time.perf_counter() - t0)
     t = statistics.median(times)
-    return {"value": scene.num_tris / t / 1e6, "unit": "Mtris/s", "cores": threads, "kind": "port",
-            "sample": f"whole {scene.num_tris}-triangle frame, median of {runs} runs after 1 warm-up",
-            "seconds_per_frame": t}
+    t1 = []
+    for _ in range(single_runs):
+        t0 = time.perf_counter()
+        oracle.render(scene, nthreads=1)
+        t1.append(time.perf_counter() - t0)
+    out = {"value": scene.num_tris / t / 1e6, "unit": "Mtris/s", "cores": threads, "kind": "port",
+           "sample": f"whole {scene.num_tris}-triangle frame, median of {runs} runs after 1 warm-up "
+                     f"({threads} threads)" + (f" and of {single_runs} runs on 1 thread" if single_runs else ""),
+           "seconds_per_frame": t, "host": host}
+    if t1:
+        t_single = statistics.median(t1)
+        out["single_thread"] = {"value": scene.num_tris / t_single / 1e6, "seconds_per_frame": t_single}
+    return out
+
+
+def plain_workload_leg(name, steps=10, warmup=2, triid=True, cpu=True):
+    """One plain-draw workload beside the headline one (the skewed 4M +
+    1024-triangle 8x src-over frame): Mtris/s and Gfragments/s over `steps`
+    synchronous draws, per-stage times and per-kernel algorithmic GB/s, and
+    the CPU oracle on one frame."""
+    import torch
+    import cudaraster
+    scene, desc = workload(name)
+    r = cudaraster.SceneRenderer(scene, triangle_ids=triid)
+    r.cr.setDebugParams(count_fragments=True)
+    r.draw()
+    cnt = r.cr.getCounters()
+    fragments, visible = cnt["fragments"], cnt["visible_subtris"]
+    r.cr.setDebugParams()
+    for _ in range(warmup):
+        r.draw()
+    torch.cuda.synchronize()
+    stage = {"setupTime": 0.0, "binTime": 0.0, "coarseTime": 0.0, "fineTime": 0.0}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.draw()
+        st = r.cr.getStats()
+        for k in stage:
+            stage[k] += st[k]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    counters = r.cr.getCounters()
+    r.close()
+    avg = {k: v / steps for k, v in stage.items()}
+    kernels = {}
+    for key, kname, b in (("setupTime", "setupBinKernel", setup_algorithmic_bytes(scene, counters, visible)),
+                          ("coarseTime", "coarseKernel", coarse_algorithmic_bytes(scene, counters, visible)),
+                          ("fineTime", "fineKernel", fine_algorithmic_bytes(scene, visible, triid))):
+        kernels[kname] = {"ms": avg[key] * 1e3, "algorithmic_bytes": b,
+                          "GBps": b / avg[key] / 1e9 if avg[key] > 0 else None}
+        kernels[kname]["frac"] = kernels[kname]["GBps"] / HBM_PEAK_GBS if kernels[kname]["GBps"] else None
+    ms = elapsed / steps * 1e3
+    out = {"workload": desc, "triangles": scene.num_tris, "steps": steps, "ms_per_step": ms,
+           "mtris_per_s": scene.num_tris * steps / elapsed / 1e6,
+           "gfragments_per_s": fragments * steps / elapsed / 1e9, "fragments_per_frame": fragments,
+           "stage_ms": {k: v * 1e3 for k, v in avg.items()}, "kernels": kernels,
+           "frame_algorithmic_GBps": frame_algorithmic_bytes(scene, triid) / (ms * 1e-3) / 1e9,
+           "counters": counters}
+    out["frame_roofline_frac"] = out["frame_algorithmic_GBps"] / HBM_PEAK_GBS
+    if cpu:
+        # One multi-threaded frame (a 1-thread frame of this workload would
+        # take minutes of host time).
+        out["cpu_baseline"] = cpu_baseline(scene, runs=1, single_runs=0)
+    return out
 
 
 def main():
@@ -326,6 +448,8 @@ def main():
                     help="skip the differentiable workload (32 views, forward + backward)")
     ap.add_argument("--no-multi-view", action="store_true",
                     help="skip the batched/plain multi-view ratio (64 views)")
+    ap.add_argument("--no-c2", action="store_true",
+                    help="skip the 4M + 1024-triangle 8x src-over leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB copy kernel (reference for HBM counter scaling)")
     ap.add_argument("--only", choices=("multi-view", "diff", "vertex", "resolve"),
@@ -429,7 +553,12 @@ def main():
         "frame_algorithmic_GBps": frame_algorithmic_bytes(scene, triid) / (ms * 1e-3) / 1e9,
         "counters": counters,
     }
+    # The whole frame against the HBM peak (compulsory bytes of the draw:
+    # inputs read once, surfaces written once), beside the per-kernel figure.
+    line["frame_roofline_frac"] = line["frame_algorithmic_GBps"] / HBM_PEAK_GBS
     r.close()
+    if args.workload == "c1" and not args.no_c2:
+        line["c2"] = plain_workload_leg("c2", triid=triid, cpu=not args.no_cpu_baseline)
     if not args.no_multi_view:
         mv = multi_view_ratio(triid=triid)
         line["batched_over_plain_views_ratio"] = mv["ratio"]

@@ -75,6 +75,15 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout.decode(errors="replace"))
     os.replace(tmp, LIB)
+    # The benchmark's native timing loop (bench.py), linked against libcr.so.
+    bench_src = os.path.join(HERE, "bench_native", "plain_draws.cpp")
+    bench_lib = os.path.join(LIB_DIR, "libcrbench.so")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{os.path.join(REPO, 'include')}", bench_src,
+           "-o", bench_lib + ".tmp", f"-L{LIB_DIR}", "-lcr", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("bench harness build failed:\n" + r.stdout.decode(errors="replace"))
+    os.replace(bench_lib + ".tmp", bench_lib)
     return LIB
 
 

@@ -82,14 +82,16 @@ cr_status validatePipe(cr_context* ctx, const cr_pipe_desc* p) {
   if (p->samples_log2 < 0 || p->samples_log2 > 3) return fail(ctx, CR_ERR_INVALID, "pipe: samples_log2 must be 0..3");
   if (p->vertex_struct_size < 16 || (p->vertex_struct_size & 15))
     return fail(ctx, CR_ERR_INVALID, "pipe: vertex_struct_size must be a positive multiple of 16 (Vec4f records)");
-  if (p->fragment_shader != CR_SHADER_GOURAUD && p->fragment_shader != CR_SHADER_CONSTANT)
+  if (p->fragment_shader != CR_SHADER_GOURAUD && p->fragment_shader != CR_SHADER_CONSTANT &&
+      p->fragment_shader != CR_SHADER_TEX_PHONG)
     return fail(ctx, CR_ERR_INVALID, "pipe: unknown fragment shader %d", p->fragment_shader);
   if (p->fragment_shader == CR_SHADER_GOURAUD && p->vertex_struct_size < 32)
     return fail(ctx, CR_ERR_INVALID, "pipe: GouraudShader needs a colour varying (vertex_struct_size >= 32)");
+  if (p->fragment_shader == CR_SHADER_TEX_PHONG && p->vertex_struct_size < 64)
+    return fail(ctx, CR_ERR_INVALID,
+                "pipe: texPhong needs camera position, normal and texture coordinate varyings (vertex_struct_size >= 64)");
   if (p->blend_shader < CR_BLEND_REPLACE || p->blend_shader > CR_BLEND_DEPTH_ONLY)
     return fail(ctx, CR_ERR_INVALID, "pipe: unknown blend shader %d", p->blend_shader);
-  if (p->render_mode_flags & CR_RENDER_QUADS)
-    return fail(ctx, CR_ERR_UNSUPPORTED, "pipe: RenderModeFlag_EnableQuads is not supported");
   if (p->render_mode_flags & ~(uint32_t)(CR_RENDER_DEPTH | CR_RENDER_LERP | CR_RENDER_QUADS))
     return fail(ctx, CR_ERR_INVALID, "pipe: unknown render mode flags 0x%x", p->render_mode_flags);
   return CR_OK;
@@ -101,6 +103,12 @@ using namespace crhost;
 
 namespace crhost {
 
+bool countingDraw(const cr_context* ctx) {
+  return ctx->dbg.count_fragments != 0 || ctx->dbg.profiling_mode == CR_PROFILING_COUNTERS;
+}
+
+bool profilingDraw(const cr_context* ctx) { return ctx->dbg.profiling_mode == CR_PROFILING_COUNTERS; }
+
 cr_status validateDraw(cr_context* ctx) {
   if (!ctx->color) return fail(ctx, CR_ERR_STATE, "CudaRaster: Surfaces not set!");
   if (!ctx->pipeSet) return fail(ctx, CR_ERR_STATE, "CudaRaster: Pixel pipe not set!");
@@ -111,6 +119,8 @@ cr_status validateDraw(cr_context* ctx) {
   if (ctx->triid && (ctx->triid->width != ctx->color->width || ctx->triid->height != ctx->color->height ||
                      ctx->triid->samples != ctx->color->samples))
     return fail(ctx, CR_ERR_STATE, "triangle-id surface does not match the colour surface");
+  if (ctx->pipe.fragment_shader == CR_SHADER_TEX_PHONG && !ctx->shaderConstantsSet)
+    return fail(ctx, CR_ERR_STATE, "texPhong: shader constants (materials, atlas) not set");
   return CR_OK;
 }
 
@@ -131,10 +141,11 @@ cr_status ensureBuffers(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   CR_HIP(ctx, ctx->tileStart.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileCount.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileList.ensure((size_t)ctx->maxTileEntries * 4));
-  if (ctx->dbg.count_fragments) {
+  if (countingDraw(ctx)) {
     CR_HIP(ctx, ctx->tileFrags.ensure((size_t)g.nt * 4));
     CR_HIP(ctx, ctx->chunkSubtris.ensure((size_t)std::max(numChunks, 1) * 4));
   }
+  if (profilingDraw(ctx)) CR_HIP(ctx, ctx->profBuf.ensure(2 * kProfCount * sizeof(unsigned long long)));
   return CR_OK;
 }
 
@@ -158,8 +169,9 @@ DrawBuffers contextBuffers(cr_context* ctx) {
   B.tileStart = ctx->tileStart.as<uint32_t>();
   B.tileCount = ctx->tileCount.as<uint32_t>();
   B.tileList = ctx->tileList.as<uint32_t>();
-  B.tileFrags = ctx->dbg.count_fragments ? ctx->tileFrags.as<uint32_t>() : nullptr;
-  B.chunkSubtris = ctx->dbg.count_fragments ? ctx->chunkSubtris.as<uint32_t>() : nullptr;
+  B.tileFrags = countingDraw(ctx) ? ctx->tileFrags.as<uint32_t>() : nullptr;
+  B.chunkSubtris = countingDraw(ctx) ? ctx->chunkSubtris.as<uint32_t>() : nullptr;
+  B.prof = profilingDraw(ctx) ? ctx->profBuf.as<unsigned long long>() : nullptr;
   B.counters = ctx->counters.as<Counters>();
   return B;
 }
@@ -184,17 +196,28 @@ Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks, const D
   P.heightBins = g.hb;
   P.numBins = g.nb;
   P.samplesLog2 = ctx->pipe.samples_log2;
-  P.flags = ctx->pipe.render_mode_flags & (kFlagDepth | kFlagLerp);
+  P.flags = ctx->pipe.render_mode_flags & (kFlagDepth | kFlagLerp | kFlagQuads);
   P.blend = ctx->pipe.blend_shader;
   P.shader = ctx->pipe.fragment_shader;
   P.constColor = ctx->pipe.const_color;
+  if (ctx->pipe.fragment_shader == CR_SHADER_TEX_PHONG) {
+    const cr_shader_constants& k = ctx->shaderConstants;
+    P.materials = reinterpret_cast<const Material*>(k.materials);
+    P.numMaterials = k.num_materials;
+    P.triMaterial = k.triangle_material_idx;
+    P.numTriMaterials = k.num_triangle_materials;
+    P.atlas = reinterpret_cast<const float4*>(k.atlas);
+    P.atlasWidth = k.atlas_width;
+    P.atlasHeight = k.atlas_height;
+  }
   P.deferredClear = ctx->deferredClear ? 1 : 0;
   P.clearColor = ctx->clearColor;
   P.clearDepth = ctx->clearDepth;
   P.hiz = (ctx->dbg.disable_hiz || ctx->dbg.count_fragments) ? 0 : 1;
   P.coarseThreads = ctx->dbg.coarse_threads;
-  P.tileFrags = ctx->dbg.count_fragments ? B.tileFrags : nullptr;
-  P.chunkSubtris = ctx->dbg.count_fragments ? B.chunkSubtris : nullptr;
+  P.tileFrags = countingDraw(ctx) ? B.tileFrags : nullptr;
+  P.chunkSubtris = countingDraw(ctx) ? B.chunkSubtris : nullptr;
+  P.prof = profilingDraw(ctx) ? B.prof : nullptr;
   P.maxSubtris = (int32_t)B.maxSubtris;
   P.triSubtris = B.triSubtris;
   P.triHeader = B.triHeader;
@@ -229,7 +252,11 @@ cr_status launchAll(cr_context* ctx, const Params& P, hipStream_t stream) {
   launchFragmentSum(P, stream);
   CR_HIP(ctx, hipGetLastError());
   CR_HIP(ctx, hipMemcpyAsync(ctx->hostCounters, P.counters, sizeof(Counters), hipMemcpyDeviceToHost, stream));
+  if (P.prof)
+    CR_HIP(ctx, hipMemcpyAsync(ctx->hostProf, P.prof, 2 * kProfCount * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, stream));
   ctx->evRecorded = true;
+  ctx->lastProfiled = P.prof != nullptr;
   return CR_OK;
 }
 
@@ -242,6 +269,13 @@ uint32_t recordCounters(cr_context* ctx) {
   ctx->last.tile_entries = c.tileEntries;
   ctx->last.fragments = (int64_t)c.fragments;
   ctx->last.visible_subtris = (int64_t)c.visibleSubtris;
+  memset(&ctx->lastProf, 0, sizeof(ctx->lastProf));
+  if (ctx->lastProfiled) {
+    for (int i = 0; i < kProfCount; i++) {
+      ctx->lastProf.num[i] = (int64_t)ctx->hostProf[2 * i];
+      ctx->lastProf.denom[i] = (int64_t)ctx->hostProf[2 * i + 1];
+    }
+  }
   return c.overflow;
 }
 
@@ -305,7 +339,7 @@ cr_status drawImpl(cr_context* ctx, hipStream_t stream, bool sync) {
 
 extern "C" {
 
-int32_t cr_abi_version(void) { return (1 << 16) | 5; }
+int32_t cr_abi_version(void) { return (1 << 16) | 6; }
 
 const char* cr_last_error(const cr_context* ctx) { return ctx ? ctx->err.c_str() : g_lastError.c_str(); }
 
@@ -323,6 +357,8 @@ cr_status cr_context_create(cr_context** out) {
   if (e == hipSuccess) scratchInit(ctx->device);
   if (e == hipSuccess) e = ctx->counters.ensure(sizeof(Counters));
   if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hostCounters, sizeof(Counters), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&ctx->hostProf, 2 * kProfCount * sizeof(unsigned long long), hipHostMallocDefault);
   for (int i = 0; i < 5 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->evPending, hipEventDisableTiming);
   if (e != hipSuccess) {
@@ -330,6 +366,7 @@ cr_status cr_context_create(cr_context** out) {
     return fail(nullptr, CR_ERR_DEVICE, "cr_context_create: %s", hipGetErrorString(e));
   }
   memset(ctx->hostCounters, 0, sizeof(Counters));
+  memset(ctx->hostProf, 0, 2 * kProfCount * sizeof(unsigned long long));
   *out = ctx;
   return CR_OK;
 }
@@ -340,9 +377,10 @@ cr_status cr_context_destroy(cr_context* ctx) {
   if (ctx->pending) (void)hipStreamSynchronize(ctx->pendingStream);
   for (DevBuf* b : {&ctx->triSubtris, &ctx->triHeader, &ctx->triData, &ctx->subtriTri, &ctx->chunkBase,
                     &ctx->binCnt, &ctx->binOff, &ctx->binList, &ctx->tileStart, &ctx->tileCount,
-                    &ctx->tileList, &ctx->tileFrags, &ctx->chunkSubtris, &ctx->counters})
+                    &ctx->tileList, &ctx->tileFrags, &ctx->chunkSubtris, &ctx->counters, &ctx->profBuf})
     b->release();
   if (ctx->hostCounters) (void)hipHostFree(ctx->hostCounters);
+  if (ctx->hostProf) (void)hipHostFree(ctx->hostProf);
   for (int i = 0; i < 5; i++)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
   if (ctx->evPending) (void)hipEventDestroy(ctx->evPending);
@@ -494,6 +532,9 @@ cr_status cr_pipe_lookup(const char* name, int32_t blend, int32_t samplesLog2, u
   } else if (strcmp(name, "PixelPipe_gouraud") == 0) {
     d.vertex_struct_size = 32;
     d.fragment_shader = CR_SHADER_GOURAUD;
+  } else if (strcmp(name, "PixelPipe_texPhong") == 0) {
+    d.vertex_struct_size = 64;                 // ShadedVertex_texPhong (Shaders.hpp:84-90)
+    d.fragment_shader = CR_SHADER_TEX_PHONG;
   } else {
     return fail(nullptr, CR_ERR_INVALID, "CudaRaster: Invalid pixel pipe!");
   }
@@ -581,19 +622,94 @@ cr_status cr_get_counters(cr_context* ctx, cr_counters* out) {
 
 cr_status cr_get_profiling_info(cr_context* ctx, char* buf, size_t len) {
   if (!ctx || !buf || len == 0) return fail(ctx, CR_ERR_INVALID, "cr_get_profiling_info: NULL argument");
-  cr_stats s;
-  cr_status st = cr_get_stats(ctx, &s);
-  if (st != CR_OK) return st;
-  const cr_counters& c = ctx->last;
-  double total = (double)s.setup_time + s.bin_time + s.coarse_time + s.fine_time;
-  snprintf(buf, len,
-           "TriangleSetup:  %8.3f ms\nBinRaster:      %8.3f ms\nCoarseRaster:   %8.3f ms\nFineRaster:     %8.3f ms\n"
-           "Total:          %8.3f ms\n\nTriangles       %lld\nSubtriangles    %lld\nBin entries     %lld\n"
-           "Tile entries    %lld\nFragments       %lld%s\nRetries         %lld\n",
-           s.setup_time * 1e3, s.bin_time * 1e3, s.coarse_time * 1e3, s.fine_time * 1e3, total * 1e3,
-           (long long)c.num_tris, (long long)c.num_subtris, (long long)c.bin_entries, (long long)c.tile_entries,
-           (long long)c.fragments, ctx->dbg.count_fragments ? "" : " (not counted: set count_fragments)",
-           (long long)c.retries);
+  std::string out("\n");
+  char line[256];
+  auto add = [&](const char* fmt, auto... v) {
+    snprintf(line, sizeof(line), fmt, v...);
+    out += line;
+  };
+  if (!ctx->pipeSet) out += "Pixel pipe not set!\n";
+  if (!ctx->lastProfiled) {
+    // ProfilingMode_Default (CudaRaster.cpp:386-422): stage times with their
+    // share of the draw, then the queue sizes.  Setup and binning are one
+    // kernel here, so binRaster reads 0.
+    cr_stats s;
+    cr_status st = cr_get_stats(ctx, &s);
+    if (st != CR_OK) return st;
+    const cr_counters& c = ctx->last;
+    const double total = (double)s.setup_time + s.bin_time + s.coarse_time + s.fine_time;
+    const double pct = total > 0.0 ? 100.0 / total : 0.0;
+    out += "ProfilingMode_Default\n---------------------\n\n";
+    add("%-16s%.3f ms (%.0f%%)\n", "triangleSetup", s.setup_time * 1e3, s.setup_time * pct);
+    add("%-16s%.3f ms (%.0f%%)\n", "binRaster", s.bin_time * 1e3, s.bin_time * pct);
+    add("%-16s%.3f ms (%.0f%%)\n", "coarseRaster", s.coarse_time * 1e3, s.coarse_time * pct);
+    add("%-16s%.3f ms (%.0f%%)\n", "fineRaster", s.fine_time * 1e3, s.fine_time * pct);
+    out += "\n";
+    const double mb = 1.0 / (1024.0 * 1024.0);
+    add("%-16s%-10lld(%.1f MB)\n", "numSubtris", (long long)c.num_subtris,
+        (double)c.num_subtris * (1 + sizeof(Header) + sizeof(TriData)) * mb);
+    add("%-16s%-10lld(%.1f MB)\n", "numBinEntries", (long long)c.bin_entries, (double)c.bin_entries * 16 * mb);
+    add("%-16s%-10lld(%.1f MB)\n", "numTileEntries", (long long)c.tile_entries, (double)c.tile_entries * 4 * mb);
+    if (countingDraw(ctx)) add("%-16s%lld\n", "fragments", (long long)c.fragments);
+    add("%-16s%lld\n", "retries", (long long)c.retries);
+  } else {
+    // ProfilingMode_Counters (CudaRaster.cpp:424-447): num / max(denom, 1)
+    // per counter, in the reference's layout.
+    cr_profile_counters pc;
+    cr_status st = cr_get_profile_counters(ctx, &pc);
+    if (st != CR_OK) return st;
+    auto v = [&](int i) { return (double)pc.num[i] / std::max((double)pc.denom[i], 1.0); };
+    out += "ProfilingMode_Counters\n----------------------\n\n";
+    out += "TriangleSetup:\n";
+    add("- Viewport cull        %.1f%%\n", v(CR_PROF_SETUP_VIEWPORT_CULL));
+    add("- Backface cull        %.1f%%\n", v(CR_PROF_SETUP_BACKFACE_CULL));
+    add("- Between pixels cull  %.1f%%\n", v(CR_PROF_SETUP_BETWEEN_PIXELS_CULL));
+    add("- Clipped              %.1f%%\n", v(CR_PROF_SETUP_CLIPPED));
+    add("- Avg. samples / tri   %.2f\n\n", v(CR_PROF_SETUP_SAMPLES_PER_TRI));
+    out += "BinRaster:\n";
+    add("- Queue overflows      %.0f\n", v(CR_PROF_BIN_QUEUE_OVERFLOWS));
+    add("- Avg. triangles/chunk %.1f\n", v(CR_PROF_BIN_TRIS_PER_CHUNK));
+    add("- Avg. tri bb size     %.1f\n", v(CR_PROF_BIN_TRI_BB_AREA));
+    add("- Single-bin boxes     %.1f%%\n\n", v(CR_PROF_BIN_TRI_SINGLE_BIN));
+    out += "CoarseRaster:\n";
+    add("- Bins                 %.0f\n", v(CR_PROF_COARSE_BINS));
+    add("- Rounds / Bin         %.1f\n", v(CR_PROF_COARSE_ROUNDS_PER_BIN));
+    add("- Triangles / Round    %.1f\n", v(CR_PROF_COARSE_TRIS_PER_ROUND));
+    add("- Tiles / Round        %.1f\n", v(CR_PROF_COARSE_TILES_PER_ROUND));
+    add("- Emits / Round        %.1f\n", v(CR_PROF_COARSE_EMITS_PER_ROUND));
+    add("- Emits / Triangle     %.2f\n", v(CR_PROF_COARSE_EMITS_PER_TRI));
+    add("- Case A               %.0f%%\n", v(CR_PROF_COARSE_CASE_A));
+    add("- Case B               %.0f%%\n", v(CR_PROF_COARSE_CASE_B));
+    add("- Case C               %.0f%%\n\n", v(CR_PROF_COARSE_CASE_C));
+    out += "FineRaster:\n";
+    out += "- Triangles culled\n";
+    add("  - Early Z kill       %.1f%%\n", v(CR_PROF_FINE_EARLY_Z_CULL));
+    add("  - Empty coverage     %.1f%%\n", v(CR_PROF_FINE_EMPTY_CULL));
+    add("- Z kills              %.1f%%\n", v(CR_PROF_FINE_Z_KILL));
+    add("- Post-kill warp util. %.1f%%\n", v(CR_PROF_FINE_WARP_UTIL));
+    add("- Blend attempt rounds %.2f\n", v(CR_PROF_FINE_BLEND_ROUNDS));
+    add("- Avg. tri/tile        %.0f\n", v(CR_PROF_FINE_TRI_PER_TILE));
+    add("- Avg. frag/tri        %.1f\n", v(CR_PROF_FINE_FRAG_PER_TRI));
+    add("- Avg. frag/tile       %.0f\n", v(CR_PROF_FINE_FRAG_PER_TILE));
+  }
+  out += "\n";
+  if (out.size() + 1 > len) {
+    buf[0] = 0;
+    return fail(ctx, CR_ERR_INVALID, "cr_get_profiling_info: the report needs %zu bytes", out.size() + 1);
+  }
+  memcpy(buf, out.c_str(), out.size() + 1);
+  return CR_OK;
+}
+
+cr_status cr_get_profile_counters(cr_context* ctx, cr_profile_counters* out) {
+  if (!ctx || !out) return fail(ctx, CR_ERR_INVALID, "cr_get_profile_counters: NULL argument");
+  static_assert((int)CR_PROF_COUNT == (int)kProfCount, "profile counter tables");
+  *out = ctx->lastProf;
+  if (ctx->lastProfiled) {
+    // Host-side counter: grow-and-retry rounds of the draw.
+    out->num[CR_PROF_BIN_QUEUE_OVERFLOWS] = ctx->last.retries;
+    out->denom[CR_PROF_BIN_QUEUE_OVERFLOWS] = 1;
+  }
   return CR_OK;
 }
 
@@ -609,6 +725,24 @@ cr_status cr_interpolate(cr_context* ctx, const cr_surface* triId, const float* 
   InterpGeom G{triId->width, triId->height, triId->textureW, numChannels, numTris, numVertices};
   CR_HIP(ctx, hipSetDevice(ctx->device));
   launchInterpForward(G, triId->dev, pos, attr, tris, out, triImage, static_cast<hipStream_t>(stream));
+  CR_HIP(ctx, hipGetLastError());
+  return CR_OK;
+}
+
+cr_status cr_interpolate_ids(cr_context* ctx, const int32_t* triImage, int32_t width, int32_t height,
+                             const float* pos, int64_t numVertices, const float* attr, int32_t numChannels,
+                             const int32_t* tris, int32_t numTris, float* out, void* stream) {
+  if (!ctx || !triImage || !out) return fail(ctx, CR_ERR_INVALID, "cr_interpolate_ids: NULL argument");
+  if (width <= 0 || height <= 0 || width > kMaxViewport || height > kMaxViewport || numChannels < 0 ||
+      numVertices < 0 || numTris < 0 || (numChannels > 0 && !attr) || (numTris > 0 && (!pos || !tris)) ||
+      ((uintptr_t)pos & 15) || ((uintptr_t)tris & 3) || ((uintptr_t)triImage & 3))
+    return fail(ctx, CR_ERR_INVALID, "cr_interpolate_ids: bad sizes or buffers");
+  // An int32 id image is a single-sample id surface with a pitch of `width`
+  // texels (-1 reads as 0xFFFFFFFF: no triangle).
+  InterpGeom G{width, height, width, numChannels, numTris, numVertices};
+  CR_HIP(ctx, hipSetDevice(ctx->device));
+  launchInterpForward(G, reinterpret_cast<const uint32_t*>(triImage), pos, attr, tris, out, nullptr,
+                      static_cast<hipStream_t>(stream));
   CR_HIP(ctx, hipGetLastError());
   return CR_OK;
 }
@@ -655,10 +789,33 @@ cr_status cr_vertex_shader_passthrough(cr_context* ctx, const float* modelPos, i
   return CR_OK;
 }
 
+cr_status cr_set_shader_constants(cr_context* ctx, const cr_shader_constants* c) {
+  if (!ctx) return fail(nullptr, CR_ERR_INVALID, "cr_set_shader_constants: context is NULL");
+  if (!c) {
+    ctx->shaderConstantsSet = false;
+    return CR_OK;
+  }
+  static_assert(sizeof(cr_material) == sizeof(Material) && sizeof(cr_texture_spec) == sizeof(TexSpec),
+                "material layout");
+  if (c->num_materials < 0 || (c->num_materials > 0 && !c->materials) || c->num_triangle_materials < 0 ||
+      (c->num_triangle_materials > 0 && !c->triangle_material_idx) || ((uintptr_t)c->materials & 15) ||
+      ((uintptr_t)c->triangle_material_idx & 3))
+    return fail(ctx, CR_ERR_INVALID, "cr_set_shader_constants: bad material table or index buffer");
+  if (!c->atlas || ((uintptr_t)c->atlas & 15) || c->atlas_width < 1 || c->atlas_height < 1 ||
+      c->atlas_width > 16384 || c->atlas_height > 16384)
+    return fail(ctx, CR_ERR_INVALID, "cr_set_shader_constants: the atlas must be a 16-byte aligned device "
+                                     "buffer of 1..16384 x 1..16384 float4 texels");
+  ctx->shaderConstants = *c;
+  ctx->shaderConstantsSet = true;
+  return CR_OK;
+}
+
 cr_status cr_set_debug_params(cr_context* ctx, const cr_debug_params* p) {
   if (!ctx || !p) return fail(ctx, CR_ERR_INVALID, "cr_set_debug_params: NULL argument");
   if (p->coarse_threads != 0 && p->coarse_threads != 256 && p->coarse_threads != 1024)
     return fail(ctx, CR_ERR_INVALID, "cr_set_debug_params: coarse_threads must be 0, 256 or 1024");
+  if (p->profiling_mode != CR_PROFILING_DEFAULT && p->profiling_mode != CR_PROFILING_COUNTERS)
+    return fail(ctx, CR_ERR_INVALID, "cr_set_debug_params: unknown profiling mode %d", p->profiling_mode);
   ctx->dbg = *p;
   return CR_OK;
 }

@@ -444,6 +444,8 @@ cr_status validateView(cr_context* ctx, const cr_view& v, int i) {
     return fail(ctx, CR_ERR_INVALID, "batch view %d: triangle-id surface does not match the colour surface", i);
   cr_status st = validatePipe(ctx, &v.pipe);
   if (st != CR_OK) return st;
+  if (v.pipe.fragment_shader == CR_SHADER_TEX_PHONG)
+    return fail(ctx, CR_ERR_UNSUPPORTED, "batch view %d: the texPhong shader draws through cr_draw_triangles", i);
   if (v.pipe.samples_log2 != v.color->samplesLog2)
     return fail(ctx, CR_ERR_INVALID, "batch view %d: mismatch in multisampling between pixel pipe and surface", i);
   if (v.num_tris < 0 || v.num_tris > kMaxTris || v.num_vertices < 0)

@@ -65,9 +65,13 @@ __device__ __forceinline__ uint32_t blockScanC(uint32_t v, uint32_t* sums, uint3
   return r;
 }
 
-__global__ void __launch_bounds__(64) resetCountersKernel(Counters* c, uint32_t numTris) {
+// Clears the counter block (and a profiling draw's profile counters) with
+// plain stores: graph-safe, no memset node.
+__global__ void __launch_bounds__(64) resetCountersKernel(Counters* c, uint32_t numTris, unsigned long long* prof) {
   uint32_t* w = reinterpret_cast<uint32_t*>(c);
   for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 4); i += 64) w[i] = 0;
+  if (prof != nullptr)
+    for (int i = threadIdx.x; i < 2 * kProfCount; i += 64) prof[i] = 0;
   __syncthreads();
   if (threadIdx.x == 0) c->numSubtris = numTris;
 }
@@ -187,14 +191,20 @@ __device__ __forceinline__ void forEachHitCoop(const TileTest& q, F&& f) {
   }
 }
 
+// How a wave walked its entries' tiles (the profiling counters' coarse
+// cases): A every box at most 2x2 tiles, one lane per box; B the union box
+// of the wave, one ballot per tile; C small boxes per lane and large boxes
+// by the whole wave.
+enum CoarseCase : int { kCaseA = 0, kCaseB = 1, kCaseC = 2 };
+
 // One round of up to 1024 bin entries: record the tiles each entry touches.
 // COUNT: add to cnt[tile] for every tile of the entry's box (an upper bound
 // of the exact count: list space only, the exact lists are built by the
 // emit pass); otherwise set bits in emitMask[wave][tile] for the tiles the
-// entry's edges reach.
+// entry's edges reach.  Returns the CoarseCase the wave took.
 template <bool COUNT>
-__device__ __forceinline__ void coarseRecord(const TileTest& r, bool valid, const int bl, uint32_t* cnt,
-                                             uint64_t (*emitMask)[kBinTiles]) {
+__device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const int bl, uint32_t* cnt,
+                                            uint64_t (*emitMask)[kBinTiles]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int w = valid ? r.x1 - r.x0 + 1 : 0, h = valid ? r.y1 - r.y0 + 1 : 0;
   const bool has = valid && w > 0 && h > 0;
@@ -232,7 +242,7 @@ __device__ __forceinline__ void coarseRecord(const TileTest& r, bool valid, cons
           else emitMask[wave][x + (y << bl)] = m;
         }
       }
-    return;
+    return kCaseB;
   }
   // Scattered boxes: small ones by their own lane, large ones by the wave.
   const bool big = has && w * h > kCoopArea;
@@ -253,6 +263,7 @@ __device__ __forceinline__ void coarseRecord(const TileTest& r, bool valid, cons
       else atomicOr((unsigned long long*)&emitMask[wave][x + (y << bl)], (unsigned long long)bj);
     });
   }
+  return allSmall ? kCaseA : kCaseC;
 }
 
 // Coarse raster of bin `bin` of one draw (one CT-lane workgroup).
@@ -271,8 +282,12 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   __shared__ int32_t segOf[kCoarseThreads];
   __shared__ int32_t sCarry;
   __shared__ uint32_t sBase;
+  // Profiling draws: entries, rounds, emits, tiles with emits, entries per case.
+  __shared__ uint32_t sProf[8];
 
   if (P.counters->overflow & 3u) return;
+  const bool prof = P.prof != nullptr;
+  if (prof && threadIdx.x < 8) sProf[threadIdx.x] = 0;
   const int bl = P.binLog2, tpb = 1 << (2 * bl);   // tiles per bin side log2, tiles per bin
   const int bx = bin % P.widthBins, by = bin / P.widthBins;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -367,16 +382,29 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
           coarseRecord<true>(r, valid, bl, cnt, emitMask);
           continue;
         }
-        coarseRecord<false>(r, valid, bl, cnt, emitMask);
+        const int wcase = coarseRecord<false>(r, valid, bl, cnt, emitMask);
+        if (prof) {
+          const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
+          if (lane == 0 && nv) {
+            atomicAdd(&sProf[0], nv);
+            atomicAdd(&sProf[5 + wcase], nv);
+          }
+          if (threadIdx.x == 0) sProf[1]++;
+        }
         __syncthreads();
         if (threadIdx.x < (unsigned)tpb) {
-          uint32_t acc = run[threadIdx.x];
+          const uint32_t before = run[threadIdx.x];
+          uint32_t acc = before;
 #pragma unroll
           for (int q = 0; q < kCoarseWaves; q++) {
             emitPre[q][threadIdx.x] = acc;
             acc += (uint32_t)__popcll(emitMask[q][threadIdx.x]);
           }
           run[threadIdx.x] = acc;
+          if (prof && acc != before) {
+            atomicAdd(&sProf[2], acc - before);
+            atomicAdd(&sProf[3], 1u);
+          }
         }
         __syncthreads();
         // Write the entries: entry `lane` goes to tile ti at its rank among
@@ -428,6 +456,19 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
     const int gx = (bx << bl) + x, gy = (by << bl) + y;
     if (gx < P.widthTiles && gy < P.heightTiles) P.tileCount[gx + gy * P.widthTiles] = run[threadIdx.x] - myStart;
   }
+  if (prof && threadIdx.x == 0) {
+    // (The emit pass ended with a barrier: every wave's additions are in.)
+    const uint32_t entries = sProf[0], rounds = sProf[1], emits = sProf[2];
+    profAdd(P.prof, kProfCoarseBins, entries ? 1 : 0, 0);
+    profAdd(P.prof, kProfCoarseRoundsPerBin, rounds, entries ? 1 : 0);
+    profAdd(P.prof, kProfCoarseTrisPerRound, entries, rounds);
+    profAdd(P.prof, kProfCoarseTilesPerRound, sProf[3], rounds);
+    profAdd(P.prof, kProfCoarseEmitsPerRound, emits, rounds);
+    profAdd(P.prof, kProfCoarseEmitsPerTri, emits, entries);
+    profAdd(P.prof, kProfCoarseCaseA, 100ull * sProf[5], entries);
+    profAdd(P.prof, kProfCoarseCaseB, 100ull * sProf[6], entries);
+    profAdd(P.prof, kProfCoarseCaseC, 100ull * sProf[7], entries);
+  }
 }
 
 template <int CT>
@@ -451,7 +492,7 @@ __global__ void __launch_bounds__(64) resetBatchKernel(const Params* __restrict_
 }
 
 void launchReset(const Params& P, hipStream_t stream) {
-  hipLaunchKernelGGL(resetCountersKernel, dim3(1), dim3(64), 0, stream, P.counters, (uint32_t)P.numTris);
+  hipLaunchKernelGGL(resetCountersKernel, dim3(1), dim3(64), 0, stream, P.counters, (uint32_t)P.numTris, P.prof);
 }
 
 int coarseShapeFor(const Params& P) {

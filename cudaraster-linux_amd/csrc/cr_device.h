@@ -30,9 +30,25 @@ constexpr uint32_t kDepthMax = 0xFFFFFFFFu - (2200u << 3);
 constexpr int kBaryMax = (1 << (30 - kSubpixelLog2)) - 1;
 __host__ __device__ constexpr uint32_t lerpError(int s) { return 2200u << s; }
 
-enum : uint32_t { kFlagDepth = 1, kFlagLerp = 2 };
+enum : uint32_t { kFlagDepth = 1, kFlagLerp = 2, kFlagQuads = 4 };
 enum : int32_t { kBlendReplace = 0, kBlendSrcOver = 1, kBlendAdditive = 2, kBlendDepthOnly = 3 };
-enum : int32_t { kShaderGouraud = 0, kShaderConstant = 1 };
+enum : int32_t { kShaderGouraud = 0, kShaderConstant = 1, kShaderTexPhong = 2 };
+
+// TextureSpec and Material (test/shader/Shaders.hpp:33-51), the layout of
+// cr_texture_spec / cr_material.
+struct TexSpec {
+  float size[2];         // level-0 texels; size[0] == 0: no texture
+  uint32_t pad[2];
+  float4 mip[16];        // per level: scaleX, scaleY, biasX, biasY in atlas coordinates
+};
+struct Material {
+  float4 diffuse;
+  float specular[3];
+  float glossiness;
+  TexSpec diffuseTexture;
+  TexSpec alphaTexture;
+};
+static_assert(sizeof(TexSpec) == 272 && sizeof(Material) == 576, "material layout");
 
 struct Header {          // CRTriangleHeader
   int16_t v0x, v0y, v1x, v1y, v2x, v2y;
@@ -62,6 +78,47 @@ struct Counters {
 };
 static_assert(sizeof(Counters) == 256, "counter block");
 
+// Profiling counters (ProfilingMode_Counters): each one a numerator and a
+// denominator, accumulated by the stages of a profiling draw, in the order of
+// the report (cr_profile_counter in cr.h has the same order).  Percentages
+// add 100 to the numerator per event and 1 to the denominator per candidate,
+// as the reference's CR_COUNT (PixelPipe.inl:91-101, PrivateDefs.hpp:158-204).
+enum ProfCounter : int {
+  kProfSetupViewportCull = 0,
+  kProfSetupBackfaceCull,
+  kProfSetupBetweenPixelsCull,
+  kProfSetupClipped,
+  kProfSetupSamplesPerTri,
+  kProfBinQueueOverflows,
+  kProfBinTrisPerChunk,
+  kProfBinTriBBArea,
+  kProfBinTriSingleBin,
+  kProfCoarseBins,
+  kProfCoarseRoundsPerBin,
+  kProfCoarseTrisPerRound,
+  kProfCoarseTilesPerRound,
+  kProfCoarseEmitsPerRound,
+  kProfCoarseEmitsPerTri,
+  kProfCoarseCaseA,
+  kProfCoarseCaseB,
+  kProfCoarseCaseC,
+  kProfFineTriPerTile,
+  kProfFineEarlyZCull,
+  kProfFineEmptyCull,
+  kProfFineZKill,
+  kProfFineWarpUtil,
+  kProfFineBlendRounds,
+  kProfFineFragPerTri,
+  kProfFineFragPerTile,
+  kProfCount
+};
+
+// Adds (num, denom) to counter c of the profile block (64-bit atomics).
+__device__ __forceinline__ void profAdd(unsigned long long* prof, int c, uint64_t num, uint64_t denom) {
+  if (num) atomicAdd(&prof[2 * c], (unsigned long long)num);
+  if (denom) atomicAdd(&prof[2 * c + 1], (unsigned long long)denom);
+}
+
 struct Params {
   // Input.
   int32_t numTris;
@@ -87,6 +144,15 @@ struct Params {
   int32_t coarseThreads;           // coarse workgroup size (256 or 1024), 0: chosen from the draw
   uint32_t* tileFrags;             // per-tile fragment counts (counting draws only), else null
   uint32_t* chunkSubtris;          // per-setup-chunk visible subtriangles (counting draws only), else null
+  unsigned long long* prof;        // profile counters, 2 * kProfCount (profiling draws only), else null
+  // texPhong shader constants (Shaders.hpp:57-65): materials, per-triangle
+  // material index, float4 texture atlas.
+  const Material* materials;
+  int32_t numMaterials;
+  const int32_t* triMaterial;
+  int64_t numTriMaterials;
+  const float4* atlas;
+  int32_t atlasWidth, atlasHeight;
   // Setup output.
   int32_t maxSubtris;
   uint8_t* triSubtris;

@@ -25,6 +25,7 @@
 #include <utility>
 
 #include "cr_device.h"
+#include "cr_texphong.h"
 
 namespace crhip {
 
@@ -185,9 +186,11 @@ constexpr int recWords() { return kChunk * (VIS ? Rec<S>::SIZE_VIS : Rec<S>::SIZ
 // Fine raster of tile group `group` (W tiles, one wave each) of one draw.
 // COUNT: counting draw (fragment totals per tile into P.tileFrags).
 // BL: blend mode known at compile time (in-order shading), or -1 for P.blend.
+// TEX: the texPhong fragment shader (in-order shading; it may discard).
 // recAll: the workgroup's LDS record arrays, W * recWords<S, VIS>() words.
-template <int S, bool VIS, bool COUNT, int W, bool DEPTH, int BL>
+template <int S, bool VIS, bool COUNT, int W, bool DEPTH, int BL, bool TEX = false>
 __device__ __forceinline__ void fineBody(const Params& P, const int group, uint32_t* __restrict__ recAll) {
+  static_assert(!(TEX && VIS), "texPhong shades in order");
   constexpr int N = 1 << S;
   using L = Rec<S>;
   constexpr int RS = VIS ? L::SIZE_VIS : L::SIZE_FULL;
@@ -245,6 +248,11 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
   constexpr CentroidOrder<S> co;
 
   uint64_t frags = 0;
+  // Profiling draws (COUNT with a profile block): records culled by
+  // hierarchical Z, records walked, walked records with no covered sample,
+  // covered samples, fragments whose covered samples all fail the depth
+  // test, fragments written.
+  uint64_t pHiz = 0, pWalked = 0, pEmpty = 0, pSamples = 0, pZKill = 0, pWritten = 0;
   const uint32_t start = P.tileStart[tile];
   const bool hiz = DEPTH && P.hiz;
 
@@ -275,6 +283,10 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
     }
     const uint64_t keepMask = __ballot(keep);
     const uint32_t m = (uint32_t)__popcll(keepMask);
+    if (COUNT) {
+      pHiz += n - m;
+      pWalked += m;
+    }
     if (keep) {
       uint32_t* r = rec + (uint32_t)__popcll(keepMask & laneMaskLt64()) * RS;
       int32_t ax[3], ay[3];
@@ -327,7 +339,15 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
       // walk stores per sample; at 1x the code is always the pixel centre.
       r[L::SLOT] = VIS ? ((slot << 4) | (N == 1 ? 15u : 0u)) : slot;
       if (!VIS) r[L::TRI] = (slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris];
-      if (!VIS && gouraud) {
+      if (TEX) {
+        // Barycentric planes and vertex indices; the shader reads the
+        // varyings, material and textures itself.
+        const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
+                    t3 = tdata[(size_t)slot * 4 + 3];
+        uint32_t* q = r + L::SH;
+        q[0] = t1.x; q[1] = t1.y; q[2] = t1.z; q[3] = t1.w; q[4] = t2.x; q[5] = t2.y; q[6] = t2.z;
+        q[7] = t2.w; q[8] = t3.x; q[9] = t3.y; q[10] = t3.z; q[11] = t3.w;
+      } else if (!VIS && gouraud) {
         const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
                     t3 = tdata[(size_t)slot * 4 + 3];
         const float4 c0 = varying0(verts, t3.y, P.vertexStride), c1 = varying0(verts, t3.z, P.vertexStride),
@@ -363,6 +383,11 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
                   __ballot(Lk[2] <= (int32_t)cur[L::R + 2 * N + i]);
         anyM |= covM[i];
       }
+      if (COUNT) {
+        pEmpty += anyM == 0;
+#pragma unroll
+        for (int i = 0; i < N; i++) pSamples += (uint64_t)__popcll(covM[i]);
+      }
       if (anyM == 0) continue;
       if (COUNT) frags += (uint64_t)__popcll(anyM);
 
@@ -379,10 +404,15 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
           wrM[i] = covM[i] & __ballot(dz[i] < dep[i]);     // LESS (FineRaster.inl:697)
           anyWrM |= wrM[i];
         }
+        if (COUNT) {
+          pZKill += (uint64_t)__popcll(anyM & ~anyWrM);
+          pWritten += (uint64_t)__popcll(anyWrM);
+        }
         if (anyWrM == 0) continue;
       } else {
 #pragma unroll
         for (int i = 0; i < N; i++) { wrM[i] = covM[i]; dz[i] = 0; }
+        if (COUNT) pWritten += (uint64_t)__popcll(anyM);
       }
       bool cov[N], wr[N];
       bool anyWr = false;
@@ -411,7 +441,16 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
         }
       } else {
         uint32_t src = P.constColor;
-        if (anyWr && gouraud) {
+        const uint32_t tri = r[L::TRI];
+        bool discard = false;
+        if (TEX) {
+          if (anyWr) {
+            const uint32_t* q = r + L::SH;
+            const Planes pl = {(int32_t)q[0], (int32_t)q[1], (int32_t)q[2], (int32_t)q[3], (int32_t)q[4],
+                               (int32_t)q[5], (int32_t)q[6], (int32_t)q[7], (int32_t)q[8]};
+            discard = shadeTexPhong<S>(P, pl, q[9], q[10], q[11], tri, code, px, py, &src);
+          }
+        } else if (anyWr && gouraud) {
           const uint32_t* q = r + L::SH;
           const uint4 t1 = make_uint4(q[0], q[1], q[2], q[3]);
           const uint4 t2 = make_uint4(q[4], q[5], q[6], q[7]);
@@ -424,10 +463,10 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
                                         __uint_as_float(q[20]));
           src = shadeGouraud<S>(lerpOn, code, px, py, t1, t2, t3, c0, c1, c2);
         }
-        const uint32_t tri = r[L::TRI];
+        // A discarded fragment writes neither depth nor colour.
 #pragma unroll
         for (int i = 0; i < N; i++) {
-          if (wr[i]) {
+          if (wr[i] && !discard) {
             if (DEPTH) dep[i] = dz[i];
             col[i] = blendPixel(blend, src, col[i]);
             aux[i] = tri;
@@ -487,6 +526,17 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
   // Fragment statistics only on counting draws: one plain store per tile
   // (a device-wide atomic per tile serialises across the XCDs).
   if (COUNT && lane == 0) P.tileFrags[tile] = (uint32_t)frags;
+  if (COUNT && P.prof != nullptr && lane == 0 && cnt > 0) {
+    profAdd(P.prof, kProfSetupSamplesPerTri, pSamples, 0);
+    profAdd(P.prof, kProfFineTriPerTile, cnt, 1);
+    profAdd(P.prof, kProfFineEarlyZCull, 100 * pHiz, cnt);
+    profAdd(P.prof, kProfFineEmptyCull, 100 * pEmpty, pWalked);
+    profAdd(P.prof, kProfFineZKill, 100 * pZKill, frags);
+    profAdd(P.prof, kProfFineWarpUtil, 100 * frags, 64 * pWalked);
+    profAdd(P.prof, kProfFineBlendRounds, pWritten, pWritten);
+    profAdd(P.prof, kProfFineFragPerTri, frags, pWalked);
+    profAdd(P.prof, kProfFineFragPerTile, frags, 1);
+  }
 }
 
 // In-order shading blends in the per-triangle loop: SrcOver and Additive
@@ -509,6 +559,24 @@ __global__ void __launch_bounds__(64 * kFineWaves, (fineMinWaves<S, VIS>())) fin
   const int group = xcdGroup(groups);
   if (group >= groups) return;
   fineDispatch<S, VIS, COUNT, kFineWaves, DEPTH>(P, group, recAll);
+}
+
+// The texPhong shader: in-order shading with the shader compiled in, blend
+// mode from the draw.
+template <int S, bool COUNT, bool DEPTH>
+__global__ void __launch_bounds__(64 * kFineWaves) fineTexKernel(const Params P) {
+  __shared__ __attribute__((aligned(16))) uint32_t recAll[kFineWaves * recWords<S, false>()];
+  const int groups = (P.numTiles + kFineWaves - 1) / kFineWaves;
+  const int group = xcdGroup(groups);
+  if (group >= groups) return;
+  fineBody<S, false, COUNT, kFineWaves, DEPTH, -1, true>(P, group, recAll);
+}
+
+template <int S, bool COUNT, bool DEPTH>
+static void launchFineTex(const Params& P, hipStream_t stream) {
+  const int groups = (P.numTiles + kFineWaves - 1) / kFineWaves;
+  const int per = (groups + 7) >> 3;
+  hipLaunchKernelGGL((fineTexKernel<S, COUNT, DEPTH>), dim3(per * 8), dim3(64 * kFineWaves), 0, stream, P);
 }
 
 // Batched draws: the tile groups of the views of one pipe variant form one
@@ -569,7 +637,8 @@ using AllFine = FineTables<0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15>
 static_assert(kFineVariants == 16, "variant table");
 
 int fineVariant(const Params& P) {
-  const bool vis = (P.blend == kBlendReplace || P.blend == kBlendDepthOnly) && P.maxSubtris < (1 << 28);
+  const bool vis = (P.blend == kBlendReplace || P.blend == kBlendDepthOnly) && P.maxSubtris < (1 << 28) &&
+                   P.shader != kShaderTexPhong;
   const bool depth = (P.flags & kFlagDepth) != 0;
   return (P.samplesLog2 * 2 + (vis ? 1 : 0)) * 2 + (depth ? 1 : 0);
 }
@@ -585,6 +654,19 @@ void launchFineBatch(int variant, int tilesPerGroup, int banded, const Params* v
 // Host-side launcher.  Deferred shading needs the visibility word
 // (slot << 4 | centroid) to hold every slot: slots < 2^28.
 void launchFine(const Params& P, hipStream_t stream) {
+  if (P.shader == kShaderTexPhong) {
+    using L = void (*)(const Params&, hipStream_t);
+    static constexpr L table[16] = {
+        &launchFineTex<0, false, false>, &launchFineTex<0, false, true>, &launchFineTex<0, true, false>,
+        &launchFineTex<0, true, true>,   &launchFineTex<1, false, false>, &launchFineTex<1, false, true>,
+        &launchFineTex<1, true, false>,  &launchFineTex<1, true, true>,   &launchFineTex<2, false, false>,
+        &launchFineTex<2, false, true>,  &launchFineTex<2, true, false>,  &launchFineTex<2, true, true>,
+        &launchFineTex<3, false, false>, &launchFineTex<3, false, true>,  &launchFineTex<3, true, false>,
+        &launchFineTex<3, true, true>};
+    const int count = P.tileFrags != nullptr ? 1 : 0, depth = (P.flags & kFlagDepth) ? 1 : 0;
+    table[P.samplesLog2 * 4 + count * 2 + depth](P, stream);
+    return;
+  }
   if (P.tileFrags != nullptr) {
     // Counting draws run the in-order variant with per-tile totals.
     const bool depth = (P.flags & kFlagDepth) != 0;

@@ -116,9 +116,14 @@ struct DrawBuffers {
   uint32_t* tileList = nullptr;
   uint32_t* tileFrags = nullptr;      // counting draws only
   uint32_t* chunkSubtris = nullptr;   // counting draws only
+  unsigned long long* prof = nullptr; // profiling draws only: 2 * kProfCount counters
   crhip::Counters* counters = nullptr;
 };
 
+// Whether the context's next draw runs the counting kernel variants, and
+// whether it records the profile counters.
+bool countingDraw(const cr_context* ctx);
+bool profilingDraw(const cr_context* ctx);
 
 // drawTriangles' state checks (CudaRaster.cpp:243-260).
 cr_status validateDraw(cr_context* ctx);
@@ -160,13 +165,18 @@ struct cr_context {
   int64_t iofs = 0;
   int32_t numTris = 0;
   cr_debug_params dbg{};
+  cr_shader_constants shaderConstants{};
+  bool shaderConstantsSet = false;
 
   int64_t maxSubtris = 1, maxBinEntries = 1, maxTileEntries = 1;
   crhost::DevBuf triSubtris, triHeader, triData, subtriTri;
   crhost::DevBuf chunkBase, binCnt, binOff, binList;
   crhost::DevBuf tileStart, tileCount, tileList, tileFrags, chunkSubtris;
-  crhost::DevBuf counters;
+  crhost::DevBuf counters, profBuf;
   crhip::Counters* hostCounters = nullptr;   // pinned
+  unsigned long long* hostProf = nullptr;    // pinned, 2 * kProfCount
+  cr_profile_counters lastProf{};
+  bool lastProfiled = false;                 // the last launch recorded profile counters
   hipEvent_t ev[5] = {};
   bool evRecorded = false;
   hipStream_t pendingStream = nullptr;

@@ -121,10 +121,12 @@ __global__ void __launch_bounds__(256) interpForwardKernel(const InterpGeom G, c
 // x' = x - qx w, y' = y - qy w to the clip x, y, w of each corner; the
 // results are staged in LDS.  The pixels then form runs of one triangle
 // each; phases 2 and 3 sum every (run, corner, channel) and (run, corner,
-// x/y/w) entry over the run's pixels and issue one FP64 atomic per entry,
-// consecutive lanes on consecutive entries -- consecutive addresses within a
-// run, which device-scope atomics serve several times faster than scattered
-// ones.
+// x/y/w) entry over the run's pixels in FP64 and add it to the output
+// gradient with one FP32 atomic per entry, consecutive lanes on consecutive
+// entries -- consecutive addresses within a run, which device-scope atomics
+// serve several times faster than scattered ones.  A vertex receives one
+// addition per (tile, triangle) run that touches it, in whatever order the
+// atomics land, so the last bits of a gradient can differ between runs.
 constexpr int kBwdTile = 16;
 constexpr int kBwdThreads = kBwdTile * kBwdTile;
 constexpr int kBwdStageC = 8;                  // channels of dL/dout staged in LDS per pixel

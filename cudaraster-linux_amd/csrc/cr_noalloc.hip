@@ -51,7 +51,7 @@ inline size_t alignUp(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 // Bytes of the per-draw tables (everything but the three queues whose
 // capacity is chosen).
-size_t fixedBytes(int64_t numTris, const crhost::Geometry& g, int32_t numChunks, bool counting) {
+size_t fixedBytes(int64_t numTris, const crhost::Geometry& g, int32_t numChunks, bool counting, bool profiling) {
   const size_t nc = (size_t)std::max(numChunks, 1);
   size_t b = alignUp(sizeof(Counters));
   b += alignUp((size_t)std::max<int64_t>(numTris, 1));      // triSubtris
@@ -59,6 +59,7 @@ size_t fixedBytes(int64_t numTris, const crhost::Geometry& g, int32_t numChunks,
   b += 2 * alignUp((size_t)g.nb * nc * 4);                    // binCnt, binOff
   b += 2 * alignUp((size_t)g.nt * 4);                         // tileStart, tileCount
   if (counting) b += alignUp((size_t)g.nt * 4) + alignUp(nc * 4);   // tileFrags, chunkSubtris
+  if (profiling) b += alignUp(2 * kProfCount * sizeof(unsigned long long));
   return b;
 }
 
@@ -71,7 +72,7 @@ size_t queueBytes(int64_t numTris, int64_t subtris, int64_t bin, int64_t tile) {
 
 // Lays the draw out in the array at `base`.
 crhost::DrawBuffers layout(uint8_t* base, int64_t numTris, const crhost::Geometry& g, int32_t numChunks,
-                           bool counting, int64_t subtris, int64_t bin, int64_t tile) {
+                           bool counting, bool profiling, int64_t subtris, int64_t bin, int64_t tile) {
   crhost::DrawBuffers B;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -94,6 +95,7 @@ crhost::DrawBuffers layout(uint8_t* base, int64_t numTris, const crhost::Geometr
     B.tileFrags = reinterpret_cast<uint32_t*>(take((size_t)g.nt * 4));
     B.chunkSubtris = reinterpret_cast<uint32_t*>(take(nc * 4));
   }
+  if (profiling) B.prof = reinterpret_cast<unsigned long long*>(take(2 * kProfCount * sizeof(unsigned long long)));
   B.triHeader = reinterpret_cast<Header*>(take((size_t)subtris * sizeof(Header)));
   B.triData = reinterpret_cast<TriData*>(take((size_t)subtris * sizeof(TriData)));
   B.subtriTri = reinterpret_cast<uint32_t*>(take((size_t)std::max<int64_t>(subtris - numTris, 1) * 4));
@@ -173,8 +175,8 @@ extern "C" cr_status cr_draw_triangles_noalloc(cr_context* ctx, void* stream) {
   const Geometry g = geometryOf(ctx->color);
   const int64_t nt = ctx->numTris;
   const int32_t numChunks = (int32_t)((nt + kSetupChunkTris - 1) / kSetupChunkTris);
-  const bool counting = ctx->dbg.count_fragments != 0;
-  const size_t fixed = fixedBytes(nt, g, numChunks, counting);
+  const bool counting = countingDraw(ctx), profiling = profilingDraw(ctx);
+  const size_t fixed = fixedBytes(nt, g, numChunks, counting, profiling);
   // Demands: the least each queue must hold (first the allocating entry
   // point's first-draw estimates, later the exact counts of an overflowed
   // draw); capacities: the demands scaled up to fill the array.
@@ -205,7 +207,8 @@ extern "C" cr_status cr_draw_triangles_noalloc(cr_context* ctx, void* stream) {
                   "cr_draw_triangles_noalloc: the draw needs %lld subtriangle slots, %lld bin and %lld tile "
                   "entries, more than the %zu-byte scratch array holds",
                   (long long)dSub, (long long)dBin, (long long)dTile, kScratchBytes);
-    const DrawBuffers B = layout(static_cast<uint8_t*>(base), nt, g, numChunks, counting, subtris, bin, tile);
+    const DrawBuffers B =
+        layout(static_cast<uint8_t*>(base), nt, g, numChunks, counting, profiling, subtris, bin, tile);
     const Params P = makeParams(ctx, g, numChunks, B);
     st = launchAll(ctx, P, s);
     if (st != CR_OK) return st;

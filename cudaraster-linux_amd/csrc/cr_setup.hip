@@ -160,9 +160,10 @@ __device__ __forceinline__ void setupTriangle(const Params& P, Header* th, TriDa
   }
   uint4* tdv = reinterpret_cast<uint4*>(td);
   tdv[0] = make_uint4(zp[0], zp[1], zp[2], zslope);
-  if (P.shader == kShaderGouraud) {
-    // Barycentric planes and vertex indices: read only by the Gouraud
-    // shader (a constant-colour draw writes the depth plane alone).
+  if (P.shader != kShaderConstant) {
+    // Barycentric planes and vertex indices: read only by the shaders that
+    // interpolate varyings (a constant-colour draw writes the depth plane
+    // alone).
     tdv[1] = make_uint4(wp[0], wp[1], wp[2], up[0]);
     tdv[2] = make_uint4(up[1], up[2], vp[0], vp[1]);
     tdv[3] = make_uint4(vp[2], (uint32_t)vidx.x, (uint32_t)vidx.y, (uint32_t)vidx.z);
@@ -282,12 +283,19 @@ __device__ __noinline__ int setupClipped(const Params P, int t, int3 vidx, float
   return numSub;
 }
 
+// Outcome of one triangle's setup, as the reference's setup counters
+// classify it (TriangleSetup.inl:255-330).
+enum SetupOutcome : int { kSetupVisible = 0, kSetupFrustum, kSetupBackface, kSetupBetween, kSetupClipPath,
+                          kSetupDropped };
+
 // One triangle: returns its number of subtriangles and their first slot.
 // When the triangle is set up without clipping, *hv receives the header's
-// vertex words (as written to P.triHeader[t]) and *hvOk is set.
-__device__ __forceinline__ int setupOne(const Params& P, int t, int* base, uint4* hv, bool* hvOk) {
+// vertex words (as written to P.triHeader[t]) and *hvOk is set.  *why
+// receives the SetupOutcome.
+__device__ __forceinline__ int setupOne(const Params& P, int t, int* base, uint4* hv, bool* hvOk, int* why) {
   *base = t;
   *hvOk = false;
+  *why = kSetupDropped;
   int3 vidx;
   const int32_t* ip = P.indices + (int64_t)t * 3;
   vidx.x = ip[0]; vidx.y = ip[1]; vidx.z = ip[2];
@@ -312,6 +320,7 @@ __device__ __forceinline__ int setupOne(const Params& P, int t, int* base, uint4
                ((v0.w < -v0.z) & (v1.w < -v1.z) & (v2.w < -v2.z));
     if (out) {
       P.triSubtris[t] = 0;
+      *why = kSetupFrustum;
       return 0;
     }
   }
@@ -328,6 +337,7 @@ __device__ __forceinline__ int setupOne(const Params& P, int t, int* base, uint4
     if (loxy >= -32768 && hixy <= 32767 && hixy - loxy <= aabbLimit) {
       int res = prepare(P, p0, p1, p2, lo, hi, d1, d2, area);
       P.triSubtris[t] = (res == 0) ? 1 : 0;
+      *why = res == 0 ? kSetupVisible : (res == 1 ? kSetupBackface : kSetupBetween);
       if (res == 0) {
         setupTriangle(P, &P.triHeader[t], &P.triData[t], vidx, v0, v1, v2, make_float2(0.0f, 0.0f),
                       make_float2(1.0f, 0.0f), make_float2(0.0f, 1.0f), p0, p1, p2, rcpW, d1, d2, area);
@@ -339,6 +349,7 @@ __device__ __forceinline__ int setupOne(const Params& P, int t, int* base, uint4
       return res == 0 ? 1 : 0;
     }
   }
+  *why = kSetupClipPath;
   return setupClipped(P, t, vidx, v0, v1, v2, base);
 }
 
@@ -425,15 +436,45 @@ __device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
   __shared__ uint32_t list[kSetupChunk];
   __shared__ uint32_t sums[kSetupWaves + 1];
   __shared__ uint32_t sBase;
+  __shared__ uint32_t sProf[8];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int t = chunk * kSetupChunk + threadIdx.x;
 
   // ---- setup ---------------------------------------------------------------
-  int base = t, k = 0;
+  int base = t, k = 0, why = kSetupDropped;
   uint4 hvOwn = make_uint4(0u, 0u, 0u, 0u);   // header vertex words kept from setup (no re-read)
   bool hvOwnOk = false;
-  if (t < P.numTris) k = setupOne(P, t, &base, &hvOwn, &hvOwnOk);
+  if (t < P.numTris) k = setupOne(P, t, &base, &hvOwn, &hvOwnOk, &why);
+  if (P.prof != nullptr) {
+    // Profiling draws: per-outcome triangle counts of the chunk, then its
+    // triangles with at least one subtriangle.
+    if (threadIdx.x < 8) sProf[threadIdx.x] = 0;
+    __syncthreads();
+    const bool valid = t < P.numTris;
+    const uint32_t nValid = (uint32_t)__popcll(__ballot(valid));
+    const uint32_t nFr = (uint32_t)__popcll(__ballot(valid && why == kSetupFrustum));
+    const uint32_t nBf = (uint32_t)__popcll(__ballot(valid && why == kSetupBackface));
+    const uint32_t nBt = (uint32_t)__popcll(__ballot(valid && why == kSetupBetween));
+    const uint32_t nCl = (uint32_t)__popcll(__ballot(valid && why == kSetupClipPath));
+    const uint32_t nVis = (uint32_t)__popcll(__ballot(valid && k > 0));
+    if (lane == 0) {
+      atomicAdd(&sProf[0], nValid);
+      atomicAdd(&sProf[1], nFr);
+      atomicAdd(&sProf[2], nBf);
+      atomicAdd(&sProf[3], nBt);
+      atomicAdd(&sProf[4], nCl);
+      atomicAdd(&sProf[5], nVis);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      profAdd(P.prof, kProfSetupViewportCull, 100ull * sProf[1], sProf[0]);
+      profAdd(P.prof, kProfSetupBackfaceCull, 100ull * sProf[2], sProf[0]);
+      profAdd(P.prof, kProfSetupBetweenPixelsCull, 100ull * sProf[3], sProf[0]);
+      profAdd(P.prof, kProfSetupClipped, 100ull * sProf[4], sProf[0]);
+      profAdd(P.prof, kProfSetupSamplesPerTri, 0, sProf[5]);
+    }
+  }
 
   for (int i = threadIdx.x; i < P.numBins; i += kSetupChunk) cnt[i] = 0;
   // Window = up to kSetupChunk consecutive subtriangles of the chunk in
@@ -493,6 +534,11 @@ __device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
       for (int q = 0; q < kSetupWaves; q++) acc += (uint32_t)__popcll(emitMask[q][i]);
       cnt[i] = acc;
     }
+    if (P.prof != nullptr) {
+      // Subtriangles whose box lies in a single bin.
+      const uint32_t n1 = (uint32_t)__popcll(__ballot(bb0.x0 == bb0.x1 && bb0.y0 == bb0.y1));
+      if (lane == 0 && n1) atomicAdd(&sProf[6], n1);
+    }
     __syncthreads();
   }
 
@@ -500,6 +546,11 @@ __device__ __forceinline__ void setupBinBody(const Params& P, const int chunk) {
   uint32_t tot2;
   const uint32_t c = (threadIdx.x < (unsigned)P.numBins) ? cnt[threadIdx.x] : 0u;
   const uint32_t off = blockScan<kSetupChunk>(c, sums, tot2);
+  if (P.prof != nullptr && threadIdx.x == 0) {
+    profAdd(P.prof, kProfBinTrisPerChunk, total, 1);
+    profAdd(P.prof, kProfBinTriBBArea, tot2, total);
+    profAdd(P.prof, kProfBinTriSingleBin, 100ull * sProf[6], total);
+  }
   if (threadIdx.x == 0) {
     uint32_t b = tot2 ? atomicAdd(&P.counters->binEntries, tot2) : 0u;
     if (tot2 && (uint64_t)b + tot2 > (uint64_t)P.maxBinEntries) {

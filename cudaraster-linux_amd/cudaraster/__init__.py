@@ -22,10 +22,20 @@ from . import scenes  # noqa: F401  (re-exported workloads)
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libcr.so")
 
+PROFILING_DEFAULT, PROFILING_COUNTERS = 0, 1
+# cr_profile_counter order (include/cr/cr.h).
+PROFILE_COUNTERS = [
+    "setup_viewport_cull", "setup_backface_cull", "setup_between_pixels_cull", "setup_clipped",
+    "setup_samples_per_tri", "bin_queue_overflows", "bin_tris_per_chunk", "bin_tri_bb_area",
+    "bin_tri_single_bin", "coarse_bins", "coarse_rounds_per_bin", "coarse_tris_per_round",
+    "coarse_tiles_per_round", "coarse_emits_per_round", "coarse_emits_per_tri", "coarse_case_a",
+    "coarse_case_b", "coarse_case_c", "fine_tri_per_tile", "fine_early_z_cull", "fine_empty_cull",
+    "fine_z_kill", "fine_warp_util", "fine_blend_rounds", "fine_frag_per_tri", "fine_frag_per_tile",
+]
 CR_OK, CR_ERR_INVALID, CR_ERR_STATE, CR_ERR_DEVICE, CR_ERR_OVERFLOW, CR_ERR_UNSUPPORTED, CR_ERR_NOMEM = range(7)
 FORMAT_RGBA8, FORMAT_DEPTH32, FORMAT_TRIANGLE_ID = 0, 1, 2
 RENDER_DEPTH, RENDER_LERP, RENDER_QUADS = 1, 2, 4
-SHADER_GOURAUD, SHADER_CONSTANT = 0, 1
+SHADER_GOURAUD, SHADER_CONSTANT, SHADER_TEX_PHONG = 0, 1, 2
 BLEND_REPLACE, BLEND_SRC_OVER, BLEND_ADDITIVE, BLEND_DEPTH_ONLY = 0, 1, 2, 3
 
 # The exported C symbols (include/cr/cr.h).
@@ -40,7 +50,8 @@ EXPORTS = [
     "cr_batch_create", "cr_batch_destroy", "cr_batch_draw", "cr_batch_autotune", "cr_batch_reserve",
     "cr_batch_use_graph", "cr_batch_get_info", "cr_batch_get_view_counters",
     "cr_interpolate", "cr_interpolate_backward", "cr_batch_configure", "cr_vertex_shader_passthrough",
-    "cr_surface_resolve", "cr_draw_triangles_noalloc",
+    "cr_surface_resolve", "cr_draw_triangles_noalloc", "cr_get_profile_counters", "cr_set_shader_constants",
+    "cr_interpolate_ids",
 ]
 
 
@@ -71,7 +82,25 @@ class Counters(ctypes.Structure):
 
 class DebugParams(ctypes.Structure):
     _fields_ = [("disable_hiz", ctypes.c_int32), ("count_fragments", ctypes.c_int32),
-                ("coarse_threads", ctypes.c_int32)]
+                ("coarse_threads", ctypes.c_int32), ("profiling_mode", ctypes.c_int32)]
+
+
+class ShaderConstants(ctypes.Structure):
+    """cr_shader_constants: the texPhong shader's materials, per-triangle
+    material indices and float4 texture atlas (device pointers)."""
+    _fields_ = [("materials", ctypes.c_void_p), ("num_materials", ctypes.c_int32),
+                ("atlas_width", ctypes.c_int32), ("atlas_height", ctypes.c_int32),
+                ("triangle_material_idx", ctypes.c_void_p), ("num_triangle_materials", ctypes.c_int64),
+                ("atlas", ctypes.c_void_p)]
+
+
+class ProfileCounters(ctypes.Structure):
+    """cr_profile_counters: numerator and denominator per counter."""
+    _fields_ = [("num", ctypes.c_int64 * len(PROFILE_COUNTERS)),
+                ("denom", ctypes.c_int64 * len(PROFILE_COUNTERS))]
+
+    def as_dict(self):
+        return {n: (int(self.num[i]), int(self.denom[i])) for i, n in enumerate(PROFILE_COUNTERS)}
 
 
 class View(ctypes.Structure):
@@ -154,6 +183,9 @@ def load(path: str | None = None):
         "cr_vertex_shader_passthrough": [vp, vp, i64, ctypes.POINTER(ctypes.c_float), vp, i32, vp],
         "cr_surface_resolve": [vp, vp, vp],
         "cr_draw_triangles_noalloc": [vp, vp],
+        "cr_get_profile_counters": [vp, ctypes.POINTER(ProfileCounters)],
+        "cr_set_shader_constants": [vp, ctypes.POINTER(ShaderConstants)],
+        "cr_interpolate_ids": [vp, vp, i32, i32, vp, i64, vp, i32, vp, i32, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -238,8 +270,13 @@ class CudaSurface:
         int32 tensor [height, width] of RGBA8 words (row 0 at the bottom)."""
         import torch
         w, h = self.width, self.height
+        if self.format != FORMAT_RGBA8:
+            raise ValueError("resolve needs an RGBA8 surface")
         if out is None:
             out = torch.empty((h, w), dtype=torch.int32, device="cuda")
+        elif not (hasattr(out, "is_cuda") and out.is_cuda and out.is_contiguous() and
+                  out.dtype in (torch.int32, torch.uint32) and out.numel() >= h * w):
+            raise ValueError(f"out must be a contiguous 32-bit device tensor of at least {h}x{w} elements")
         s = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
         _check(load().cr_surface_resolve(self._h, _ptr(out), s))
         return out
@@ -312,6 +349,29 @@ class CudaRaster:
         _check(load().cr_set_pixel_pipe(self.ctx, ctypes.byref(pipe) if pipe is not None else None), self.ctx)
         self._stride = pipe.vertex_struct_size if pipe is not None else None
 
+    def setShaderConstants(self, materials, tri_material, atlas):
+        """The texPhong shader's constants (cr_set_shader_constants):
+        `materials` a device uint8 tensor of 576-byte Material records
+        (scenes.MATERIAL_DTYPE), `tri_material` device int32 [T], `atlas`
+        device float32 [H, W, 4]; None clears them."""
+        L = load()
+        if materials is None:
+            self._keep.pop("shader_constants", None)
+            _check(L.cr_set_shader_constants(self.ctx, None), self.ctx)
+            return
+        for name, t, dt in (("materials", materials, "uint8"), ("tri_material", tri_material, "int32"),
+                            ("atlas", atlas, "float32")):
+            if not (hasattr(t, "is_cuda") and t.is_cuda and t.is_contiguous() and str(t.dtype) == f"torch.{dt}"):
+                raise ValueError(f"{name} must be a contiguous device tensor of {dt}")
+        if materials.numel() % 576:
+            raise ValueError("materials must hold whole 576-byte Material records")
+        if atlas.dim() != 3 or atlas.shape[2] != 4:
+            raise ValueError("atlas must be [height, width, 4] float32")
+        c = ShaderConstants(materials.data_ptr(), materials.numel() // 576, int(atlas.shape[1]), int(atlas.shape[0]),
+                            tri_material.data_ptr(), tri_material.numel(), atlas.data_ptr())
+        self._keep["shader_constants"] = (materials, tri_material, atlas)
+        _check(L.cr_set_shader_constants(self.ctx, ctypes.byref(c)), self.ctx)
+
     def setVertexBuffer(self, buf, ofs=0, num_vertices=None):
         """num_vertices defaults to the whole tensor divided by the pipe's
         vertex struct size (set the pipe first)."""
@@ -353,8 +413,19 @@ class CudaRaster:
         (device buffer of Vec3f) into the first 16 bytes of each out_stride-byte
         record of `out` (a vertex buffer for setVertexBuffer).  pos_to_clip
         is a 4x4 matrix indexed [row][column] (numpy or nested lists)."""
+        import torch
         m = np.ascontiguousarray(np.asarray(pos_to_clip, dtype=np.float32).reshape(4, 4).T).reshape(16)
+        if not (hasattr(model_pos, "is_cuda") and model_pos.is_cuda and model_pos.dtype == torch.float32 and
+                model_pos.is_contiguous()):
+            raise ValueError("model_pos must be a contiguous float32 device tensor of Vec3f positions")
         n = (model_pos.numel() // 3) if num_vertices is None else int(num_vertices)
+        if n < 0 or model_pos.numel() < 3 * n:
+            raise ValueError(f"model_pos holds {model_pos.numel()} floats, {n} vertices need {3 * n}")
+        if not (hasattr(out, "is_cuda") and out.is_cuda and out.is_contiguous()):
+            raise ValueError("out must be a contiguous device tensor")
+        if out.numel() * out.element_size() < n * int(out_stride):
+            raise ValueError(f"out holds {out.numel() * out.element_size()} bytes, {n} records of "
+                             f"{int(out_stride)} bytes need {n * int(out_stride)}")
         s = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
         _check(load().cr_vertex_shader_passthrough(self.ctx, _ptr(model_pos), n,
                                                    m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _ptr(out),
@@ -375,12 +446,20 @@ class CudaRaster:
         return c.as_dict()
 
     def getProfilingInfo(self) -> str:
-        buf = ctypes.create_string_buffer(4096)
-        _check(load().cr_get_profiling_info(self.ctx, buf, 4096), self.ctx)
+        buf = ctypes.create_string_buffer(8192)
+        _check(load().cr_get_profiling_info(self.ctx, buf, 8192), self.ctx)
         return buf.value.decode()
 
-    def setDebugParams(self, disable_hiz=False, count_fragments=False, coarse_threads=0):
-        p = DebugParams(1 if disable_hiz else 0, 1 if count_fragments else 0, int(coarse_threads))
+    def getProfileCounters(self) -> dict:
+        """Counters of the last PROFILING_COUNTERS draw: name -> (num, denom)."""
+        c = ProfileCounters()
+        _check(load().cr_get_profile_counters(self.ctx, ctypes.byref(c)), self.ctx)
+        return c.as_dict()
+
+    def setDebugParams(self, disable_hiz=False, count_fragments=False, coarse_threads=0,
+                       profiling_mode=PROFILING_DEFAULT):
+        p = DebugParams(1 if disable_hiz else 0, 1 if count_fragments else 0, int(coarse_threads),
+                        int(profiling_mode))
         _check(load().cr_set_debug_params(self.ctx, ctypes.byref(p)), self.ctx)
 
     def close(self):
@@ -426,6 +505,13 @@ class SceneRenderer:
         if debug:
             self.cr.setDebugParams(**debug)
         self.vbuf, self.ibuf = device_buffers(scene, device)
+        if getattr(scene, "materials", None) is not None:
+            import torch
+            self.shader_constants = (
+                torch.from_numpy(np.ascontiguousarray(scene.materials).view(np.uint8).reshape(-1).copy()).to(device),
+                torch.from_numpy(np.ascontiguousarray(scene.tri_material, dtype=np.int32)).to(device),
+                torch.from_numpy(np.ascontiguousarray(scene.atlas, dtype=np.float32)).to(device))
+            self.cr.setShaderConstants(*self.shader_constants)
         self.cr.setVertexBuffer(self.vbuf, 0, scene.vertices.shape[0])
         self.cr.setIndexBuffer(self.ibuf, 0, scene.num_tris)
 

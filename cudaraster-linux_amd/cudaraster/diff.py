@@ -35,11 +35,16 @@ class Interpolate:
 
         class _Interpolate(torch.autograd.Function):
             @staticmethod
-            def forward(ctx, pos, attr, tris, tri_image, raster):
+            def forward(ctx, pos, attr, tris, tri_image, raster, ids_given):
                 H, W = tri_image.shape
                 C = attr.shape[1]
                 out = torch.empty((H, W, C), dtype=torch.float32, device=pos.device)
-                raster._interpolate(pos, attr, tris, out, tri_image)
+                if ids_given:
+                    # Visibility from the caller's id image, not from whatever
+                    # the rasterizer drew last.
+                    raster._interpolate_ids(pos, attr, tris, out, tri_image)
+                else:
+                    raster._interpolate(pos, attr, tris, out, tri_image)
                 ctx.save_for_backward(pos, attr, tris, tri_image)
                 ctx.raster = raster
                 return out
@@ -48,7 +53,7 @@ class Interpolate:
             def backward(ctx, grad_out):
                 pos, attr, tris, tri_image = ctx.saved_tensors
                 gp, ga = ctx.raster._interpolate_backward(pos, attr, tris, tri_image, grad_out.contiguous())
-                return gp, ga, None, None, None
+                return gp, ga, None, None, None, None
 
         cls._fn = _Interpolate
         return cls._fn
@@ -151,12 +156,15 @@ class DiffRasterizer:
 
     def render(self, pos, attr, tris, tri_image=None):
         """Interpolated attributes [H, W, C].  Without tri_image, draws pos/tris
-        first; the interpolation then also writes the triangle ids."""
+        first; the interpolation then also writes the triangle ids.  With
+        tri_image (int32 [H, W], e.g. from rasterize()), interpolates over
+        those ids, whatever has been drawn since."""
         import torch
-        if tri_image is None:
+        given = tri_image is not None
+        if not given:
             self._draw(pos, tris)
             tri_image = torch.empty((self.height, self.width), dtype=torch.int32, device=pos.device)
-        return Interpolate.function().apply(pos, attr, tris, tri_image, self)
+        return Interpolate.function().apply(pos, attr, tris, tri_image, self, given)
 
     def render_views(self, pos, attr, tris):
         """Several views at once: lists of pos [V_i, 4], attr [V_i, C] and
@@ -171,21 +179,47 @@ class DiffRasterizer:
         return InterpolateViews.function().apply(self, len(pos), *pos, *attr, *tris)
 
     # -- raw calls ------------------------------------------------------------
+    def _check_inputs(self, pos, attr, tris, tri_image, channels):
+        """Shapes, dtypes, devices and layout of what the kernels read, checked
+        before any pointer reaches them."""
+        import torch
+        if not (pos.is_cuda and pos.dtype == torch.float32 and pos.dim() == 2 and pos.shape[1] == 4 and
+                pos.is_contiguous()):
+            raise CrError(1, "pos must be a contiguous float32 [V, 4] device tensor")
+        if not (tris.is_cuda and tris.dtype == torch.int32 and tris.dim() == 2 and tris.shape[1] == 3 and
+                tris.is_contiguous()):
+            raise CrError(1, "tris must be a contiguous int32 [T, 3] device tensor")
+        if channels > 0 and not (attr is not None and attr.is_cuda and attr.dtype == torch.float32 and
+                                 attr.dim() == 2 and attr.shape == (pos.shape[0], channels) and attr.is_contiguous()):
+            raise CrError(1, "attr must be a contiguous float32 [V, C] device tensor with one row per vertex")
+        if not (tri_image.is_cuda and tri_image.dtype == torch.int32 and tri_image.is_contiguous() and
+                tuple(tri_image.shape) == (self.height, self.width)):
+            raise CrError(1, "tri_image must be a contiguous int32 [H, W] device tensor at the rasterizer's resolution")
+
     def _interpolate(self, pos, attr, tris, out, tri_image, channels=None):
         C = attr.shape[1] if channels is None else channels
-        if attr is not None and (attr.dtype.itemsize != 4 or not attr.is_contiguous()):
-            raise CrError(1, "attr must be a contiguous float32 [V, C] device tensor")
+        self._check_inputs(pos, attr, tris, tri_image, C)
         hw = self.height * self.width
-        if tri_image.numel() != hw or not tri_image.is_contiguous() or (C > 0 and (out.numel() != hw * C or
-                                                                                 not out.is_contiguous())):
+        if C > 0 and (out.numel() != hw * C or not out.is_contiguous() or not out.is_cuda):
             raise CrError(1, "interpolation output buffers do not match the rasterizer's resolution")
         _check(load().cr_interpolate(self.cr.ctx, self.ids.handle, pos.data_ptr(), pos.shape[0],
                                      attr.data_ptr() if attr is not None else None, C, tris.data_ptr(),
                                      tris.shape[0], out.data_ptr(), tri_image.data_ptr(), _stream()), self.cr.ctx)
 
+    def _interpolate_ids(self, pos, attr, tris, out, tri_image):
+        C = attr.shape[1]
+        self._check_inputs(pos, attr, tris, tri_image, C)
+        if out.numel() != self.height * self.width * C or not out.is_contiguous():
+            raise CrError(1, "interpolation output buffer does not match the rasterizer's resolution")
+        _check(load().cr_interpolate_ids(self.cr.ctx, tri_image.data_ptr(), self.width, self.height, pos.data_ptr(),
+                                         pos.shape[0], attr.data_ptr(), C, tris.data_ptr(), tris.shape[0],
+                                         out.data_ptr(), _stream()), self.cr.ctx)
+
     def _interpolate_backward(self, pos, attr, tris, tri_image, grad_out):
         import torch
-        if grad_out.numel() != tri_image.numel() * attr.shape[1] or not grad_out.is_contiguous():
+        self._check_inputs(pos, attr, tris, tri_image, attr.shape[1])
+        if (grad_out.dtype != torch.float32 or grad_out.numel() != tri_image.numel() * attr.shape[1] or
+                not grad_out.is_contiguous()):
             raise CrError(1, "gradient of the image does not match its shape")
         gp = torch.empty_like(pos)
         ga = torch.empty_like(attr)

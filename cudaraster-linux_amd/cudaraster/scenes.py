@@ -26,9 +26,16 @@ import numpy as np
 # RenderModeFlag_* (cudaraster/cuda/PixelPipe.hpp:30-35)
 FLAG_DEPTH = 1
 FLAG_LERP = 2
+FLAG_QUADS = 4
 
 BLEND_REPLACE, BLEND_SRCOVER, BLEND_ADDITIVE, BLEND_DEPTHONLY = 0, 1, 2, 3
-SHADER_GOURAUD, SHADER_CONSTANT = 0, 1
+SHADER_GOURAUD, SHADER_CONSTANT, SHADER_TEXPHONG = 0, 1, 2
+
+# TextureSpec and Material (test/shader/Shaders.hpp:33-51).
+TEXSPEC_DTYPE = np.dtype([("size", "<f4", (2,)), ("pad", "<u4", (2,)), ("mipLevels", "<f4", (16, 4))])
+MATERIAL_DTYPE = np.dtype([("diffuseColor", "<f4", (4,)), ("specularColor", "<f4", (3,)), ("glossiness", "<f4"),
+                           ("diffuseTexture", TEXSPEC_DTYPE), ("alphaTexture", TEXSPEC_DTYPE)])
+assert TEXSPEC_DTYPE.itemsize == 272 and MATERIAL_DTYPE.itemsize == 576
 
 ZNEAR, ZFAR = 0.1, 100.0
 
@@ -48,6 +55,10 @@ class Scene:
     clear_depth: float = 1.0
     deferred_clear: bool = True
     const_color: int = 0xFF0000FF
+    # texPhong shader constants (None for the other shaders).
+    materials: np.ndarray = None        # MATERIAL_DTYPE [M]
+    tri_material: np.ndarray = None     # int32 [T]
+    atlas: np.ndarray = None            # float32 [H, W, 4]
 
     @property
     def stride(self) -> int:
@@ -401,3 +412,86 @@ def diff_views(num_views=32, num_tris=200_000, width=512, height=512, channels=8
     """The differentiable workload: `num_views` independent views."""
     n = max(1, int(num_tris * scale))
     return [diff_view(n, width, height, channels, seed=seed + 101 * k) for k in range(num_views)]
+
+
+def _mip_chain(rng, size, kind):
+    """Level 0 .. 1x1 of a size x size float4 texture (box-filtered)."""
+    y, x = np.mgrid[0:size, 0:size].astype(np.float32) / size
+    if kind == "checker":
+        c = ((np.floor(x * 8) + np.floor(y * 8)) % 2)[..., None]
+        base = rng.random((1, 1, 4), dtype=np.float32) * 0.5
+        tex = base + c * (1.0 - base) * rng.random((1, 1, 4), dtype=np.float32)
+        tex = tex + 0.1 * rng.random((size, size, 4), dtype=np.float32)
+    else:   # alpha: radial bands in the second channel
+        r = np.sqrt((x - 0.5) ** 2 + (y - 0.5) ** 2)
+        tex = np.zeros((size, size, 4), np.float32)
+        tex[..., 1] = 0.5 + 0.5 * np.cos(r * 40.0)
+        tex[..., 0] = rng.random((size, size), dtype=np.float32)
+    levels = [tex.astype(np.float32)]
+    while levels[-1].shape[0] > 1:
+        t = levels[-1]
+        levels.append((0.25 * (t[0::2, 0::2] + t[1::2, 0::2] + t[0::2, 1::2] + t[1::2, 1::2])).astype(np.float32))
+    return levels
+
+
+def texture_atlas(rng, size=32, kinds=("checker", "alpha")):
+    """An atlas holding one mip chain per kind, each in a 2*size x size
+    region (level l at x offset 2*size - 2*size/2^l), and the TextureSpec of
+    each (mip levels past the 1x1 level repeat it)."""
+    aw, ah = 2 * size * len(kinds), size
+    atlas = np.zeros((ah, aw, 4), np.float32)
+    specs = []
+    for k, kind in enumerate(kinds):
+        spec = np.zeros((), TEXSPEC_DTYPE)
+        spec["size"] = (size, size)
+        x0 = 2 * size * k
+        for l, lev in enumerate(_mip_chain(rng, size, kind)):
+            n = lev.shape[0]
+            atlas[0:n, x0:x0 + n] = lev
+            spec["mipLevels"][l] = (n / aw, n / ah, x0 / aw, 0.0)
+            x0 += n
+            last = l
+        for l in range(last + 1, 16):
+            spec["mipLevels"][l] = spec["mipLevels"][last]
+        specs.append(spec)
+    return atlas, specs
+
+
+def tex_phong(num_tris, width=256, height=192, samples_log2=0, seed=5, quads=False, area_range=(4.0, 3000.0),
+              blend=BLEND_REPLACE, flags=FLAG_DEPTH | FLAG_LERP, tex_scale=(0.05, 20.0), name="tex_phong") -> Scene:
+    """Textured Phong triangles (PixelPipe_texPhong, test/shader/Shaders.cu:
+    122-199) over the random_tris() layout: ShadedVertex_texPhong records
+    (clip position, camera position, camera normal, texture coordinate),
+    texture coordinates at log-uniform densities (mip levels 0..5 and
+    beyond), four materials -- diffuse texture, diffuse + alpha texture
+    (alpha-tested), untextured, untextured with alpha 0.3 (always
+    discarded) -- and a few triangles with material indices outside the
+    table (discarded)."""
+    rng = np.random.default_rng(seed)
+    base = random_tris(num_tris, width, height, samples_log2, seed=seed + 77, area_range=area_range)
+    n = base.num_tris
+    clip = base.vertices[:, :4].reshape(n, 3, 4)
+    w = clip[..., 3:4]
+    cam = np.concatenate([clip[..., :2], -w, np.ones_like(w)], axis=-1)
+    nrm = rng.normal(size=(n, 3, 3)).astype(np.float32)
+    nrm[..., 2] = -np.abs(nrm[..., 2]) - 0.3
+    nrm = np.concatenate([nrm, np.zeros((n, 3, 1), np.float32)], axis=-1)
+    scale = np.exp(np.log(tex_scale[0]) + (np.log(tex_scale[1]) - np.log(tex_scale[0])) * rng.random(n))
+    uv0 = rng.random((n, 1, 2)) * 4.0 - 2.0
+    uv = uv0 + rng.random((n, 3, 2)) * scale[:, None, None]
+    tc = np.concatenate([uv, np.zeros((n, 3, 1)), np.ones((n, 3, 1))], axis=-1)
+    verts = np.concatenate([clip, cam, nrm, tc], axis=-1).reshape(n * 3, 16).astype(np.float32)
+    atlas, (tex_a, tex_b) = texture_atlas(rng)
+    mats = np.zeros(4, MATERIAL_DTYPE)
+    mats["diffuseColor"] = [(0.8, 0.7, 0.6, 1.0), (0.5, 0.6, 0.9, 1.0), (0.3, 0.8, 0.4, 0.9), (0.9, 0.2, 0.2, 0.3)]
+    mats["specularColor"] = [(0.3, 0.3, 0.3), (0.6, 0.5, 0.4), (0.2, 0.2, 0.2), (0.5, 0.5, 0.5)]
+    mats["glossiness"] = [8.0, 32.0, 2.5, 16.0]
+    mats["diffuseTexture"][0] = tex_a
+    mats["diffuseTexture"][1] = tex_a
+    mats["alphaTexture"][1] = tex_b
+    tri_mat = rng.integers(0, 4, size=n).astype(np.int32)
+    bad = rng.random(n) < 0.01
+    tri_mat[bad] = rng.choice(np.array([-1, 4, 1000], np.int32), size=int(bad.sum()))
+    return Scene(name, width, height, samples_log2, flags | (FLAG_QUADS if quads else 0), blend, SHADER_TEXPHONG,
+                 np.ascontiguousarray(verts), base.indices, materials=mats, tri_material=tri_mat,
+                 atlas=np.ascontiguousarray(atlas))

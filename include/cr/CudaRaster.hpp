@@ -92,7 +92,14 @@ class CudaSurface {
 class CudaRaster {
  public:
   struct Stats { float setupTime, binTime, coarseTime, fineTime; };   // CudaRaster.hpp:45-51
-  struct DebugParams { bool disableHiZ = false; bool countFragments = false; int coarseThreads = 0; };
+  // DebugParams (CudaRaster.hpp:53-67) with this library's knobs; profilingMode
+  // takes the reference pipe's ProfilingMode role (CR_PROFILING_*).
+  struct DebugParams {
+    bool disableHiZ = false;
+    bool countFragments = false;
+    int coarseThreads = 0;
+    int profilingMode = CR_PROFILING_DEFAULT;
+  };
 
   CudaRaster() { check(cr_context_create(&ctx_), nullptr); }
   ~CudaRaster() { if (ctx_) cr_context_destroy(ctx_); }
@@ -156,7 +163,7 @@ class CudaRaster {
     return c;
   }
   std::string getProfilingInfo() {
-    std::string buf(4096, '\0');
+    std::string buf(8192, '\0');
     check(cr_get_profiling_info(ctx_, &buf[0], buf.size()), ctx_);
     buf.resize(buf.find('\0'));
     return buf;
@@ -166,7 +173,16 @@ class CudaRaster {
     d.disable_hiz = p.disableHiZ ? 1 : 0;
     d.count_fragments = p.countFragments ? 1 : 0;
     d.coarse_threads = p.coarseThreads;
+    d.profiling_mode = p.profilingMode;
     check(cr_set_debug_params(ctx_, &d), ctx_);
+  }
+  // The texPhong shader's materials, per-triangle material indices and
+  // texture atlas (what the demo puts in c_constants / t_textureAtlas).
+  void setShaderConstants(const cr_shader_constants& c) { check(cr_set_shader_constants(ctx_, &c), ctx_); }
+  cr_profile_counters getProfileCounters() {
+    cr_profile_counters c;
+    check(cr_get_profile_counters(ctx_, &c), ctx_);
+    return c;
   }
 
   cr_context* handle() const { return ctx_; }

@@ -58,7 +58,7 @@ typedef enum cr_format {
 enum {
     CR_RENDER_DEPTH = 1 << 0,
     CR_RENDER_LERP = 1 << 1,
-    CR_RENDER_QUADS = 1 << 2     /* accepted only as 0: numerical derivatives are not built */
+    CR_RENDER_QUADS = 1 << 2     /* numerical derivatives over 2x2 pixel quads (texPhong's texture LOD) */
 };
 
 /* Stock fragment shaders.  GOURAUD = GouraudShader (PixelPipe.inl:65-69):
@@ -67,8 +67,53 @@ enum {
  * is CONSTANT with 0xFF0000FF). */
 typedef enum cr_fragment_shader {
     CR_SHADER_GOURAUD = 0,
-    CR_SHADER_CONSTANT = 1
+    CR_SHADER_CONSTANT = 1,
+    CR_SHADER_TEX_PHONG = 2      /* FragmentShader_texPhong (test/shader/Shaders.cu:122-199), below */
 } cr_fragment_shader;
+
+/* ---- the texPhong shader (test/shader/Shaders.cu:37-48, 122-199) ----
+ * Vertex records (ShadedVertex_texPhong, Shaders.hpp:84-90): clip position,
+ * then varying 0 camera position, varying 1 camera normal, varying 2
+ * texture coordinate (x, y used): vertex_struct_size >= 64.  Per fragment:
+ * camera position and normal at the coverage centroid; the texture
+ * coordinate at the centroid with analytic derivatives, or, with
+ * CR_RENDER_QUADS, at the pixel centre with numerical derivatives over the
+ * 2x2 pixel quad; the triangle's material (an index outside the table
+ * discards the fragment); the diffuse colour from the diffuse texture and
+ * alpha from the alpha texture's second channel when their size[0] != 0,
+ * each a lerp of two bilinear lookups at the mip level the derivatives
+ * select; alpha < 0.5 discards (no depth or colour write); Phong lighting
+ * with the camera at the origin.  Arithmetic is IEEE FP32 with this
+ * library's own log2/exp2 (the reference's -use_fast_math approximations
+ * are not reproduced) and FP32 bilinear filtering with wrap addressing. */
+
+/* TextureSpec (Shaders.hpp:33-38): a texture's place in the atlas. */
+typedef struct cr_texture_spec {
+    float size[2];               /* level-0 size in texels; size[0] == 0: no texture */
+    uint32_t pad[2];
+    float mip_levels[16][4];     /* per level: scaleX, scaleY, biasX, biasY (normalised atlas coordinates) */
+} cr_texture_spec;
+
+/* Material (Shaders.hpp:44-51). */
+typedef struct cr_material {
+    float diffuse_color[4];
+    float specular_color[3];
+    float glossiness;
+    cr_texture_spec diffuse_texture;
+    cr_texture_spec alpha_texture;
+} cr_material;
+
+/* The shader's constants (Shaders.hpp:57-65 Constants::materials and
+ * triangleMaterialIdx, and t_textureAtlas); all device pointers that must
+ * stay valid while draws use them. */
+typedef struct cr_shader_constants {
+    const cr_material* materials;            /* num_materials records, 16-byte aligned */
+    int32_t num_materials;
+    int32_t atlas_width, atlas_height;       /* texels, 1..16384 */
+    const int32_t* triangle_material_idx;    /* material of input triangle t */
+    int64_t num_triangle_materials;          /* entries (triangles beyond them are discarded) */
+    const float* atlas;                      /* atlas_height x atlas_width float4 texels, 16-byte aligned */
+} cr_shader_constants;
 
 /* Stock blend shaders (PixelPipe.hpp:145-175, PixelPipe.inl:73-85). */
 typedef enum cr_blend_shader {
@@ -114,6 +159,15 @@ typedef struct cr_counters {
                                     counted only by draws with count_fragments set, else 0 */
 } cr_counters;
 
+/* ProfilingMode (cudaraster/cuda/PrivateDefs.hpp:158-204, CudaRaster.cpp:296-323):
+ * DEFAULT draws at full speed and reports stage times and queue sizes;
+ * COUNTERS draws with the counting kernel variants and records the profile
+ * counters below (cr_get_profile_counters, cr_get_profiling_info). */
+enum {
+    CR_PROFILING_DEFAULT = 0,
+    CR_PROFILING_COUNTERS = 1
+};
+
 /* CudaRaster::DebugParams (CudaRaster.hpp:53-67) has per-stage host
  * emulation; this library has one GPU implementation per stage and keeps
  * the knobs that change execution but never the image. */
@@ -121,7 +175,50 @@ typedef struct cr_debug_params {
     int32_t disable_hiz;         /* 1: no tile-level early depth rejection */
     int32_t count_fragments;     /* 1: count fragments exactly (implies disable_hiz; slower) */
     int32_t coarse_threads;      /* coarse-stage workgroup size: 256, 1024, or 0 to choose from the draw */
+    int32_t profiling_mode;      /* CR_PROFILING_* */
 } cr_debug_params;
+
+/* Profile counters of the last CR_PROFILING_COUNTERS draw, the reference's
+ * CR_PROFILING_COUNTERS table (PrivateDefs.hpp:158-204) mapped onto this
+ * pipeline: a numerator and a denominator each; the report prints
+ * num / max(denom, 1).  Percentages count 100 per event.  Setup counters are
+ * the reference's own; bin, coarse and fine counters describe this
+ * library's stages (per-chunk binning, per-bin coarse rounds, one wave per
+ * tile) where the reference's describe its own. */
+typedef enum cr_profile_counter {
+    CR_PROF_SETUP_VIEWPORT_CULL = 0,      /* % of triangles frustum-culled */
+    CR_PROF_SETUP_BACKFACE_CULL,          /* % backfacing (area <= 0) */
+    CR_PROF_SETUP_BETWEEN_PIXELS_CULL,    /* % covering no sample by the box test */
+    CR_PROF_SETUP_CLIPPED,                /* % taking the clip path */
+    CR_PROF_SETUP_SAMPLES_PER_TRI,        /* covered samples / triangles with a subtriangle */
+    CR_PROF_BIN_QUEUE_OVERFLOWS,          /* capacity grow-and-retry rounds of the draw */
+    CR_PROF_BIN_TRIS_PER_CHUNK,           /* subtriangles / 1024-triangle setup chunk */
+    CR_PROF_BIN_TRI_BB_AREA,              /* bins / subtriangle (bounding box) */
+    CR_PROF_BIN_TRI_SINGLE_BIN,           /* % of subtriangles inside one bin */
+    CR_PROF_COARSE_BINS,                  /* bins with entries */
+    CR_PROF_COARSE_ROUNDS_PER_BIN,        /* rounds (one entry per lane) / bin */
+    CR_PROF_COARSE_TRIS_PER_ROUND,        /* bin entries / round */
+    CR_PROF_COARSE_TILES_PER_ROUND,       /* tiles receiving entries / round */
+    CR_PROF_COARSE_EMITS_PER_ROUND,       /* tile entries / round */
+    CR_PROF_COARSE_EMITS_PER_TRI,         /* tile entries / bin entry */
+    CR_PROF_COARSE_CASE_A,                /* % of entries walked one lane per box (boxes <= 2x2 tiles) */
+    CR_PROF_COARSE_CASE_B,                /* % walked over the wave's union box */
+    CR_PROF_COARSE_CASE_C,                /* % walked per lane with large boxes by the wave */
+    CR_PROF_FINE_TRI_PER_TILE,            /* tile entries / tile with entries */
+    CR_PROF_FINE_EARLY_Z_CULL,            /* % of tile entries rejected by hierarchical Z */
+    CR_PROF_FINE_EMPTY_CULL,              /* % of walked entries covering no sample of the tile */
+    CR_PROF_FINE_Z_KILL,                  /* % of fragments whose samples all fail the depth test */
+    CR_PROF_FINE_WARP_UTIL,               /* % of lanes with a fragment per walked entry */
+    CR_PROF_FINE_BLEND_ROUNDS,            /* ROP rounds / written fragment (always 1: in order per lane) */
+    CR_PROF_FINE_FRAG_PER_TRI,            /* fragments / walked entry */
+    CR_PROF_FINE_FRAG_PER_TILE,           /* fragments / tile with entries */
+    CR_PROF_COUNT
+} cr_profile_counter;
+
+typedef struct cr_profile_counters {
+    int64_t num[CR_PROF_COUNT];
+    int64_t denom[CR_PROF_COUNT];
+} cr_profile_counters;
 
 /* ---- context (CudaRaster ctor + init, CudaRaster.cpp:98-128) ---- */
 cr_status cr_context_create(cr_context** out);
@@ -157,10 +254,15 @@ cr_status cr_deferred_clear(cr_context* ctx, const float rgba[4], float depth);
 cr_status cr_set_pixel_pipe(cr_context* ctx, const cr_pipe_desc* pipe);
 /* Stock pipe by name, as the demo's CR_DEFINE_PIXEL_PIPE names:
  * "PixelPipe_passthrough" (constant red, 16-byte vertices) and
- * "PixelPipe_gouraud" (GouraudVertex, GouraudShader); blend, samples and
+ * "PixelPipe_gouraud" (GouraudVertex, GouraudShader) and "PixelPipe_texPhong"
+ * (ShadedVertex_texPhong, FragmentShader_texPhong); blend, samples and
  * flags as the demo's -D defines (test/SceneCR.cpp:165-177). */
 cr_status cr_pipe_lookup(const char* name, int32_t blend_shader, int32_t samples_log2,
                          uint32_t render_mode_flags, cr_pipe_desc* out);
+/* Sets (or, with NULL, clears) the texPhong shader's constants: what the
+ * demo writes into c_constants and binds to t_textureAtlas before drawing
+ * with PixelPipe_texPhong. */
+cr_status cr_set_shader_constants(cr_context* ctx, const cr_shader_constants* constants);
 /* CudaRaster::setVertexBuffer (CudaRaster.cpp:220-224): device pointer +
  * byte offset; num_vertices bounds the indices (out-of-range triangles are
  * dropped). */
@@ -202,8 +304,14 @@ cr_status cr_draw_triangles_noalloc(cr_context* ctx, void* stream);
 /* CudaRaster::getStats (CudaRaster.cpp:346-363). */
 cr_status cr_get_stats(cr_context* ctx, cr_stats* out);
 cr_status cr_get_counters(cr_context* ctx, cr_counters* out);
-/* CudaRaster::getProfilingInfo (CudaRaster.cpp:367-497): text report. */
+/* CudaRaster::getProfilingInfo (CudaRaster.cpp:367-497): text report in
+ * the reference's layout -- stage times and queue sizes for
+ * CR_PROFILING_DEFAULT, the counter table for CR_PROFILING_COUNTERS. */
 cr_status cr_get_profiling_info(cr_context* ctx, char* buf, size_t len);
+/* The profile counters of the last draw (all zero unless it was drawn with
+ * CR_PROFILING_COUNTERS); the numbers behind getProfilingInfo's counter
+ * table (CudaRaster.cpp:424-447). */
+cr_status cr_get_profile_counters(cr_context* ctx, cr_profile_counters* out);
 /* CudaRaster::setDebugParams (CudaRaster.cpp:501-504). */
 cr_status cr_set_debug_params(cr_context* ctx, const cr_debug_params* p);
 
@@ -308,6 +416,14 @@ cr_status cr_batch_get_view_counters(const cr_batch* batch, int32_t view, cr_cou
 cr_status cr_interpolate(cr_context* ctx, const cr_surface* tri_id, const float* pos, int64_t num_vertices,
                          const float* attr, int32_t num_channels, const int32_t* tris, int32_t num_tris,
                          float* out, int32_t* tri_image, void* stream);
+/* Forward from a triangle-id image the caller holds (int32 [height][width],
+ * row 0 at the bottom, -1 or any id >= num_tris where empty -- e.g. the
+ * tri_image an earlier cr_interpolate wrote): the same out as cr_interpolate
+ * over the surface those ids were read from, whatever the context has
+ * drawn since. */
+cr_status cr_interpolate_ids(cr_context* ctx, const int32_t* tri_image, int32_t width, int32_t height,
+                             const float* pos, int64_t num_vertices, const float* attr, int32_t num_channels,
+                             const int32_t* tris, int32_t num_tris, float* out, void* stream);
 /* Backward: overwrites grad_pos[V][4] (z component 0) and grad_attr[V][C]
  * with dL/dpos and dL/dattr given grad_out[y][x][c] = dL/dout.  The terms
  * of each triangle's pixels within a 16x16 tile are summed in FP64 and added

@@ -712,6 +712,149 @@ int64_t cro_setup(const cro_pipe* P, const void* vertices, int64_t numVertices,
 }
 
 /* ------------------------------------------------------------------ */
+/* texPhong shader (test/shader/Shaders.cu:37-48, 122-199)             */
+/* ------------------------------------------------------------------ */
+
+static inline float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* log2 by the atanh series: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+ * ln m = 2 atanh(t), t = (m-1)/(m+1), |t| <= 0.1716, atanh t = t (1 + t^2/3
+ * + t^4/5 + t^6/7 + t^8/9) (the next term is below 2^-30 relative). */
+float cro_log2f(float x)
+{
+    if (isnan(x) || x < 0.0f) return NAN;
+    if (x == 0.0f) return -INFINITY;
+    if (isinf(x)) return INFINITY;
+    int e = 0;
+    uint32_t u = fbits(x);
+    if ((u >> 23) == 0) {                 /* subnormal: scale by 2^23 (exact) */
+        u = fbits(x * 8388608.0f);
+        e = -23;
+    }
+    e += (int)(u >> 23) - 127;
+    float m = bitsf((u & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 1.41421354f) { m *= 0.5f; e += 1; }
+    float t = (m - 1.0f) / (m + 1.0f);
+    float t2 = t * t;
+    float p = fmaf(t2, 0.111111112f, 0.142857149f);
+    p = fmaf(t2, p, 0.200000003f);
+    p = fmaf(t2, p, 0.333333343f);
+    p = fmaf(t2, p, 1.0f);
+    float ln = (2.0f * t) * p;
+    return fmaf(ln, 1.44269502f, (float)e);
+}
+
+/* exp2 by range reduction y = n + f, |f| <= 1/2, and the Taylor series of
+ * e^z, z = f ln 2, to z^7/7! (next term below 2^-27), scaled by 2^n. */
+float cro_exp2f(float y)
+{
+    if (isnan(y)) return y;
+    if (y >= 128.0f) return INFINITY;
+    if (y < -150.0f) return 0.0f;
+    float n = rintf(y);
+    float f = y - n;
+    float z = f * 0.693147182f;
+    float p = fmaf(z, 1.98412701e-04f, 1.38888892e-03f);
+    p = fmaf(z, p, 8.33333377e-03f);
+    p = fmaf(z, p, 4.16666679e-02f);
+    p = fmaf(z, p, 1.66666672e-01f);
+    p = fmaf(z, p, 0.5f);
+    p = fmaf(z, p, 1.0f);
+    p = fmaf(z, p, 1.0f);
+    return ldexpf(p, (int)n);
+}
+
+float cro_powf(float x, float y)
+{
+    if (y == 0.0f) return 1.0f;
+    if (x == 0.0f) return y > 0.0f ? 0.0f : INFINITY;
+    return cro_exp2f(y * cro_log2f(x));
+}
+
+/* Integer texel index of floor value f, wrapped into [0, n); NaN and
+ * out-of-range values give 0. */
+static inline int32_t wrap_index(float f, int32_t n)
+{
+    if (!(fabsf(f) < 1073741824.0f)) return 0;
+    int32_t k = (int32_t)f % n;
+    return k < 0 ? k + n : k;
+}
+
+/* tex2D with normalised coordinates, wrap addressing and bilinear
+ * filtering in FP32: texel space x = u W - 1/2, weights from the fraction,
+ * x first, then y. */
+void cro_tex_sample(const float* atlas, int32_t w, int32_t h, float u, float v, float out[4])
+{
+    float x = fmaf(u, (float)w, -0.5f), y = fmaf(v, (float)h, -0.5f);
+    float fx = floorf(x), fy = floorf(y);
+    float a = x - fx, b = y - fy;
+    int32_t i0 = wrap_index(fx, w), j0 = wrap_index(fy, h);
+    int32_t i1 = i0 + 1 == w ? 0 : i0 + 1, j1 = j0 + 1 == h ? 0 : j0 + 1;
+    const float* t00 = atlas + 4 * ((int64_t)j0 * w + i0);
+    const float* t10 = atlas + 4 * ((int64_t)j0 * w + i1);
+    const float* t01 = atlas + 4 * ((int64_t)j1 * w + i0);
+    const float* t11 = atlas + 4 * ((int64_t)j1 * w + i1);
+    for (int c = 0; c < 4; c++) {
+        float top = fmaf(t10[c] - t00[c], a, t00[c]);
+        float bot = fmaf(t11[c] - t01[c], a, t01[c]);
+        out[c] = fmaf(bot - top, b, top);
+    }
+}
+
+/* texture2D (Shaders.cu:125-148): sqr(a) + sqr(b) fused as fma(a, a, b*b);
+ * lerp(v0, v1, t) = v0 (1 - t) + v1 t (base/Math.hpp:114) fused on the left. */
+void cro_texture2d(const cro_texture_spec* spec, const float* atlas, int32_t w, int32_t h,
+                   const float tex[2], const float texDX[2], const float texDY[2], float out[4], float* lodOut)
+{
+    float ax = texDX[0] * spec->size[0], ay = texDX[1] * spec->size[1];
+    float bx = texDY[0] * spec->size[0], by = texDY[1] * spec->size[1];
+    float dxlen = fmaf(ax, ax, ay * ay);
+    float dylen = fmaf(bx, bx, by * by);
+    float lod = fminf(fmaxf(cro_log2f(fmaxf(dxlen, dylen)) * 0.5f, 0.0f), 14.0f);
+    int li = (int)lod;
+    const float* m0 = spec->mipLevels[li];
+    const float* m1 = spec->mipLevels[li + 1];
+    float tx = tex[0] - floorf(tex[0]);
+    float ty = tex[1] - floorf(tex[1]);
+    float v0[4], v1[4];
+    cro_tex_sample(atlas, w, h, fmaf(tx, m0[0], m0[2]), fmaf(ty, m0[1], m0[3]), v0);
+    cro_tex_sample(atlas, w, h, fmaf(tx, m1[0], m1[2]), fmaf(ty, m1[1], m1[3]), v1);
+    float f = lod - (float)li;
+    float g = 1.0f - f;
+    for (int c = 0; c < 4; c++) out[c] = fmaf(v0[c], g, v1[c] * f);
+    if (lodOut) *lodOut = lod;
+}
+
+/* normalize (base/Math.hpp:140-142): v * (1 * rcp(sqrt(lenSqr))), rcp(0) = 0,
+ * lenSqr accumulated r += x*x from 0 (each step fused). */
+static void normalize3(const float v[3], float out[3])
+{
+    float l = fmaf(v[2], v[2], fmaf(v[1], v[1], v[0] * v[0]));
+    float r = sqrtf(l);
+    float k = r != 0.0f ? 1.0f / r : 0.0f;
+    for (int i = 0; i < 3; i++) out[i] = v[i] * k;
+}
+
+static inline float dot3(const float a[3], const float b[3])
+{
+    return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0]));
+}
+
+void cro_lighting(const float cameraPos[3], const float cameraNormal[3], const cro_material* m,
+                  const float diffuse[3], float out[3])
+{
+    float I[3], N[3], R[3];
+    normalize3(cameraPos, I);
+    normalize3(cameraNormal, N);
+    float dotIN = dot3(I, N);
+    float s2 = dotIN * 2.0f;
+    for (int i = 0; i < 3; i++) R[i] = fmaf(-N[i], s2, I[i]);
+    float diffuseCoef = fmaf(fmaxf(-dotIN, 0.0f), 0.75f, 0.25f);
+    float specularCoef = cro_powf(fmaxf(-dot3(I, R), 0.0f), m->glossiness);
+    for (int i = 0; i < 3; i++) out[i] = fmaf(diffuseCoef, diffuse[i], specularCoef * m->specularColor[i]);
+}
+
+/* ------------------------------------------------------------------ */
 /* Rasterization in submission order                                   */
 /* ------------------------------------------------------------------ */
 
@@ -734,35 +877,148 @@ typedef struct {
     int64_t nextBand;
     pthread_mutex_t lock;
     int64_t fragments;
+    int64_t coveredSamples;
+    int64_t zKills;
 } raster_job;
 
-static void shade(const raster_job* R, const cro_data* td, int64_t sampleX2, int64_t sampleY2,
-                  float outColor[4])
+/* computeBarys (FineRaster.inl:21-44): barycentrics at half-sample position
+ * (sx, sy) of the W/U/V planes and their change per pixel step;
+ * `(F32)p.x - u * (F32)w.x` is fused as fma(-u, w.x, p.x). */
+static void compute_barys(const cro_data* td, int s, int32_t sx, int32_t sy, float b[3], float bdx[3],
+                          float bdy[3])
 {
-    const cro_pipe* P = R->P;
-    float b[3];
-    if (P->renderModeFlags & CRO_FLAG_LERP) {
-        /* computeBarys (FineRaster.inl:21-44) */
-        int32_t sx = (int32_t)sampleX2, sy = (int32_t)sampleY2;
-        int32_t W = wadd(wadd(wmul(td->wx, sx), wmul(td->wy, sy)), td->wb);
-        int32_t U = wadd(wadd(wmul(td->ux, sx), wmul(td->uy, sy)), td->ub);
-        int32_t V = wadd(wadd(wmul(td->vx, sx), wmul(td->vy, sy)), td->vb);
-        float w = 1.0f / (float)W;
-        float u = w * (float)U;
-        float v = w * (float)V;
-        b[0] = 1.0f - u - v; b[1] = u; b[2] = v;
-    } else {
-        b[0] = 0.0f; b[1] = 0.0f; b[2] = 1.0f;   /* FineRaster.inl:72-81 */
-    }
-    const float* c0 = (const float*)(R->verts + (int64_t)td->vi0 * P->vertexStride + 16);
-    const float* c1 = (const float*)(R->verts + (int64_t)td->vi1 * P->vertexStride + 16);
-    const float* c2 = (const float*)(R->verts + (int64_t)td->vi2 * P->vertexStride + 16);
-    /* interpolateVarying (PixelPipe.inl:52-59): v0*b.x + v1*b.y + v2*b.z */
-    for (int k = 0; k < 4; k++)
-        outColor[k] = fmaf(c2[k], b[2], fmaf(c0[k], b[0], c1[k] * b[1]));
+    int32_t W = wadd(wadd(wmul(td->wx, sx), wmul(td->wy, sy)), td->wb);
+    int32_t U = wadd(wadd(wmul(td->ux, sx), wmul(td->uy, sy)), td->ub);
+    int32_t V = wadd(wadd(wmul(td->vx, sx), wmul(td->vy, sy)), td->vb);
+    float w = 1.0f / (float)W;
+    float u = w * (float)U;
+    float v = w * (float)V;
+    b[0] = 1.0f - u - v; b[1] = u; b[2] = v;
+    float wd = w * (float)(1 << (s + 1));
+    float udx = wd * fmaf(-u, (float)td->wx, (float)td->ux);
+    float udy = wd * fmaf(-u, (float)td->wy, (float)td->uy);
+    float vdx = wd * fmaf(-v, (float)td->wx, (float)td->vx);
+    float vdy = wd * fmaf(-v, (float)td->wy, (float)td->vy);
+    bdx[0] = -udx - vdx; bdx[1] = udx; bdx[2] = vdx;
+    bdy[0] = -udy - vdy; bdy[1] = udy; bdy[2] = vdy;
 }
 
-static void raster_subtri(raster_job* R, int64_t slot, int32_t y0, int32_t y1, int64_t* frags)
+/* interpolateVarying (PixelPipe.inl:52-59): v0*b.x + v1*b.y + v2*b.z,
+ * fused as fma(v2, b.z, fma(v0, b.x, v1*b.y)). */
+static void interp_varying(const raster_job* R, const cro_data* td, int k, const float b[3], float out[4])
+{
+    const int64_t stride = R->P->vertexStride;
+    const float* c0 = (const float*)(R->verts + (int64_t)td->vi0 * stride + 16 + 16 * k);
+    const float* c1 = (const float*)(R->verts + (int64_t)td->vi1 * stride + 16 + 16 * k);
+    const float* c2 = (const float*)(R->verts + (int64_t)td->vi2 * stride + 16 + 16 * k);
+    for (int c = 0; c < 4; c++) out[c] = fmaf(c2[c], b[2], fmaf(c0[c], b[0], c1[c] * b[1]));
+}
+
+/* Texture coordinate (varying 2) at the centre of pixel (px, py). */
+static void center_tex(const raster_job* R, const cro_data* td, int64_t px, int64_t py, float out[2])
+{
+    int s = R->P->samplesLog2, n = 1 << s;
+    float b[3], bdx[3], bdy[3], t[4];
+    compute_barys(td, s, (int32_t)(px * n * 2 + n), (int32_t)(py * n * 2 + n), b, bdx, bdy);
+    interp_varying(R, td, 2, b, t);
+    out[0] = t[0]; out[1] = t[1];
+}
+
+/* FragmentShader_texPhong::run (Shaders.cu:152-199).  Returns 1 when the
+ * fragment is discarded (alpha test, or a material index outside the
+ * material table). */
+static int shade_tex_phong(const raster_job* R, const cro_data* td, uint32_t tri, int64_t px, int64_t py,
+                           int ctr, uint32_t* color)
+{
+    const cro_pipe* P = R->P;
+    int s = P->samplesLog2, n = 1 << s;
+    int lerp = (P->renderModeFlags & CRO_FLAG_LERP) != 0;
+    float cb[3] = {0.0f, 0.0f, 1.0f}, cdx[3] = {0, 0, 0}, cdy[3] = {0, 0, 0};   /* centroid */
+    if (lerp) {
+        int64_t sx2 = px * n * 2 + ((ctr == -1) ? n : kPattern[s][ctr] * 2 + 1);
+        int64_t sy2 = py * n * 2 + ((ctr == -1) ? n : ctr * 2 + 1);
+        compute_barys(td, s, (int32_t)sx2, (int32_t)sy2, cb, cdx, cdy);
+    }
+    float cp[4], cn[4], t[4];
+    interp_varying(R, td, 0, cb, cp);
+    interp_varying(R, td, 1, cb, cn);
+    float tex[2], texDX[2], texDY[2];
+    if (!(P->renderModeFlags & CRO_FLAG_QUADS)) {
+        /* Centroid sample, analytic derivatives. */
+        interp_varying(R, td, 2, cb, t);
+        tex[0] = t[0]; tex[1] = t[1];
+        interp_varying(R, td, 2, cdx, t);
+        texDX[0] = t[0]; texDX[1] = t[1];
+        interp_varying(R, td, 2, cdy, t);
+        texDY[0] = t[0]; texDY[1] = t[1];
+    } else if (!lerp) {
+        /* Varyings of vertex 2 everywhere: no change across the quad. */
+        float z[3] = {0.0f, 0.0f, 1.0f};
+        interp_varying(R, td, 2, z, t);
+        tex[0] = t[0]; tex[1] = t[1];
+        texDX[0] = texDX[1] = texDY[0] = texDY[1] = 0.0f;
+    } else {
+        /* Pixel centre, numerical derivatives over the 2x2 quad: dFdx is the
+         * odd column's value minus the even column's, dFdy likewise for rows
+         * (PixelPipe.hpp:59-60); the quad partners run the same shader at
+         * their own pixel centres. */
+        float a[2], b[2];
+        center_tex(R, td, px, py, tex);
+        center_tex(R, td, px & ~(int64_t)1, py, a);
+        center_tex(R, td, px | 1, py, b);
+        texDX[0] = b[0] - a[0]; texDX[1] = b[1] - a[1];
+        center_tex(R, td, px, py & ~(int64_t)1, a);
+        center_tex(R, td, px, py | 1, b);
+        texDY[0] = b[0] - a[0]; texDY[1] = b[1] - a[1];
+    }
+    int32_t mi = P->triangleMaterialIdx ? P->triangleMaterialIdx[tri] : -1;
+    if (mi < 0 || mi >= P->numMaterials) return 1;
+    const cro_material* M = &P->materials[mi];
+    float dc[4] = {M->diffuseColor[0], M->diffuseColor[1], M->diffuseColor[2], M->diffuseColor[3]};
+    if (M->diffuseTexture.size[0] != 0.0f) {
+        cro_texture2d(&M->diffuseTexture, P->atlas, P->atlasWidth, P->atlasHeight, tex, texDX, texDY, t, NULL);
+        dc[0] = t[0]; dc[1] = t[1]; dc[2] = t[2];
+    }
+    if (M->alphaTexture.size[0] != 0.0f) {
+        cro_texture2d(&M->alphaTexture, P->atlas, P->atlasWidth, P->atlasHeight, tex, texDX, texDY, t, NULL);
+        dc[3] = t[1];
+    }
+    if (dc[3] < 0.5f) return 1;
+    float lit[4];
+    cro_lighting(cp, cn, M, dc, lit);
+    lit[3] = dc[3];
+    *color = cro_to_abgr(lit);
+    return 0;
+}
+
+/* Fragment shader at pixel (px, py) with coverage centroid ctr (-1: pixel
+ * centre).  Returns 1 when the fragment is discarded. */
+static int shade_fragment(const raster_job* R, const cro_data* td, uint32_t tri, int64_t px, int64_t py, int ctr,
+                          uint32_t* color)
+{
+    const cro_pipe* P = R->P;
+    if (P->shader == CRO_SHADER_CONSTANT) {
+        *color = P->constColor;
+        return 0;
+    }
+    if (P->shader == CRO_SHADER_TEXPHONG) return shade_tex_phong(R, td, tri, px, py, ctr, color);
+    /* GouraudShader (PixelPipe.inl:65-69) at the centroid. */
+    int s = P->samplesLog2, n = 1 << s;
+    float b[3] = {0.0f, 0.0f, 1.0f}, bdx[3], bdy[3];   /* no lerp: vertex 2 (FineRaster.inl:72-81) */
+    if (P->renderModeFlags & CRO_FLAG_LERP) {
+        int64_t sx2 = px * n * 2 + ((ctr == -1) ? n : kPattern[s][ctr] * 2 + 1);
+        int64_t sy2 = py * n * 2 + ((ctr == -1) ? n : ctr * 2 + 1);
+        compute_barys(td, s, (int32_t)sx2, (int32_t)sy2, b, bdx, bdy);
+    }
+    float c[4];
+    interp_varying(R, td, 0, b, c);
+    *color = cro_to_abgr(c);
+    return 0;
+}
+
+/* frags[0]: fragments, frags[1]: covered samples, frags[2]: fragments whose
+ * covered samples all failed the depth test. */
+static void raster_subtri(raster_job* R, int64_t slot, int32_t y0, int32_t y1, int64_t frags[3])
 {
     const cro_pipe* P = R->P;
     const cro_header* h = &R->hdr[slot];
@@ -790,36 +1046,33 @@ static void raster_subtri(raster_job* R, int64_t slot, int32_t y0, int32_t y1, i
         for (int64_t px = pxlo; px <= pxhi; px++) {
             uint32_t cover = pixel_mask(&e, s, (int32_t)px, (int32_t)py);
             if (!cover) continue;
-            (*frags)++;
+            frags[0]++;
+            frags[1] += __builtin_popcount(cover);
             /* texel of sample i: row py, column tileX*8N + i*8 + (px&7) */
             int64_t base = py * pitch + (px >> 3) * 8 * n + (px & 7);
-            uint32_t write = 0;
+            uint32_t write = 0, dnew[8];
             for (int i = 0; i < n; i++) {
                 if (!(cover & (1u << i))) continue;
                 if (depthOn) {
                     uint32_t sampleX = (uint32_t)(px * n + kPattern[s][i]);
                     uint32_t sampleY = (uint32_t)(py * n + i);
                     uint32_t d = td->zx * sampleX + td->zy * sampleY + td->zb;
-                    uint32_t* dp = &R->depth[base + i * 8];
-                    if (d >= *dp) continue;   /* LESS (FineRaster.inl:697) */
-                    *dp = d;
+                    if (d >= R->depth[base + i * 8]) continue;   /* LESS (FineRaster.inl:697) */
+                    dnew[i] = d;
                 }
                 write |= 1u << i;
             }
-            if (!write) continue;
-            uint32_t src;
-            if (P->shader == CRO_SHADER_CONSTANT) {
-                src = P->constColor;
-            } else {
-                int ctr = cro_select_centroid(s, cover);
-                int64_t sx2 = px * n * 2 + ((ctr == -1) ? n : kPattern[s][ctr] * 2 + 1);
-                int64_t sy2 = py * n * 2 + ((ctr == -1) ? n : ctr * 2 + 1);
-                float c[4];
-                shade(R, td, sx2, sy2, c);
-                src = cro_to_abgr(c);
+            if (!write) {
+                frags[2]++;
+                continue;
             }
+            /* Shade, then write: a discarded fragment writes neither depth
+             * nor colour. */
+            uint32_t src;
+            if (shade_fragment(R, td, tri, px, py, cro_select_centroid(s, cover), &src)) continue;
             for (int i = 0; i < n; i++) {
                 if (!(write & (1u << i))) continue;
+                if (depthOn) R->depth[base + i * 8] = dnew[i];
                 uint32_t* cp = &R->color[base + i * 8];
                 if (P->blend != CRO_BLEND_DEPTHONLY) *cp = cro_blend(P->blend, src, *cp);
                 if (R->triid) R->triid[base + i * 8] = tri;
@@ -831,7 +1084,7 @@ static void raster_subtri(raster_job* R, int64_t slot, int32_t y0, int32_t y1, i
 static void* raster_worker(void* arg)
 {
     raster_job* R = (raster_job*)arg;
-    int64_t frags = 0;
+    int64_t frags[3] = {0, 0, 0};
     for (;;) {
         pthread_mutex_lock(&R->lock);
         int64_t band = R->nextBand++;
@@ -840,10 +1093,12 @@ static void* raster_worker(void* arg)
         int32_t y0 = (int32_t)band * BAND_ROWS;
         int32_t y1 = y0 + BAND_ROWS < R->hr ? y0 + BAND_ROWS : R->hr;
         for (int64_t k = R->bandStart[band]; k < R->bandStart[band + 1]; k++)
-            raster_subtri(R, R->order[R->bandList[k]], y0, y1, &frags);
+            raster_subtri(R, R->order[R->bandList[k]], y0, y1, frags);
     }
     pthread_mutex_lock(&R->lock);
-    R->fragments += frags;
+    R->fragments += frags[0];
+    R->coveredSamples += frags[1];
+    R->zKills += frags[2];
     pthread_mutex_unlock(&R->lock);
     return NULL;
 }
@@ -870,6 +1125,10 @@ int32_t cro_render(const cro_pipe* P, const void* vertices, int64_t numVertices,
         !color || !depth || numTris < 0)
         return -1;
     if (P->shader == CRO_SHADER_GOURAUD && P->vertexStride < 32) return -1;
+    if (P->shader == CRO_SHADER_TEXPHONG &&
+        (P->vertexStride < 64 || (P->numMaterials > 0 && (!P->materials || !P->triangleMaterialIdx)) ||
+         (P->atlas && (P->atlasWidth < 1 || P->atlasHeight < 1))))
+        return -1;
     int n = 1 << P->samplesLog2;
     int32_t wr = (P->width + 7) & ~7, hr = (P->height + 7) & ~7;
     int64_t texels = (int64_t)wr * n * hr;
@@ -890,8 +1149,11 @@ int32_t cro_render(const cro_pipe* P, const void* vertices, int64_t numVertices,
                               subtriTri, maxSlots, nthreads, cull);
 
     /* Submission order: triangle t, then its subtriangles in fan order. */
-    int64_t numSub = 0;
-    for (int64_t t = 0; t < numTris; t++) numSub += triSubtris[t];
+    int64_t numSub = 0, trisVisible = 0;
+    for (int64_t t = 0; t < numTris; t++) {
+        numSub += triSubtris[t];
+        trisVisible += triSubtris[t] != 0;
+    }
     int64_t* order = (int64_t*)malloc(sizeof(int64_t) * (size_t)(numSub + 1));
     int64_t k = 0;
     for (int64_t t = 0; t < numTris; t++) {
@@ -936,6 +1198,9 @@ int32_t cro_render(const cro_pipe* P, const void* vertices, int64_t numVertices,
         stats->culledBetween = cull[3];
         stats->clipped = cull[4] + cull[5];
         stats->fragments = R.fragments;
+        stats->coveredSamples = R.coveredSamples;
+        stats->zKills = R.zKills;
+        stats->trisVisible = trisVisible;
     }
     free(triSubtris); free(hdr); free(data); free(subtriTri); free(order);
     free(bandStart); free(fill); free(bandList);

@@ -36,6 +36,8 @@ extern "C" {
 enum {
     CRO_FLAG_DEPTH = 1,   /* RenderModeFlag_EnableDepth (PixelPipe.hpp:32) */
     CRO_FLAG_LERP  = 2,   /* RenderModeFlag_EnableLerp  (PixelPipe.hpp:33) */
+    CRO_FLAG_QUADS = 4,   /* RenderModeFlag_EnableQuads (PixelPipe.hpp:34): numerical
+                             derivatives over 2x2 pixel quads (PixelPipe.hpp:59-69) */
 };
 
 enum {
@@ -48,7 +50,23 @@ enum {
 enum {
     CRO_SHADER_GOURAUD  = 0,  /* PixelPipe.inl:65-69: toABGR(lerp(varying 0)) */
     CRO_SHADER_CONSTANT = 1,  /* test/shader/PassThrough.cu:44-54 style */
+    CRO_SHADER_TEXPHONG = 2,  /* FragmentShader_texPhong, test/shader/Shaders.cu:122-199 */
 };
+
+/* TextureSpec and Material (test/shader/Shaders.hpp:33-51), same layout. */
+typedef struct {
+    float    size[2];           /* level-0 texels; size[0] == 0: no texture */
+    uint32_t pad[2];
+    float    mipLevels[16][4];  /* scaleX, scaleY, biasX, biasY into the atlas */
+} cro_texture_spec;
+
+typedef struct {
+    float            diffuseColor[4];
+    float            specularColor[3];
+    float            glossiness;
+    cro_texture_spec diffuseTexture;
+    cro_texture_spec alphaTexture;
+} cro_material;
 
 typedef struct {
     int32_t  width;           /* viewport width in pixels (1..2048)  */
@@ -62,6 +80,12 @@ typedef struct {
     int32_t  deferredClear;   /* 1: clear before drawing */
     uint32_t clearColor;      /* ABGR8888 */
     uint32_t clearDepth;      /* encoded depth */
+    /* texPhong resources (Shaders.hpp:57-65 Constants, t_textureAtlas). */
+    int32_t  numMaterials;
+    int32_t  atlasWidth, atlasHeight;
+    const cro_material* materials;
+    const int32_t* triangleMaterialIdx;   /* per input triangle */
+    const float* atlas;                    /* atlasHeight x atlasWidth float4 texels */
 } cro_pipe;
 
 /* CRTriangleHeader (PrivateDefs.hpp:26-36). */
@@ -87,9 +111,36 @@ typedef struct {
     int64_t culledBetween;     /* prepareTriangle == 2, fast path */
     int64_t clipped;           /* took the clip path */
     int64_t fragments;         /* (subtriangle, pixel) pairs with >= 1 covered sample */
+    int64_t coveredSamples;    /* covered samples summed over those pairs (before the
+                                  depth test; SetupSamplesPerTri's numerator,
+                                  FineRaster.inl:684, 1032) */
+    int64_t zKills;            /* pairs whose covered samples all fail LESS
+                                  (FineZKill, FineRaster.inl:702, 1069) */
+    int64_t trisVisible;       /* input triangles with >= 1 subtriangle
+                                  (SetupSamplesPerTri's denominator, BinRaster.inl:96) */
 } cro_stats;
 
 /* ---- single rules (exported for the pinning tests) ---- */
+/* The texPhong shader's transcendental functions, written from published
+ * methods with IEEE-exact operations only (the reference compiles its
+ * shaders with nvcc -use_fast_math, whose approximations are not
+ * restated): log2 by the atanh series ln m = 2 atanh((m-1)/(m+1)) on the
+ * mantissa, exp2 by range reduction and the Taylor series of e^z, and
+ * pow(x, y) = exp2(y log2 x). */
+float    cro_log2f(float x);
+float    cro_exp2f(float y);
+float    cro_powf(float x, float y);
+/* Bilinear lookup of the float4 atlas at normalised (u, v), wrap addressing
+ * (tex2D on t_textureAtlas, Shaders.cu:140-141). */
+void     cro_tex_sample(const float* atlas, int32_t w, int32_t h, float u, float v, float out[4]);
+/* texture2D (Shaders.cu:125-148): LOD from the texture-coordinate
+ * derivatives, two bilinear lookups, lerp.  *lod receives the LOD. */
+void     cro_texture2d(const cro_texture_spec* spec, const float* atlas, int32_t w, int32_t h,
+                       const float tex[2], const float texDX[2], const float texDY[2],
+                       float out[4], float* lod);
+/* evaluateLighting (Shaders.cu:37-48). */
+void     cro_lighting(const float cameraPos[3], const float cameraNormal[3], const cro_material* m,
+                      const float diffuse[3], float out[3]);
 uint32_t cro_to_abgr(const float c[4]);            /* Util.inl:30-38 (device)   */
 uint32_t cro_to_abgr_host(const float c[4]);       /* Math.cpp:41-48 (host)     */
 uint32_t cro_blend(int32_t mode, uint32_t src, uint32_t dst);

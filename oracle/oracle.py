@@ -24,7 +24,10 @@ class Pipe(ctypes.Structure):
                 ("blend", ctypes.c_int32), ("shader", ctypes.c_int32),
                 ("constColor", ctypes.c_uint32), ("vertexStride", ctypes.c_int32),
                 ("deferredClear", ctypes.c_int32), ("clearColor", ctypes.c_uint32),
-                ("clearDepth", ctypes.c_uint32)]
+                ("clearDepth", ctypes.c_uint32), ("numMaterials", ctypes.c_int32),
+                ("atlasWidth", ctypes.c_int32), ("atlasHeight", ctypes.c_int32),
+                ("materials", ctypes.c_void_p), ("triangleMaterialIdx", ctypes.c_void_p),
+                ("atlas", ctypes.c_void_p)]
 
 
 class Header(ctypes.Structure):
@@ -43,7 +46,7 @@ class Data(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in ("numTris", "numSubtris", "culledFrustum",
                                               "culledBackface", "culledBetween", "clipped",
-                                              "fragments")]
+                                              "fragments", "coveredSamples", "zKills", "trisVisible")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -108,6 +111,18 @@ def lib():
         L.cro_render.argtypes = [ctypes.POINTER(Pipe), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                  ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_int32, ctypes.POINTER(Stats)]
+        for fn in ("cro_log2f", "cro_exp2f"):
+            getattr(L, fn).restype = ctypes.c_float
+            getattr(L, fn).argtypes = [ctypes.c_float]
+        L.cro_powf.restype = ctypes.c_float
+        L.cro_powf.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.cro_tex_sample.restype = None
+        L.cro_tex_sample.argtypes = [f4, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float, f4]
+        L.cro_texture2d.restype = None
+        L.cro_texture2d.argtypes = [ctypes.c_void_p, f4, ctypes.c_int32, ctypes.c_int32, f4, f4, f4, f4,
+                                    ctypes.POINTER(ctypes.c_float)]
+        L.cro_lighting.restype = None
+        L.cro_lighting.argtypes = [f4, f4, ctypes.c_void_p, f4, f4]
         L.cro_vertex_passthrough.restype = None
         L.cro_vertex_passthrough.argtypes = [f4, ctypes.c_int64, f4, f4, ctypes.c_int64]
         _lib = L
@@ -180,6 +195,17 @@ def make_pipe(scene, deferred_clear=None) -> Pipe:
     p.deferredClear = 1 if dc else 0
     p.clearColor = to_abgr_host(scene.clear_color)
     p.clearDepth = clear_depth(scene.clear_depth)
+    if getattr(scene, "materials", None) is not None:
+        # texPhong constants: the arrays stay referenced by the Pipe object.
+        mats = np.ascontiguousarray(scene.materials)
+        tri_mat = np.ascontiguousarray(scene.tri_material, dtype=np.int32)
+        atlas = np.ascontiguousarray(scene.atlas, dtype=np.float32)
+        p._keep = (mats, tri_mat, atlas)
+        p.numMaterials = mats.shape[0]
+        p.materials = mats.ctypes.data
+        p.triangleMaterialIdx = tri_mat.ctypes.data
+        p.atlas = atlas.ctypes.data
+        p.atlasHeight, p.atlasWidth = atlas.shape[0], atlas.shape[1]
     return p
 
 
@@ -265,4 +291,43 @@ def resolve(color, width, height, samples_log2):
     for ch in range(4):
         v = ((t >> np.uint32(8 * ch)) & np.uint32(0xFF)).astype(np.uint32).sum(axis=2)
         out |= ((v + np.uint32(n >> 1)) >> np.uint32(samples_log2)) << np.uint32(8 * ch)
+    return out
+
+
+def log2f(x) -> float:
+    return float(lib().cro_log2f(float(x)))
+
+
+def exp2f(y) -> float:
+    return float(lib().cro_exp2f(float(y)))
+
+
+def powf(x, y) -> float:
+    return float(lib().cro_powf(float(x), float(y)))
+
+
+def tex_sample(atlas, u, v):
+    a = np.ascontiguousarray(atlas, dtype=np.float32)
+    out = np.zeros(4, np.float32)
+    lib().cro_tex_sample(_fptr(a), a.shape[1], a.shape[0], float(u), float(v), _fptr(out))
+    return out
+
+
+def texture2d(spec, atlas, tex, tex_dx, tex_dy):
+    """texture2D of a TEXSPEC_DTYPE record; returns (rgba, lod)."""
+    a = np.ascontiguousarray(atlas, dtype=np.float32)
+    sp = np.ascontiguousarray(np.asarray(spec).reshape(1))
+    vs = [np.ascontiguousarray(v, dtype=np.float32) for v in (tex, tex_dx, tex_dy)]
+    out = np.zeros(4, np.float32)
+    lod = ctypes.c_float()
+    lib().cro_texture2d(sp.ctypes.data, _fptr(a), a.shape[1], a.shape[0], *(_fptr(v) for v in vs), _fptr(out),
+                        ctypes.byref(lod))
+    return out, float(lod.value)
+
+
+def lighting(camera_pos, camera_normal, material, diffuse):
+    m = np.ascontiguousarray(np.asarray(material).reshape(1))
+    vs = [np.ascontiguousarray(v, dtype=np.float32) for v in (camera_pos, camera_normal, diffuse)]
+    out = np.zeros(3, np.float32)
+    lib().cro_lighting(_fptr(vs[0]), _fptr(vs[1]), m.ctypes.data, _fptr(vs[2]), _fptr(out))
     return out

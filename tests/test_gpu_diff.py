@@ -181,3 +181,86 @@ def test_async_visibility_overflow_is_reported_then_recovers():
         assert np.array_equal(out.detach().cpu().numpy(), expect["out"])
     finally:
         dr.close()
+
+
+def test_render_uses_the_given_ids_not_the_last_draw():
+    """rasterize(A), rasterize(B), then render(A, ids_A): the image and the
+    gradients follow A's visibility although B was drawn last."""
+    torch = _torch()
+    import diff_ref
+    from cudaraster import scenes
+    from cudaraster.diff import DiffRasterizer
+    a, b = (scenes.diff_view(4000, 96, 80, channels=4, seed=s) for s in (71, 72))
+    dr = DiffRasterizer(96, 80)
+    try:
+        ta = {k: torch.from_numpy(getattr(a, k)).cuda() for k in ("pos", "attr", "tris")}
+        tb = {k: torch.from_numpy(getattr(b, k)).cuda() for k in ("pos", "tris")}
+        ids_a = dr.rasterize(ta["pos"], ta["tris"])
+        dr.rasterize(tb["pos"], tb["tris"])
+        pos = ta["pos"].clone().requires_grad_(True)
+        attr = ta["attr"].clone().requires_grad_(True)
+        out = dr.render(pos, attr, ta["tris"], ids_a)
+        out.square().sum().backward()
+        torch.cuda.synchronize()
+        ids = ids_a.cpu().numpy().astype(np.int64)
+        ref = diff_ref.interpolate(a.pos, a.attr, a.tris, ids)
+        assert np.abs(out.detach().cpu().numpy() - ref).max() < 1e-6
+        gp, ga = diff_ref.backward(a.pos, a.attr, a.tris, ids, 2.0 * ref)
+        assert np.abs(attr.grad.cpu().numpy() - ga).max() <= 1e-6 * np.abs(ga).max()
+        assert np.abs(pos.grad.cpu().numpy() - gp).max() <= 1e-6 * np.abs(gp).max()
+    finally:
+        dr.close()
+
+
+@pytest.mark.parametrize("channels,offset", [(3, 0), (5, 0), (8, 1), (4, 3)])
+def test_scalar_channel_path(channels, offset):
+    """Channel counts that are not a multiple of 4, and attribute / output
+    storage that is not 16-byte aligned, take the forward's per-channel path:
+    forward and gradients against the FP64 reference."""
+    torch = _torch()
+    import diff_ref
+    from cudaraster import scenes
+    from cudaraster.diff import DiffRasterizer
+    v = scenes.diff_view(3000, 80, 64, channels=channels, seed=80 + channels)
+    dr = DiffRasterizer(80, 64)
+    try:
+        pos = torch.from_numpy(v.pos).cuda().requires_grad_(True)
+        flat = torch.zeros(offset + v.attr.size, dtype=torch.float32, device="cuda")
+        flat[offset:] = torch.from_numpy(v.attr.reshape(-1)).cuda()
+        attr = flat[offset:].view(v.attr.shape).detach().requires_grad_(True)
+        assert attr.data_ptr() % 16 == (4 * offset) % 16
+        tris = torch.from_numpy(v.tris).cuda()
+        target = torch.from_numpy(v.target).cuda()
+        out = dr.render(pos, attr, tris)
+        ids = dr.rasterize(pos, tris).cpu().numpy().astype(np.int64)
+        ((out - target) ** 2).sum().backward()
+        torch.cuda.synchronize()
+        ref = diff_ref.interpolate(v.pos, v.attr, v.tris, ids)
+        assert np.abs(out.detach().cpu().numpy() - ref).max() < 1e-6
+        gp, ga = diff_ref.backward(v.pos, v.attr, v.tris, ids, 2.0 * (ref - v.target))
+        assert np.abs(attr.grad.cpu().numpy() - ga).max() <= 1e-6 * np.abs(ga).max()
+        assert np.abs(pos.grad.cpu().numpy() - gp).max() <= 1e-6 * np.abs(gp).max()
+    finally:
+        dr.close()
+
+
+def test_interpolation_rejects_wrong_buffers():
+    """Wrong dtypes, shapes or layouts are refused before a kernel sees them."""
+    torch = _torch()
+    from cudaraster import CrError, scenes
+    from cudaraster.diff import DiffRasterizer
+    v = scenes.diff_view(500, 32, 32, channels=4, seed=90)
+    dr = DiffRasterizer(32, 32)
+    try:
+        pos = torch.from_numpy(v.pos).cuda()
+        attr = torch.from_numpy(v.attr).cuda()
+        tris = torch.from_numpy(v.tris).cuda()
+        ids = dr.rasterize(pos, tris)
+        bad = [(pos, attr.to(torch.int32), tris, ids), (pos, attr, tris.to(torch.int64), ids),
+               (pos.double(), attr, tris, ids), (pos, attr, tris, ids.to(torch.float32)),
+               (pos, attr[:-1], tris, ids), (pos, attr, tris, ids[:, :16].contiguous())]
+        for p, a, t, i in bad:
+            with pytest.raises(CrError):
+                dr.render(p, a, t, i)
+    finally:
+        dr.close()

@@ -95,3 +95,25 @@ def test_random_frames_match_goldens(name):
     finally:
         r.close()
     assert np.array_equal(got, cudaraster.load_png(os.path.join(GOLDEN, name)))
+
+
+def test_resolve_rejects_wrong_outputs():
+    """resolve(out=...) checks size, dtype, layout and the surface format
+    before the library call."""
+    torch = _torch()
+    import cudaraster
+    col = cudaraster.CudaSurface((40, 24), cudaraster.FORMAT_RGBA8, 4)
+    dep = cudaraster.CudaSurface((40, 24), cudaraster.FORMAT_DEPTH32, 4)
+    try:
+        col.resolve(torch.empty((24, 40), dtype=torch.int32, device="cuda"))
+        for out in (torch.empty((24, 39), dtype=torch.int32, device="cuda"),
+                    torch.empty((24, 40), dtype=torch.float64, device="cuda"),
+                    torch.empty((40, 24), dtype=torch.int32, device="cuda").t(),
+                    torch.empty((24, 40), dtype=torch.int32)):
+            with pytest.raises(ValueError):
+                col.resolve(out)
+        with pytest.raises(ValueError):
+            dep.resolve()
+    finally:
+        col.close()
+        dep.close()

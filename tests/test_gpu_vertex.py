@@ -78,3 +78,25 @@ def test_cube_from_model_positions(oracle_lib):
     finally:
         r.close()
     assert_parity(got, ref, "cube through the vertex shader")
+
+
+def test_wrapper_rejects_undersized_buffers():
+    """vertexShaderPassthrough checks its buffers before the library call:
+    an output smaller than num_vertices records, fewer model positions than
+    vertices, or a wrong dtype raise ValueError and launch nothing."""
+    torch = _torch()
+    import cudaraster
+    cr = cudaraster.CudaRaster().init()
+    try:
+        m = np.eye(4, dtype=np.float32)
+        pos = torch.zeros(300, dtype=torch.float32, device="cuda")      # 100 vertices
+        out = torch.zeros(100 * 8, dtype=torch.float32, device="cuda")  # 100 records of 32 bytes
+        cr.vertexShaderPassthrough(pos, m, out, out_stride=32)
+        for kw in ({"out": out[:-1]}, {"num_vertices": 101}, {"model_pos": pos.double()},
+                   {"model_pos": pos.cpu()}, {"out_stride": 48}):
+            args = {"model_pos": pos, "out": out, "out_stride": 32, **kw}
+            with pytest.raises(ValueError):
+                cr.vertexShaderPassthrough(args["model_pos"], m, args["out"], out_stride=args["out_stride"],
+                                           num_vertices=args.get("num_vertices"))
+    finally:
+        cr.close()

@@ -567,7 +567,7 @@ def python_render(oracle_lib, scene):
     order = []
     for t, k in enumerate(tri_sub):
         order += [t] if k == 1 else [int(hdr[t]["misc"]) + j for j in range(k)]
-    frags = 0
+    frags = samples = zkills = 0
     pat = [oracle_lib.msaa_pattern(s, i) for i in range(n)]
     for slot in order:
         h, d = hdr[slot], dat[slot]
@@ -581,7 +581,9 @@ def python_render(oracle_lib, scene):
                 if not any(cov):
                     continue
                 frags += 1
+                samples += sum(cov)
                 col_base = (px >> 3) * 8 * n + (px & 7)
+                written = 0
                 for i in range(n):
                     if not cov[i]:
                         continue
@@ -590,6 +592,10 @@ def python_render(oracle_lib, scene):
                         continue
                     depth[py, col_base + 8 * i] = z
                     tid[py, col_base + 8 * i] = int(sub[slot])
+                    written += 1
+                zkills += written == 0
+    python_render.counts = {"fragments": frags, "coveredSamples": samples, "zKills": zkills,
+                            "trisVisible": sum(1 for k in tri_sub if k)}
     return depth.astype(np.uint32), tid.astype(np.uint32), frags
 
 
@@ -602,6 +608,42 @@ def test_oracle_equals_python_sequential_renderer(oracle_lib, s):
     assert np.array_equal(r["depth"], depth)
     assert np.array_equal(r["triid"], tid)
     assert r["stats"]["fragments"] == frags
+    for k, v in python_render.counts.items():
+        assert r["stats"][k] == v, k
+
+
+def test_profile_counts_closed_forms(oracle_lib):
+    """The profiling counts of the oracle (covered samples, depth-killed
+    fragments, triangles with a visible subtriangle) on scenes whose values
+    are fixed by construction: every triangle drawn twice in a row kills
+    every fragment of the second copy (LESS), and doubles the samples and
+    visible triangles; a full-viewport quad covers every sample once."""
+    from cudaraster import scenes
+    for s in (0, 1, 3):
+        sc = scenes.random_tris(80, 48, 40, samples_log2=s, seed=30 + s, area_range=(4.0, 300.0))
+        one = oracle_lib.render(sc)["stats"]
+        idx = np.repeat(sc.indices, 2, axis=0)
+        dup = scenes.Scene("dup", sc.width, sc.height, s, sc.flags, sc.blend, sc.shader, sc.vertices,
+                           np.ascontiguousarray(idx))
+        two = oracle_lib.render(dup)["stats"]
+        assert two["fragments"] == 2 * one["fragments"]
+        assert two["coveredSamples"] == 2 * one["coveredSamples"]
+        assert two["trisVisible"] == 2 * one["trisVisible"] == 2 * one["numSubtris"]
+        assert two["zKills"] == one["zKills"] + one["fragments"]
+        # Without the depth test nothing is killed.
+        nod = scenes.Scene("nodepth", sc.width, sc.height, s, scenes.FLAG_LERP, sc.blend, sc.shader,
+                           sc.vertices, np.ascontiguousarray(idx))
+        assert oracle_lib.render(nod)["stats"]["zKills"] == 0
+        # A quad over the whole (tile-rounded) surface: two triangles, each
+        # sample covered exactly once.
+        W, H = 48, 40
+        quad = np.array([[-1, -1, 0.5, 1], [1, -1, 0.5, 1], [1, 1, 0.5, 1], [-1, 1, 0.5, 1]], np.float32)
+        verts = np.concatenate([quad, np.ones((4, 4), np.float32)], axis=1)
+        q = scenes.Scene("quad", W, H, s, sc.flags, sc.blend, sc.shader, np.ascontiguousarray(verts),
+                         np.array([[0, 1, 2], [0, 2, 3]], np.int32))
+        st = oracle_lib.render(q)["stats"]
+        assert st["coveredSamples"] == W * H * (1 << s)
+        assert st["zKills"] == 0 and st["trisVisible"] == 2
 
 
 def test_depth_test_is_strictly_less(oracle_lib):
