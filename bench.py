@@ -65,7 +65,7 @@ def coarse_algorithmic_bytes(scene, c, visible_subtris):
     chunks = (scene.num_tris + 1023) // 1024
     bins = ((scene.width + 127) // 128) * ((scene.height + 127) // 128)
     tiles = ((scene.width + 7) // 8) * ((scene.height + 7) // 8)
-    return c["bin_entries"] * 16 + bins * chunks * 8 + c["tile_entries"] * 4 + tiles * 8
+    return c["bin_entries"] * 16 + bins * chunks * 8 + c["tile_list_entries"] * 4 + tiles * 8
 
 
 def measured_traffic(kernel, workload_key, triid):

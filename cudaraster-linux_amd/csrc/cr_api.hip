@@ -141,6 +141,7 @@ cr_status ensureBuffers(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   CR_HIP(ctx, ctx->tileStart.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileCount.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileList.ensure((size_t)ctx->maxTileEntries * 4));
+  CR_HIP(ctx, ctx->tileSegNext.ensure((size_t)std::max<int64_t>(ctx->maxTileEntries >> kTileSegLog2, 1) * 4));
   if (countingDraw(ctx)) {
     CR_HIP(ctx, ctx->tileFrags.ensure((size_t)g.nt * 4));
     CR_HIP(ctx, ctx->chunkSubtris.ensure((size_t)std::max(numChunks, 1) * 4));
@@ -169,6 +170,7 @@ DrawBuffers contextBuffers(cr_context* ctx) {
   B.tileStart = ctx->tileStart.as<uint32_t>();
   B.tileCount = ctx->tileCount.as<uint32_t>();
   B.tileList = ctx->tileList.as<uint32_t>();
+  B.tileSegNext = ctx->tileSegNext.as<uint32_t>();
   B.tileFrags = countingDraw(ctx) ? ctx->tileFrags.as<uint32_t>() : nullptr;
   B.chunkSubtris = countingDraw(ctx) ? ctx->chunkSubtris.as<uint32_t>() : nullptr;
   B.prof = profilingDraw(ctx) ? ctx->profBuf.as<unsigned long long>() : nullptr;
@@ -233,6 +235,7 @@ Params makeParams(cr_context* ctx, const Geometry& g, int32_t numChunks, const D
   P.tileCount = B.tileCount;
   P.maxTileEntries = (int32_t)B.maxTileEntries;
   P.tileList = B.tileList;
+  P.tileSegNext = B.tileSegNext;
   P.color = ctx->color->dev;
   P.depth = ctx->depth->dev;
   P.triid = ctx->triid ? ctx->triid->dev : nullptr;
@@ -267,6 +270,7 @@ uint32_t recordCounters(cr_context* ctx) {
   ctx->last.num_subtris = c.numSubtris;
   ctx->last.bin_entries = c.binEntries;
   ctx->last.tile_entries = c.tileEntries;
+  ctx->last.tile_list_entries = c.tileListEntries;
   ctx->last.fragments = (int64_t)c.fragments;
   ctx->last.visible_subtris = (int64_t)c.visibleSubtris;
   memset(&ctx->lastProf, 0, sizeof(ctx->lastProf));
@@ -377,7 +381,8 @@ cr_status cr_context_destroy(cr_context* ctx) {
   if (ctx->pending) (void)hipStreamSynchronize(ctx->pendingStream);
   for (DevBuf* b : {&ctx->triSubtris, &ctx->triHeader, &ctx->triData, &ctx->subtriTri, &ctx->chunkBase,
                     &ctx->binCnt, &ctx->binOff, &ctx->binList, &ctx->tileStart, &ctx->tileCount,
-                    &ctx->tileList, &ctx->tileFrags, &ctx->chunkSubtris, &ctx->counters, &ctx->profBuf})
+                    &ctx->tileList, &ctx->tileSegNext, &ctx->tileFrags, &ctx->chunkSubtris, &ctx->counters,
+                    &ctx->profBuf, &ctx->interpAcc})
     b->release();
   if (ctx->hostCounters) (void)hipHostFree(ctx->hostCounters);
   if (ctx->hostProf) (void)hipHostFree(ctx->hostProf);
@@ -747,21 +752,49 @@ cr_status cr_interpolate_ids(cr_context* ctx, const int32_t* triImage, int32_t w
   return CR_OK;
 }
 
+namespace {
+
+cr_status interpolateBackward(cr_context* ctx, const char* name, bool deterministic, const int32_t* triImage,
+                              int32_t width, int32_t height, const float* pos, int64_t numVertices,
+                              const float* attr, int32_t numChannels, const int32_t* tris, int32_t numTris,
+                              const float* gradOut, float* gradPos, float* gradAttr, void* stream) {
+  if (!ctx || !triImage || !gradOut || !gradPos || (numChannels > 0 && (!attr || !gradAttr)))
+    return fail(ctx, CR_ERR_INVALID, "%s: NULL argument", name);
+  if (width <= 0 || height <= 0 || width > kMaxViewport || height > kMaxViewport || numChannels < 0 ||
+      numVertices < 0 || numTris < 0 || (numTris > 0 && (!pos || !tris)) || ((uintptr_t)pos & 15) ||
+      ((uintptr_t)tris & 3))
+    return fail(ctx, CR_ERR_INVALID, "%s: bad sizes or buffers", name);
+  CR_HIP(ctx, hipSetDevice(ctx->device));
+  InterpGeom G{width, height, width, numChannels, numTris, numVertices};
+  unsigned long long* acc = nullptr;
+  if (deterministic) {
+    const int64_t words = 2 * numVertices * (4 + (int64_t)numChannels);
+    CR_HIP(ctx, ctx->interpAcc.ensure((size_t)std::max<int64_t>(words, 1) * 8));
+    acc = ctx->interpAcc.as<unsigned long long>();
+  }
+  launchInterpBackward(G, triImage, pos, attr, tris, gradOut, gradPos, gradAttr, acc,
+                       static_cast<hipStream_t>(stream));
+  CR_HIP(ctx, hipGetLastError());
+  return CR_OK;
+}
+
+}  // namespace
+
 cr_status cr_interpolate_backward(cr_context* ctx, const int32_t* triImage, int32_t width, int32_t height,
                                   const float* pos, int64_t numVertices, const float* attr, int32_t numChannels,
                                   const int32_t* tris, int32_t numTris, const float* gradOut, float* gradPos,
                                   float* gradAttr, void* stream) {
-  if (!ctx || !triImage || !gradOut || !gradPos || (numChannels > 0 && (!attr || !gradAttr)))
-    return fail(ctx, CR_ERR_INVALID, "cr_interpolate_backward: NULL argument");
-  if (width <= 0 || height <= 0 || width > kMaxViewport || height > kMaxViewport || numChannels < 0 ||
-      numVertices < 0 || numTris < 0 || (numTris > 0 && (!pos || !tris)) || ((uintptr_t)pos & 15) ||
-      ((uintptr_t)tris & 3))
-    return fail(ctx, CR_ERR_INVALID, "cr_interpolate_backward: bad sizes or buffers");
-  CR_HIP(ctx, hipSetDevice(ctx->device));
-  InterpGeom G{width, height, width, numChannels, numTris, numVertices};
-  launchInterpBackward(G, triImage, pos, attr, tris, gradOut, gradPos, gradAttr, static_cast<hipStream_t>(stream));
-  CR_HIP(ctx, hipGetLastError());
-  return CR_OK;
+  return interpolateBackward(ctx, "cr_interpolate_backward", false, triImage, width, height, pos, numVertices,
+                             attr, numChannels, tris, numTris, gradOut, gradPos, gradAttr, stream);
+}
+
+cr_status cr_interpolate_backward_deterministic(cr_context* ctx, const int32_t* triImage, int32_t width,
+                                                int32_t height, const float* pos, int64_t numVertices,
+                                                const float* attr, int32_t numChannels, const int32_t* tris,
+                                                int32_t numTris, const float* gradOut, float* gradPos,
+                                                float* gradAttr, void* stream) {
+  return interpolateBackward(ctx, "cr_interpolate_backward_deterministic", true, triImage, width, height, pos,
+                             numVertices, attr, numChannels, tris, numTris, gradOut, gradPos, gradAttr, stream);
 }
 
 cr_status cr_surface_resolve(const cr_surface* color, void* out, void* stream) {

@@ -68,7 +68,7 @@ struct cr_batch {
   int device = 0;
   std::vector<ViewState> views;
   DevBuf triSubtris, triHeader, triData, subtriTri, chunkBase, binCnt, binOff, binList;
-  DevBuf tileStart, tileCount, tileList, counters, saved;
+  DevBuf tileStart, tileCount, tileList, tileSegNext, counters, saved;
   Counters* hostCounters = nullptr;   // pinned, one block per view
   Tables groups[kMaxGroups];          // views split into cost-balanced groups, one launch chain each
   int numGroups = 1;
@@ -124,10 +124,11 @@ cr_status layout(cr_batch* b) {
   CR_HIP(ctx, b->tileStart.ensure(nTile * 4));
   CR_HIP(ctx, b->tileCount.ensure(nTile * 4));
   CR_HIP(ctx, b->tileList.ensure(nTileList * 4));
+  CR_HIP(ctx, b->tileSegNext.ensure(std::max<size_t>(nTileList >> kTileSegLog2, 1) * 4));
   CR_HIP(ctx, b->counters.ensure(b->views.size() * sizeof(Counters)));
   b->reserved = b->triSubtris.bytes + b->triHeader.bytes + b->triData.bytes + b->subtriTri.bytes +
                 b->chunkBase.bytes + b->binCnt.bytes + b->binOff.bytes + b->binList.bytes + b->tileStart.bytes +
-                b->tileCount.bytes + b->tileList.bytes + b->counters.bytes;
+                b->tileCount.bytes + b->tileList.bytes + b->tileSegNext.bytes + b->counters.bytes;
   b->graphValid = false;
   b->laidOut = true;
   return CR_OK;
@@ -178,6 +179,7 @@ Params viewParams(const cr_batch* b, int i) {
   P.tileCount = b->tileCount.as<uint32_t>() + v.oTile;
   P.maxTileEntries = (int32_t)v.maxTile;
   P.tileList = b->tileList.as<uint32_t>() + v.oTileList;
+  P.tileSegNext = b->tileSegNext.as<uint32_t>() + (v.oTileList >> kTileSegLog2);   // oTileList: a multiple of 64
   P.color = d.color->dev;
   P.depth = d.depth->dev;
   P.triid = d.tri_id ? d.tri_id->dev : nullptr;
@@ -513,7 +515,8 @@ cr_status cr_batch_destroy(cr_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (DevBuf* d : {&b->triSubtris, &b->triHeader, &b->triData, &b->subtriTri, &b->chunkBase, &b->binCnt, &b->binOff,
-                    &b->binList, &b->tileStart, &b->tileCount, &b->tileList, &b->counters, &b->saved,
+                    &b->binList, &b->tileStart, &b->tileCount, &b->tileList, &b->tileSegNext, &b->counters,
+                    &b->saved,
                     &b->subset.params, &b->subset.starts})
     d->release();
   for (Tables& t : b->groups) {
@@ -693,6 +696,7 @@ cr_status cr_batch_get_view_counters(const cr_batch* b, int32_t view, cr_counter
   out->num_subtris = c.numSubtris;
   out->bin_entries = c.binEntries;
   out->tile_entries = c.tileEntries;
+  out->tile_list_entries = c.tileListEntries;
   out->retries = b->retries;
   return CR_OK;
 }

@@ -1,26 +1,28 @@
-// Stage 3: coarse raster -- per-bin, order-preserving tile queues.
+// Stage 3: coarse raster -- per-bin, order-preserving tile lists.
 //
 // The reference merges 16 bin streams by lowest triangle index and emits
 // 32-entry linked tile segments (cudaraster/cuda/CoarseRaster.inl:28-809).
-// Here one block of 16 waves owns a bin and builds one contiguous list per
-// tile in submission order, in two passes over the bin's entries:
+// Here one block of waves owns a bin and walks the bin's entries once, in
+// submission order, in rounds of one entry per lane:
 //
-//   pass 0  count the entries of every tile of the bin;
-//   (alloc) exclusive scan over the 256 tiles, one atomic reserves the bin's
-//           share of the tile queue;
-//   pass 1  emit: per-wave 64-bit emit masks per tile, prefix over waves,
-//           rank = popcount of the lower lanes -- entries land in order.
+//   test    each entry's tiles: exact-sign edge tests of the tiles' closed
+//           squares inside the entry's box (bin-local);
+//   rank    per-wave 64-bit emit masks per tile, a prefix over the waves:
+//           every entry's position in its tile's list;
+//   extend  per tile, the 32-entry segments the round needs beyond the
+//           tile's last, partly filled segment: one block scan over the
+//           tiles and one atomic reserve them for the whole bin, and they
+//           are chained to the tile's list;
+//   write   every entry to its segment and place in it.
+//
+// Lists are chains of segments of the fine stage's chunk size, filled in
+// order, so no count pass (and no second read of the bin queue) is needed
+// to size them; a list wastes at most its last segment's unused room.
 //
 // The bin's entries are the concatenation, in chunk order, of the segments
 // the fused setup/bin stage wrote per chunk of triangles; a segment table is
-// scanned into LDS and every thread finds its entry by binary search.
-//
-// Tile coverage is conservative: bounding box clipped to the bin and the
-// surface, then an exact-sign edge test of each tile's closed square.  Per
-// wave, the test runs per lane when every triangle's box is at most 2x2
-// tiles or the boxes are scattered, and cooperatively (one ballot per tile
-// of the wave's union box) when the boxes overlap -- the large-triangle
-// case.  Binning never changes the image; the fine stage is exact.
+// scanned into LDS and every thread finds its entry by a max-scan over the
+// segment starts.  Binning never changes the image; the fine stage is exact.
 #include "cr_device.h"
 
 namespace crhip {
@@ -124,7 +126,6 @@ struct TileTest {
 // (x, y) = p x d at the box origin + rel, |rel| < 2^29 (<= 15 tile steps of
 // 128 subpixels times |d| < 2^17), so the threshold c - p0 x d clamped to
 // +-2^30 decides every tile exactly in 32 bits.
-template <bool BOX_ONLY = false>
 __device__ __forceinline__ TileTest tileTestOf(const Params& P, uint4 h, int bx, int by) {
   TileTest r;
   const int32_t ox = P.viewportWidth << (kSubpixelLog2 - 1), oy = P.viewportHeight << (kSubpixelLog2 - 1);
@@ -141,7 +142,6 @@ __device__ __forceinline__ TileTest tileTestOf(const Params& P, uint4 h, int bx,
   const int32_t lastY = min(P.heightTiles - ty0, 1 << bl) - 1;
   r.x0 = max((minx >> ts) - tx0, 0); r.x1 = min((maxx >> ts) - tx0, lastX);
   r.y0 = max((miny >> ts) - ty0, 0); r.y1 = min((maxy >> ts) - ty0, lastY);
-  if (BOX_ONLY) return r;
   const int64_t BX = (int64_t)(tx0 + r.x0) << ts, BY = (int64_t)(ty0 + r.y0) << ts;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
@@ -181,13 +181,13 @@ __device__ __forceinline__ TileTest readTileTest(const TileTest& r, int j) {
 
 // Calls f(x, y) for every tile of q's box that q hits, the wave's lanes
 // covering the box 16 x 4 tiles at a time (bins are at most 16 x 16 tiles).
-template <bool BOX_ONLY, class F>
+template <class F>
 __device__ __forceinline__ void forEachHitCoop(const TileTest& q, F&& f) {
   const int lane = threadIdx.x & 63;
   const int x = q.x0 + (lane & 15);
   for (int y0 = q.y0; y0 <= q.y1; y0 += 4) {
     const int y = y0 + (lane >> 4);
-    if (x <= q.x1 && y <= q.y1 && (BOX_ONLY || q.hit(x, y))) f(x, y);
+    if (x <= q.x1 && y <= q.y1 && q.hit(x, y)) f(x, y);
   }
 }
 
@@ -197,13 +197,10 @@ __device__ __forceinline__ void forEachHitCoop(const TileTest& q, F&& f) {
 // by the whole wave.
 enum CoarseCase : int { kCaseA = 0, kCaseB = 1, kCaseC = 2 };
 
-// One round of up to 1024 bin entries: record the tiles each entry touches.
-// COUNT: add to cnt[tile] for every tile of the entry's box (an upper bound
-// of the exact count: list space only, the exact lists are built by the
-// emit pass); otherwise set bits in emitMask[wave][tile] for the tiles the
-// entry's edges reach.  Returns the CoarseCase the wave took.
-template <bool COUNT>
-__device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const int bl, uint32_t* cnt,
+// One round of up to one bin entry per lane: set bit `lane` of
+// emitMask[wave][tile] for every tile the entry's edges reach.  Returns the
+// CoarseCase the wave took.
+__device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const int bl,
                                             uint64_t (*emitMask)[kBinTiles]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int w = valid ? r.x1 - r.x0 + 1 : 0, h = valid ? r.y1 - r.y0 + 1 : 0;
@@ -235,12 +232,9 @@ __device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const
   if (coop) {
     for (int y = uy0; y <= uy1; y++)
       for (int x = ux0; x <= ux1; x++) {
-        const bool hit = has && x >= r.x0 && x <= r.x1 && y >= r.y0 && y <= r.y1 && (COUNT || r.hit(x, y));
+        const bool hit = has && x >= r.x0 && x <= r.x1 && y >= r.y0 && y <= r.y1 && r.hit(x, y);
         const uint64_t m = __ballot(hit);
-        if (m != 0 && lane == 0) {
-          if (COUNT) atomicAdd(&cnt[x + (y << bl)], (uint32_t)__popcll(m));
-          else emitMask[wave][x + (y << bl)] = m;
-        }
+        if (m != 0 && lane == 0) emitMask[wave][x + (y << bl)] = m;
       }
     return kCaseB;
   }
@@ -249,18 +243,14 @@ __device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const
   if (has && !big) {
     for (int y = r.y0; y <= r.y1; y++)
       for (int x = r.x0; x <= r.x1; x++)
-        if (COUNT || r.hit(x, y)) {
-          if (COUNT) atomicAdd(&cnt[x + (y << bl)], 1u);
-          else atomicOr((unsigned long long*)&emitMask[wave][x + (y << bl)], (unsigned long long)bit);
-        }
+        if (r.hit(x, y)) atomicOr((unsigned long long*)&emitMask[wave][x + (y << bl)], (unsigned long long)bit);
   }
   for (uint64_t b = __ballot(big); b != 0; b &= b - 1) {
     const int j = (int)__builtin_ctzll(b);
     const TileTest q = readTileTest(r, j);
     const uint64_t bj = 1ull << j;
-    forEachHitCoop<COUNT>(q, [&](int x, int y) {
-      if (COUNT) atomicAdd(&cnt[x + (y << bl)], 1u);
-      else atomicOr((unsigned long long*)&emitMask[wave][x + (y << bl)], (unsigned long long)bj);
+    forEachHitCoop(q, [&](int x, int y) {
+      atomicOr((unsigned long long*)&emitMask[wave][x + (y << bl)], (unsigned long long)bj);
     });
   }
   return allSmall ? kCaseA : kCaseC;
@@ -274,14 +264,22 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   constexpr int kSegBatch = 2 * CT;     // chunks per segment-table batch (two per lane)
   __shared__ uint32_t segStart[kSegBatch + 1];
   __shared__ uint32_t segAddr[kSegBatch];
-  __shared__ uint32_t cnt[kBinTiles];
-  __shared__ uint32_t run[kBinTiles];
   __shared__ uint64_t emitMask[kCoarseWaves][kBinTiles];
   __shared__ uint32_t emitPre[kCoarseWaves][kBinTiles];
+  // Per tile of the bin: list length so far, first and last segment, and
+  // for the round being written its length before the round, its last
+  // segment before the round and the first segment the round added.
+  __shared__ uint32_t tTotal[kBinTiles];
+  __shared__ uint32_t tFirst[kBinTiles];
+  __shared__ uint32_t tLast[kBinTiles];
+  __shared__ uint32_t tOld[kBinTiles];
+  __shared__ uint32_t tOldLast[kBinTiles];
+  __shared__ uint32_t tNew[kBinTiles];
   __shared__ uint32_t sums[kCoarseWaves + 1];
   __shared__ int32_t segOf[kCoarseThreads];
   __shared__ int32_t sCarry;
-  __shared__ uint32_t sBase;
+  __shared__ uint32_t sSegBase;
+  __shared__ uint32_t sWritten;
   // Profiling draws: entries, rounds, emits, tiles with emits, entries per case.
   __shared__ uint32_t sProf[8];
 
@@ -293,171 +291,174 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t* bCnt = P.binCnt + (size_t)bin * P.numChunks;
   const uint32_t* bOff = P.binOff + (size_t)bin * P.numChunks;
+  const uint32_t segCap = (uint32_t)P.maxTileEntries >> kTileSegLog2;   // segments the list buffer holds
 
-  for (int i = threadIdx.x; i < kBinTiles; i += kCoarseThreads) cnt[i] = 0;
+  for (int i = threadIdx.x; i < kBinTiles; i += kCoarseThreads) {
+    tTotal[i] = 0;
+    tFirst[i] = kNoSeg;
+    tLast[i] = kNoSeg;
+  }
+  if (threadIdx.x == 0) sWritten = 0;
   __syncthreads();
-  uint32_t myStart = 0;                 // thread t < tpb: first list entry of the bin's tile t
 
-  for (int pass = 0; pass < 2; pass++) {
-    if (pass == 1) {
-      // Allocate the bin's tile lists: exclusive scan over the tiles.
-      const uint32_t v = (threadIdx.x < (unsigned)tpb) ? cnt[threadIdx.x] : 0u;
-      uint32_t total;
-      const uint32_t ex = blockScanC<kCoarseThreads>(v, sums, total);
-      if (threadIdx.x == 0) {
-        uint32_t base = total ? atomicAdd(&P.counters->tileEntries, total) : 0u;
-        if (total && (uint64_t)base + total > (uint64_t)P.maxTileEntries) {
-          atomicOr(&P.counters->overflow, 4u);
-          base = 0xFFFFFFFFu;
+  for (int cb = 0; cb < P.numChunks; cb += kSegBatch) {
+    // Segment table of chunks [cb, cb + nb): start of each segment in the
+    // bin's entry order and its address in the bin queue.
+    const int nb = min(kSegBatch, P.numChunks - cb);
+    const int i0 = 2 * threadIdx.x, i1 = i0 + 1;
+    const uint32_t n0 = (i0 < nb) ? bCnt[cb + i0] : 0u, n1 = (i1 < nb) ? bCnt[cb + i1] : 0u;
+    if (i0 < nb) segAddr[i0] = P.chunkBase[cb + i0] + bOff[cb + i0];
+    if (i1 < nb) segAddr[i1] = P.chunkBase[cb + i1] + bOff[cb + i1];
+    uint32_t segTotal;
+    const uint32_t s0 = blockScanC<kCoarseThreads>(n0 + n1, sums, segTotal);
+    if (i0 < nb) segStart[i0] = s0;
+    if (i1 < nb) segStart[i1] = s0 + n0;
+    if (threadIdx.x == 0) segStart[nb] = segTotal;
+    __syncthreads();
+
+    for (uint32_t r0 = 0; r0 < segTotal; r0 += kCoarseThreads) {
+      // Segment of entry r0 + t: every non-empty segment starting in this
+      // round marks its first entry; an inclusive max-scan then gives each
+      // entry the last segment started at or before it (the carry: the
+      // segment of entry r0 - 1).
+      segOf[threadIdx.x] = -1;
+      for (int i = threadIdx.x; i < kCoarseWaves * kBinTiles; i += kCoarseThreads)
+        emitMask[i / kBinTiles][i % kBinTiles] = 0ull;
+      __syncthreads();
+      const int carry = (r0 == 0) ? -1 : sCarry;
+      if (n0 > 0 && s0 >= r0 && s0 - r0 < (uint32_t)kCoarseThreads) segOf[s0 - r0] = i0;
+      if (n1 > 0 && s0 + n0 >= r0 && s0 + n0 - r0 < (uint32_t)kCoarseThreads) segOf[s0 + n0 - r0] = i1;
+      __syncthreads();
+      const int seg = max(blockMaxScanC<kCoarseThreads>(segOf[threadIdx.x], sums), carry);
+      if (threadIdx.x == kCoarseThreads - 1) sCarry = seg;
+      const uint32_t g = r0 + threadIdx.x;
+      bool valid = g < segTotal;
+      uint32_t slot = 0;
+      TileTest r;
+      r.x0 = 1; r.x1 = 0; r.y0 = 1; r.y1 = 0;
+      if (valid) {
+        // The entry carries the slot and the snapped vertices: one
+        // contiguous 16-byte read, no dependent header fetch.
+        const uint4 e = P.binList[segAddr[seg] + (g - segStart[seg])];
+        slot = e.x;
+        valid = slot < (uint32_t)P.maxSubtris;
+        if (valid) r = tileTestOf(P, make_uint4(e.y, e.z, e.w, 0u), bx, by);
+      }
+      const int wcase = coarseRecord(r, valid, bl, emitMask);
+      if (prof) {
+        const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
+        if (lane == 0 && nv) {
+          atomicAdd(&sProf[0], nv);
+          atomicAdd(&sProf[5 + wcase], nv);
         }
-        sBase = base;
+        if (threadIdx.x == 0) sProf[1]++;
       }
       __syncthreads();
-      const uint32_t base = sBase;
-      const bool ok = base != 0xFFFFFFFFu;
+      // Rank: each wave's entries of tile t follow the earlier waves' and
+      // the tile's list so far; the segments the round needs past the
+      // tile's last, partly filled one.
+      uint32_t need = 0;
       if (threadIdx.x < (unsigned)tpb) {
-        const int x = threadIdx.x & ((1 << bl) - 1), y = threadIdx.x >> bl;
-        const int gx = (bx << bl) + x, gy = (by << bl) + y;
-        if (gx < P.widthTiles && gy < P.heightTiles) {
-          const int tile = gx + gy * P.widthTiles;
-          P.tileStart[tile] = ok ? base + ex : 0u;
-          P.tileCount[tile] = 0u;            // exact count written after the emit pass
-        }
-        run[threadIdx.x] = base + ex;
-        myStart = base + ex;
-      }
-      if (!ok || total == 0) return;
-      __syncthreads();
-    }
-
-    for (int cb = 0; cb < P.numChunks; cb += kSegBatch) {
-      // Segment table of chunks [cb, cb + nb): start of each segment in the
-      // bin's entry order and its address in the bin queue.
-      const int nb = min(kSegBatch, P.numChunks - cb);
-      const int i0 = 2 * threadIdx.x, i1 = i0 + 1;
-      const uint32_t n0 = (i0 < nb) ? bCnt[cb + i0] : 0u, n1 = (i1 < nb) ? bCnt[cb + i1] : 0u;
-      if (i0 < nb) segAddr[i0] = P.chunkBase[cb + i0] + bOff[cb + i0];
-      if (i1 < nb) segAddr[i1] = P.chunkBase[cb + i1] + bOff[cb + i1];
-      uint32_t segTotal;
-      const uint32_t s0 = blockScanC<kCoarseThreads>(n0 + n1, sums, segTotal);
-      if (i0 < nb) segStart[i0] = s0;
-      if (i1 < nb) segStart[i1] = s0 + n0;
-      if (threadIdx.x == 0) segStart[nb] = segTotal;
-      __syncthreads();
-
-      for (uint32_t r0 = 0; r0 < segTotal; r0 += kCoarseThreads) {
-        // Segment of entry r0 + t: every non-empty segment starting in this
-        // round marks its first entry; an inclusive max-scan then gives each
-        // entry the last segment started at or before it (the carry: the
-        // segment of entry r0 - 1).
-        segOf[threadIdx.x] = -1;
-        if (pass == 1) {
-          for (int i = threadIdx.x; i < kCoarseWaves * kBinTiles; i += kCoarseThreads)
-            emitMask[i / kBinTiles][i % kBinTiles] = 0ull;
-        }
-        __syncthreads();
-        const int carry = (r0 == 0) ? -1 : sCarry;
-        if (n0 > 0 && s0 >= r0 && s0 - r0 < (uint32_t)kCoarseThreads) segOf[s0 - r0] = i0;
-        if (n1 > 0 && s0 + n0 >= r0 && s0 + n0 - r0 < (uint32_t)kCoarseThreads) segOf[s0 + n0 - r0] = i1;
-        __syncthreads();
-        const int seg = max(blockMaxScanC<kCoarseThreads>(segOf[threadIdx.x], sums), carry);
-        if (threadIdx.x == kCoarseThreads - 1) sCarry = seg;
-        const uint32_t g = r0 + threadIdx.x;
-        bool valid = g < segTotal;
-        uint32_t slot = 0;
-        TileTest r;
-        r.x0 = 1; r.x1 = 0; r.y0 = 1; r.y1 = 0;
-        if (valid) {
-          // The entry carries the slot and the snapped vertices: one
-          // contiguous 16-byte read, no dependent header fetch.
-          const uint4 e = P.binList[segAddr[seg] + (g - segStart[seg])];
-          slot = e.x;
-          valid = slot < (uint32_t)P.maxSubtris;
-          if (valid) {
-            const uint4 h = make_uint4(e.y, e.z, e.w, 0u);
-            r = pass == 0 ? tileTestOf<true>(P, h, bx, by) : tileTestOf<false>(P, h, bx, by);
-          }
-        }
-        if (pass == 0) {
-          coarseRecord<true>(r, valid, bl, cnt, emitMask);
-          continue;
-        }
-        const int wcase = coarseRecord<false>(r, valid, bl, cnt, emitMask);
-        if (prof) {
-          const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
-          if (lane == 0 && nv) {
-            atomicAdd(&sProf[0], nv);
-            atomicAdd(&sProf[5 + wcase], nv);
-          }
-          if (threadIdx.x == 0) sProf[1]++;
-        }
-        __syncthreads();
-        if (threadIdx.x < (unsigned)tpb) {
-          const uint32_t before = run[threadIdx.x];
-          uint32_t acc = before;
+        const uint32_t old = tTotal[threadIdx.x];
+        uint32_t acc = old;
 #pragma unroll
-          for (int q = 0; q < kCoarseWaves; q++) {
-            emitPre[q][threadIdx.x] = acc;
-            acc += (uint32_t)__popcll(emitMask[q][threadIdx.x]);
+        for (int q = 0; q < kCoarseWaves; q++) {
+          emitPre[q][threadIdx.x] = acc;
+          acc += (uint32_t)__popcll(emitMask[q][threadIdx.x]);
+        }
+        need = ((acc + kTileSeg - 1) >> kTileSegLog2) - ((old + kTileSeg - 1) >> kTileSegLog2);
+        tOld[threadIdx.x] = old;
+        tOldLast[threadIdx.x] = tLast[threadIdx.x];
+        tTotal[threadIdx.x] = acc;
+        if (prof && acc != old) {
+          atomicAdd(&sProf[2], acc - old);
+          atomicAdd(&sProf[3], 1u);
+        }
+      }
+      // Extend: one reservation for the whole round (its own barriers order
+      // the stores above).
+      uint32_t segsRound;
+      const uint32_t segEx = blockScanC<kCoarseThreads>(need, sums, segsRound);
+      if (threadIdx.x == 0) {
+        uint32_t base = 0;
+        if (segsRound) {
+          base = atomicAdd(&P.counters->tileEntries, segsRound << kTileSegLog2);
+          if ((uint64_t)base + (segsRound << kTileSegLog2) > (uint64_t)P.maxTileEntries)
+            atomicOr(&P.counters->overflow, 4u);
+        }
+        sSegBase = base >> kTileSegLog2;
+      }
+      __syncthreads();
+      if (threadIdx.x < (unsigned)tpb && need > 0) {
+        // Chain the round's segments (consecutive ids) to the tile's list.
+        // Segments past the buffer (an overflowing draw, re-run larger) are
+        // never written.
+        const uint32_t first = sSegBase + segEx, last = first + need - 1;
+        const uint32_t prev = tOldLast[threadIdx.x];
+        if (prev == kNoSeg) tFirst[threadIdx.x] = first;
+        else if (prev < segCap) P.tileSegNext[prev] = first;
+        for (uint32_t j = first; j < last; j++)
+          if (j < segCap) P.tileSegNext[j] = j + 1;
+        if (last < segCap) P.tileSegNext[last] = kNoSeg;
+        tNew[threadIdx.x] = first;
+        tLast[threadIdx.x] = last;
+      }
+      __syncthreads();
+      // Write the entries: list position e of tile ti is entry e & 31 of
+      // the tile's (e >> 5)-th segment -- the old last segment or one of the
+      // round's.  Large boxes are walked by the whole wave, one entry at a
+      // time.
+      auto put = [&](int ti, uint32_t e, uint32_t sl) {
+        const uint32_t k = e >> kTileSegLog2, kOld = (tOld[ti] + kTileSeg - 1) >> kTileSegLog2;
+        const uint32_t sg = k < kOld ? tOldLast[ti] : tNew[ti] + (k - kOld);
+        if (sg < segCap) P.tileList[((size_t)sg << kTileSegLog2) + (e & (kTileSeg - 1))] = sl;
+      };
+      const bool hasBox = valid && r.x0 <= r.x1 && r.y0 <= r.y1;
+      const bool bigBox = hasBox && (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1) > kCoopArea;
+      if (hasBox && !bigBox) {
+        const uint64_t bit = 1ull << lane, lt = lanesBelow();
+        for (int y = r.y0; y <= r.y1; y++)
+          for (int x = r.x0; x <= r.x1; x++) {
+            const int ti = x + (y << bl);
+            const uint64_t m = emitMask[wave][ti];
+            if (m & bit) put(ti, emitPre[wave][ti] + (uint32_t)__popcll(m & lt), slot);
           }
-          run[threadIdx.x] = acc;
-          if (prof && acc != before) {
-            atomicAdd(&sProf[2], acc - before);
-            atomicAdd(&sProf[3], 1u);
+      }
+      for (uint64_t b = __ballot(bigBox); b != 0; b &= b - 1) {
+        const int j = (int)__builtin_ctzll(b);
+        const uint64_t bj = 1ull << j, ltj = bj - 1;
+        const uint32_t sj = (uint32_t)__builtin_amdgcn_readlane((int)slot, j);
+        const int qx0 = __builtin_amdgcn_readlane(r.x0, j), qx1 = __builtin_amdgcn_readlane(r.x1, j);
+        const int qy0 = __builtin_amdgcn_readlane(r.y0, j), qy1 = __builtin_amdgcn_readlane(r.y1, j);
+        const int x = qx0 + (lane & 15);
+        for (int y0 = qy0; y0 <= qy1; y0 += 4) {
+          const int y = y0 + (lane >> 4);
+          if (x <= qx1 && y <= qy1) {
+            const int ti = x + (y << bl);
+            const uint64_t m = emitMask[wave][ti];
+            if (m & bj) put(ti, emitPre[wave][ti] + (uint32_t)__popcll(m & ltj), sj);
           }
         }
-        __syncthreads();
-        // Write the entries: entry `lane` goes to tile ti at its rank among
-        // the wave's entries that hit ti.  Large boxes are walked by the
-        // whole wave, one entry at a time.
-        const bool hasBox = valid && r.x0 <= r.x1 && r.y0 <= r.y1;
-        const bool bigBox = hasBox && (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1) > kCoopArea;
-        if (hasBox && !bigBox) {
-          const uint64_t bit = 1ull << lane, lt = lanesBelow();
-          for (int y = r.y0; y <= r.y1; y++)
-            for (int x = r.x0; x <= r.x1; x++) {
-              const int ti = x + (y << bl);
-              const uint64_t m = emitMask[wave][ti];
-              if (m & bit) {
-                const uint32_t p = emitPre[wave][ti] + (uint32_t)__popcll(m & lt);
-                if (p < (uint32_t)P.maxTileEntries) P.tileList[p] = slot;
-              }
-            }
-        }
-        for (uint64_t b = __ballot(bigBox); b != 0; b &= b - 1) {
-          const int j = (int)__builtin_ctzll(b);
-          const uint64_t bj = 1ull << j, ltj = bj - 1;
-          const uint32_t sj = (uint32_t)__builtin_amdgcn_readlane((int)slot, j);
-          const int qx0 = __builtin_amdgcn_readlane(r.x0, j), qx1 = __builtin_amdgcn_readlane(r.x1, j);
-          const int qy0 = __builtin_amdgcn_readlane(r.y0, j), qy1 = __builtin_amdgcn_readlane(r.y1, j);
-          const int x = qx0 + (lane & 15);
-          for (int y0 = qy0; y0 <= qy1; y0 += 4) {
-            const int y = y0 + (lane >> 4);
-            if (x <= qx1 && y <= qy1) {
-              const int ti = x + (y << bl);
-              const uint64_t m = emitMask[wave][ti];
-              if (m & bj) {
-                const uint32_t p = emitPre[wave][ti] + (uint32_t)__popcll(m & ltj);
-                if (p < (uint32_t)P.maxTileEntries) P.tileList[p] = sj;
-              }
-            }
-          }
-        }
-        __syncthreads();
       }
       __syncthreads();
     }
     __syncthreads();
   }
-  // Exact list lengths: the emit pass advanced run[t] past every entry it
-  // wrote for tile t (the box-based allocation may leave unused room).
+  // The bin's lists: first segment and length per tile, and the entries
+  // written (the lists' total length) into the counter block.
   if (threadIdx.x < (unsigned)tpb) {
     const int x = threadIdx.x & ((1 << bl) - 1), y = threadIdx.x >> bl;
     const int gx = (bx << bl) + x, gy = (by << bl) + y;
-    if (gx < P.widthTiles && gy < P.heightTiles) P.tileCount[gx + gy * P.widthTiles] = run[threadIdx.x] - myStart;
+    if (gx < P.widthTiles && gy < P.heightTiles) {
+      P.tileStart[gx + gy * P.widthTiles] = tFirst[threadIdx.x];
+      P.tileCount[gx + gy * P.widthTiles] = tTotal[threadIdx.x];
+    }
+    if (tTotal[threadIdx.x]) atomicAdd(&sWritten, tTotal[threadIdx.x]);
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && sWritten) atomicAdd(&P.counters->tileListEntries, sWritten);
   if (prof && threadIdx.x == 0) {
-    // (The emit pass ended with a barrier: every wave's additions are in.)
+    // (The rounds ended with a barrier: every wave's additions are in.)
     const uint32_t entries = sProf[0], rounds = sProf[1], emits = sProf[2];
     profAdd(P.prof, kProfCoarseBins, entries ? 1 : 0, 0);
     profAdd(P.prof, kProfCoarseRoundsPerBin, rounds, entries ? 1 : 0);

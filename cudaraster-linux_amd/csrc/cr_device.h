@@ -25,6 +25,10 @@ constexpr int kMaxViewport = 2048;
 constexpr int kMaxBinsLog2 = 4;               // <= 16x16 bins
 constexpr int kMaxBins = 1 << (2 * kMaxBinsLog2);
 constexpr int kBinTiles = 1 << (2 * kBinLog2);
+// Tile lists are chains of 32-entry segments (the fine stage's chunk).
+constexpr int kTileSegLog2 = 5;
+constexpr int kTileSeg = 1 << kTileSegLog2;
+constexpr uint32_t kNoSeg = 0xFFFFFFFFu;
 constexpr uint32_t kDepthMin = 2200u << 3;
 constexpr uint32_t kDepthMax = 0xFFFFFFFFu - (2200u << 3);
 constexpr int kBaryMax = (1 << (30 - kSubpixelLog2)) - 1;
@@ -67,10 +71,10 @@ static_assert(sizeof(TriData) == 64, "data layout");
 // Device counters; one 256-byte block per context.
 struct Counters {
   uint32_t numSubtris;     // slot allocator for clipped subtriangles (starts at numTris)
-  uint32_t tileEntries;    // tile-list entry allocator
+  uint32_t tileEntries;    // tile-list allocator: entries reserved, in whole 32-entry segments
   uint32_t overflow;       // bit 0 subtris, bit 1 bin entries, bit 2 tile entries
   uint32_t binEntries;     // bin-queue entry allocator
-  uint32_t unused0;
+  uint32_t tileListEntries;  // tile-list entries written (the lists' total length)
   uint32_t pad0[3];
   uint64_t fragments;      // (subtriangle, pixel) pairs with >= 1 covered sample
   uint64_t visibleSubtris; // subtriangles that survive setup (counting draws only)
@@ -168,11 +172,14 @@ struct Params {
   uint32_t* binOff;                // [numBins][numChunks]
   int32_t maxBinEntries;
   uint4* binList;                  // per entry: slot, then the header's three snapped vertices
-  // Tile lists (each tile contiguous and in submission order).
-  uint32_t* tileStart;             // [numTiles]
-  uint32_t* tileCount;             // [numTiles]
-  int32_t maxTileEntries;
+  // Tile lists in submission order, each a chain of kTileSeg-entry
+  // segments: segment s holds tileList[s * kTileSeg ...], tileSegNext[s] is
+  // the next segment of its tile (kNoSeg after the last).
+  uint32_t* tileStart;             // [numTiles] first segment (kNoSeg: empty list)
+  uint32_t* tileCount;             // [numTiles] entries in the list
+  int32_t maxTileEntries;          // tileList capacity; maxTileEntries / kTileSeg segments
   uint32_t* tileList;
+  uint32_t* tileSegNext;           // [maxTileEntries / kTileSeg]
   // Surfaces: (widthPixels * N) x heightPixels texels, row 0 at the bottom.
   uint32_t* color;
   uint32_t* depth;
@@ -325,11 +332,13 @@ void launchVertexPassthrough(const float* modelPos, int64_t numVertices, const f
                              int64_t outStride, hipStream_t stream);
 void launchInterpForward(const InterpGeom& G, const uint32_t* ids, const float* pos, const float* attr,
                          const int32_t* tris, float* out, int32_t* triImage, hipStream_t stream);
-// gradPos (4 per vertex) and gradAttr (C per vertex) are cleared, then
-// accumulated with FP32 atomics (per-pixel terms summed in FP64 first).
+// gradPos (4 per vertex) and gradAttr (C per vertex) are overwritten.  The
+// per-(tile, triangle) FP64 sums are added with FP32 atomics (acc null), or,
+// bit-reproducibly, as 128-bit fixed-point integers in acc (2 * (4 + C) * V
+// 64-bit words) rounded to FP32 at the end.
 void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const float* pos, const float* attr,
                           const int32_t* tris, const float* gradOut, float* gradPos, float* gradAttr,
-                          hipStream_t stream);
+                          unsigned long long* acc, hipStream_t stream);
 
 // Largest v with start[v] <= b: the view owning workgroup (or group) b.
 // start has numViews + 1 nondecreasing entries; views without work have

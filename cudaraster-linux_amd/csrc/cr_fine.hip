@@ -253,7 +253,9 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
   // covered samples, fragments whose covered samples all fail the depth
   // test, fragments written.
   uint64_t pHiz = 0, pWalked = 0, pEmpty = 0, pSamples = 0, pZKill = 0, pWritten = 0;
-  const uint32_t start = P.tileStart[tile];
+  // The tile's list: a chain of kTileSeg-entry segments, one per chunk.
+  static_assert(kChunk == kTileSeg, "one list segment per chunk");
+  uint32_t seg = P.tileStart[tile];
   const bool hiz = DEPTH && P.hiz;
 
   for (uint32_t c = 0; c < cnt; c += kChunk) {
@@ -273,8 +275,10 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
     bool keep = false;
     uint32_t slot = 0;
     uint4 h = make_uint4(0, 0, 0, 0), z = make_uint4(0, 0, 0, 0);
+    // The next segment, read now so that its latency overlaps the chunk.
+    const uint32_t nextSeg = (c + kChunk < cnt) ? P.tileSegNext[seg] : kNoSeg;
     if (lane < (int)n) {
-      slot = tileList[start + c + lane];
+      slot = tileList[((size_t)seg << kTileSegLog2) + lane];
       if (slot < (uint32_t)P.maxSubtris) {
         h = hdr[slot];
         if (DEPTH) z = tdata[(size_t)slot * 4];
@@ -298,6 +302,11 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
       // needs no clamp.  Uniform over the wave's set-up lanes.
       const uint32_t span = (uint32_t)(abs(ax[0]) | abs(ay[0]) | abs(ax[1]) | abs(ay[1]) | abs(ax[2]) | abs(ay[2]));
       const bool near = __ballot(span >= (1u << 14)) == 0;
+      // Whole tile covered: at every sample of every pixel each edge's
+      // L = lx16*dy + ly16*(-dx) (lx16, ly16 in 0..112) is at most the
+      // sample's threshold, i.e. its largest value over the tile is at most
+      // the smallest threshold.  Exactly the per-sample test's outcome.
+      bool full = true;
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         const int kn = (k + 1) % 3;
@@ -317,13 +326,17 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
           cc = cc > (1ll << 30) ? (1ll << 30) : (cc < -(1ll << 30) ? -(1ll << 30) : cc);
           ck = (int32_t)cc;
         }
+        int32_t minR = INT32_MAX;
 #pragma unroll
         for (int i = 0; i < N; i++) {
           const int32_t ox = (2 * msaaPattern(S, i) + 1) << (3 - S);
           const int32_t oy = (2 * i + 1) << (3 - S);
           // |ox|, |oy| < 16 and |dx|, |dy| < 2^17: exact 24-bit products.
-          r[L::R + k * N + i] = (uint32_t)(ck - (__mul24(ox, dy) - __mul24(oy, dx)));
+          const int32_t rk = ck - (__mul24(ox, dy) - __mul24(oy, dx));
+          r[L::R + k * N + i] = (uint32_t)rk;
+          minR = min(minR, rk);
         }
+        full = full && __mul24(112, max(dy, 0)) + __mul24(112, max(-dx, 0)) <= minR;
         r[L::D + 2 * k] = (uint32_t)dy;
         r[L::D + 2 * k + 1] = (uint32_t)(-dx);
       }
@@ -338,7 +351,11 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
       // Deferred shading records the (slot << 4 | centroid code) words the
       // walk stores per sample; at 1x the code is always the pixel centre.
       r[L::SLOT] = VIS ? ((slot << 4) | (N == 1 ? 15u : 0u)) : slot;
-      if (!VIS) r[L::TRI] = (slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris];
+      // In-order shading: the source triangle, bit 31 set when the triangle
+      // covers the whole tile (source indices stay below 2^28).
+      if (!VIS)
+        r[L::TRI] = ((slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris]) |
+                    (full ? 0x80000000u : 0u);
       if (TEX) {
         // Barycentric planes and vertex indices; the shader reads the
         // varyings, material and textures itself.
@@ -368,20 +385,30 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
       const uint32_t* r = rec + j * RS;
       const uint32_t* cur = r;
       // |lx16|,|ly16| < 2^7 and |d| < 2^17: 24-bit multiplies are exact.
-      int32_t Lk[3];
-#pragma unroll
-      for (int k = 0; k < 3; k++)
-        Lk[k] = __mul24(lx16, (int32_t)cur[L::D + 2 * k]) + __mul24(ly16, (int32_t)cur[L::D + 2 * k + 1]);
+      // In-order shading: a record covering the whole tile skips the
+      // per-sample edge tests (a scalar branch: the flag is uniform).
+      const uint32_t triWord = VIS ? 0u : (uint32_t)__builtin_amdgcn_readfirstlane((int)r[L::TRI]);
+      const bool fullTile = !VIS && (triWord >> 31) != 0;
       // Per-sample coverage and write masks are kept as explicit 64-bit lane
       // masks (scalar registers): a per-lane bool carried from one block to
       // the next would be rebuilt from a vector register before each ballot.
       uint64_t covM[N];
       uint64_t anyM = 0;
+      if (fullTile) {
 #pragma unroll
-      for (int i = 0; i < N; i++) {
-        covM[i] = __ballot(Lk[0] <= (int32_t)cur[L::R + i]) & __ballot(Lk[1] <= (int32_t)cur[L::R + N + i]) &
-                  __ballot(Lk[2] <= (int32_t)cur[L::R + 2 * N + i]);
-        anyM |= covM[i];
+        for (int i = 0; i < N; i++) covM[i] = ~0ull;
+        anyM = ~0ull;
+      } else {
+        int32_t Lk[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+          Lk[k] = __mul24(lx16, (int32_t)cur[L::D + 2 * k]) + __mul24(ly16, (int32_t)cur[L::D + 2 * k + 1]);
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          covM[i] = __ballot(Lk[0] <= (int32_t)cur[L::R + i]) & __ballot(Lk[1] <= (int32_t)cur[L::R + N + i]) &
+                    __ballot(Lk[2] <= (int32_t)cur[L::R + 2 * N + i]);
+          anyM |= covM[i];
+        }
       }
       if (COUNT) {
         pEmpty += anyM == 0;
@@ -425,12 +452,14 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
       // Centroid (selectMSAACentroid): first covered sample in distance
       // order, the pixel centre (15) when none or all are covered.
       int code = 15;
-      bool all = true;
+      if (!fullTile) {
+        bool all = true;
 #pragma unroll
-      for (int k = N - 1; k >= 0; k--) code = cov[co.order[k]] ? co.order[k] : code;
+        for (int k = N - 1; k >= 0; k--) code = cov[co.order[k]] ? co.order[k] : code;
 #pragma unroll
-      for (int i = 0; i < N; i++) all = all & cov[i];
-      if (all) code = 15;
+        for (int i = 0; i < N; i++) all = all & cov[i];
+        if (all) code = 15;
+      }
 
       if (VIS) {
         const uint32_t v = N == 1 ? r[L::SLOT] : (r[L::SLOT] | (uint32_t)code);
@@ -441,7 +470,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
         }
       } else {
         uint32_t src = P.constColor;
-        const uint32_t tri = r[L::TRI];
+        const uint32_t tri = triWord & 0x7FFFFFFFu;
         bool discard = false;
         if (TEX) {
           if (anyWr) {
@@ -475,6 +504,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
       }
     }
     waveSync();
+    seg = nextSeg;
   }
 
   // ---- deferred shading: once per (pixel, distinct winning triangle) -------

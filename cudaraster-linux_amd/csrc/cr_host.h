@@ -87,7 +87,9 @@ constexpr int64_t kMaxSubtris = (int64_t)1 << 28;
 inline void initialCapacities(int64_t nt, const Geometry& g, int64_t& subtris, int64_t& bin, int64_t& tile) {
   subtris = std::max(subtris, std::min(nt + kSubtrisSlack, kMaxSubtris));
   bin = std::max(bin, nt + (int64_t)g.nb * 64 + kBinSlack);
-  tile = std::max(tile, 2 * nt + (int64_t)g.nt + kTileSlack);
+  // Tile lists: two entries per triangle, plus the unused room of each
+  // list's last 32-entry segment.
+  tile = std::max(tile, 2 * nt + (int64_t)g.nt * crhip::kTileSeg + kTileSlack);
 }
 
 // Records `fmt` as the context's (and the thread's) last error; returns st.
@@ -114,6 +116,7 @@ struct DrawBuffers {
   uint32_t* tileStart = nullptr;
   uint32_t* tileCount = nullptr;
   uint32_t* tileList = nullptr;
+  uint32_t* tileSegNext = nullptr;    // maxTileEntries / 32 segment links
   uint32_t* tileFrags = nullptr;      // counting draws only
   uint32_t* chunkSubtris = nullptr;   // counting draws only
   unsigned long long* prof = nullptr; // profiling draws only: 2 * kProfCount counters
@@ -171,8 +174,9 @@ struct cr_context {
   int64_t maxSubtris = 1, maxBinEntries = 1, maxTileEntries = 1;
   crhost::DevBuf triSubtris, triHeader, triData, subtriTri;
   crhost::DevBuf chunkBase, binCnt, binOff, binList;
-  crhost::DevBuf tileStart, tileCount, tileList, tileFrags, chunkSubtris;
+  crhost::DevBuf tileStart, tileCount, tileList, tileSegNext, tileFrags, chunkSubtris;
   crhost::DevBuf counters, profBuf;
+  crhost::DevBuf interpAcc;                  // fixed-point gradient accumulators (deterministic backward)
   crhip::Counters* hostCounters = nullptr;   // pinned
   unsigned long long* hostProf = nullptr;    // pinned, 2 * kProfCount
   cr_profile_counters lastProf{};

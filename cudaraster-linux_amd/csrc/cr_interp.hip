@@ -131,6 +131,33 @@ constexpr int kBwdTile = 16;
 constexpr int kBwdThreads = kBwdTile * kBwdTile;
 constexpr int kBwdStageC = 8;                  // channels of dL/dout staged in LDS per pixel
 
+// Bit-reproducible accumulation: a value is a 128-bit two's-complement
+// fixed-point number with 2^-64 resolution, two 64-bit words (low, high).
+// Each FP64 sum is converted once (magnitude truncated to the resolution,
+// |x| < 2^63) and added with integer atomics: the low word, then the high
+// word with the low word's carry (only when that is not zero).  Integer
+// addition is associative, so the total does not depend on the order the
+// atomics land in.
+__device__ __forceinline__ void fixedAdd(unsigned long long* __restrict__ acc, int64_t idx, double s) {
+  const bool neg = s < 0.0;
+  double m = fmin(fabs(s) * 18446744073709551616.0, 1.7014118346046923e38);   // |s| 2^64, below 2^127
+  const double mh = floor(m * 5.421010862427522e-20);                          // m / 2^64
+  const double ml = m - mh * 18446744073709551616.0;                           // exact
+  uint64_t lo = (uint64_t)ml, hi = (uint64_t)mh;
+  if (neg) {
+    lo = ~lo + 1u;
+    hi = ~hi + (lo == 0u ? 1u : 0u);
+  }
+  unsigned long long* w = acc + 2 * idx;
+  const unsigned long long old = atomicAdd(w, (unsigned long long)lo);
+  const uint64_t carry = (old + lo) < old ? 1u : 0u;
+  const uint64_t h = hi + carry;
+  if (h != 0u) atomicAdd(w + 1, (unsigned long long)h);
+}
+
+// DET: accumulate into the fixed-point words `acc` (positions V*4 values,
+// then attributes V*C) instead of FP32 atomics on the outputs.
+template <bool DET>
 __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const InterpGeom G,
                                                                     const int32_t* __restrict__ triImage,
                                                                     const float4* __restrict__ pos,
@@ -138,7 +165,8 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
                                                                     const int3* __restrict__ tris,
                                                                     const float* __restrict__ gradOut,
                                                                     float* __restrict__ gradPos,
-                                                                    float* __restrict__ gradAttr) {
+                                                                    float* __restrict__ gradAttr,
+                                                                    unsigned long long* __restrict__ acc) {
   __shared__ double sB[kBwdThreads][3];        // barycentrics
   __shared__ double sP[kBwdThreads][9];        // dL/d(x, y, w) per corner
   __shared__ int32_t sV[kBwdThreads][3];       // vertex indices
@@ -261,7 +289,11 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
     } else {
       for (int q = q0; q < q1 && sT[q] == t; q++) s += sB[q][k] * (double)gradOut[(int64_t)sPix[q] * C + c];
     }
-    if (s != 0.0) unsafeAtomicAdd(&gradAttr[(int64_t)sV[q0][k] * C + c], (float)s);
+    if (s != 0.0) {
+      const int64_t o = (int64_t)sV[q0][k] * C + c;
+      if (DET) fixedAdd(acc, G.numVertices * 4 + o, s);
+      else unsafeAtomicAdd(&gradAttr[o], (float)s);
+    }
   }
   // ---- phase 3: position gradients per (run, corner, x/y/w) ----------------
   for (int e = tid; e < (int)numRuns * 9; e += kBwdThreads) {
@@ -270,7 +302,11 @@ __global__ void __launch_bounds__(kBwdThreads) interpBackwardKernel(const Interp
     const int32_t t = sT[q0];
     double s = 0.0;
     for (int q = q0; q < q1 && sT[q] == t; q++) s += sP[q][rem];
-    if (s != 0.0) unsafeAtomicAdd(&gradPos[(int64_t)sV[q0][k] * 4 + (comp == 2 ? 3 : comp)], (float)s);
+    if (s != 0.0) {
+      const int64_t o = (int64_t)sV[q0][k] * 4 + (comp == 2 ? 3 : comp);
+      if (DET) fixedAdd(acc, o, s);
+      else unsafeAtomicAdd(&gradPos[o], (float)s);
+    }
   }
 }
 
@@ -296,6 +332,26 @@ __global__ void __launch_bounds__(256) zeroTwoKernel(float* __restrict__ a, int6
 
 static int gridFor(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 1 << 20); }
 
+// Zeroes n 64-bit words.
+__global__ void __launch_bounds__(256) zeroWordsKernel(unsigned long long* __restrict__ w, int64_t n) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    w[k] = 0ull;
+}
+
+// Rounds the fixed-point totals to FP32: positions (V*4), then attributes.
+__global__ void __launch_bounds__(256) fixedToFloatKernel(const unsigned long long* __restrict__ acc,
+                                                          int64_t nPos, int64_t nAttr, float* __restrict__ gradPos,
+                                                          float* __restrict__ gradAttr) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nPos + nAttr;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t lo = acc[2 * k];
+    const int64_t hi = (int64_t)acc[2 * k + 1];
+    const double v = (double)hi + (double)lo * 5.421010862427522e-20;
+    if (k < nPos) gradPos[k] = (float)v;
+    else gradAttr[k - nPos] = (float)v;
+  }
+}
+
 void launchInterpForward(const InterpGeom& G, const uint32_t* ids, const float* pos, const float* attr,
                          const int32_t* tris, float* out, int32_t* triImage, hipStream_t stream) {
   if (G.width <= 0 || G.height <= 0) return;
@@ -307,17 +363,29 @@ void launchInterpForward(const InterpGeom& G, const uint32_t* ids, const float* 
 
 void launchInterpBackward(const InterpGeom& G, const int32_t* triImage, const float* pos, const float* attr,
                           const int32_t* tris, const float* gradOut, float* gradPos, float* gradAttr,
-                          hipStream_t stream) {
+                          unsigned long long* acc, hipStream_t stream) {
   const int64_t nPos = G.numVertices * 4, nAttr = G.numVertices * G.numChannels;
-  if (nPos + nAttr > 0)
-    hipLaunchKernelGGL(zeroTwoKernel, dim3(gridFor((nPos + nAttr + 3) / 4)), dim3(256), 0, stream, gradPos, nPos,
-                       gradAttr, nAttr);
-  if (G.width > 0 && G.height > 0 && G.numTris > 0) {
-    const dim3 grid((unsigned)((G.width + kBwdTile - 1) / kBwdTile), (unsigned)((G.height + kBwdTile - 1) / kBwdTile));
-    hipLaunchKernelGGL(interpBackwardKernel, grid, dim3(kBwdThreads), 0, stream, G, triImage,
-                       reinterpret_cast<const float4*>(pos), attr, reinterpret_cast<const int3*>(tris), gradOut,
-                       gradPos, gradAttr);
+  const dim3 grid((unsigned)((G.width + kBwdTile - 1) / kBwdTile), (unsigned)((G.height + kBwdTile - 1) / kBwdTile));
+  const bool work = G.width > 0 && G.height > 0 && G.numTris > 0;
+  if (acc == nullptr) {
+    if (nPos + nAttr > 0)
+      hipLaunchKernelGGL(zeroTwoKernel, dim3(gridFor((nPos + nAttr + 3) / 4)), dim3(256), 0, stream, gradPos, nPos,
+                         gradAttr, nAttr);
+    if (work)
+      hipLaunchKernelGGL(interpBackwardKernel<false>, grid, dim3(kBwdThreads), 0, stream, G, triImage,
+                         reinterpret_cast<const float4*>(pos), attr, reinterpret_cast<const int3*>(tris), gradOut,
+                         gradPos, gradAttr, acc);
+    return;
   }
+  if (nPos + nAttr == 0) return;
+  hipLaunchKernelGGL(zeroWordsKernel, dim3(gridFor(2 * (nPos + nAttr))), dim3(256), 0, stream, acc,
+                     2 * (nPos + nAttr));
+  if (work)
+    hipLaunchKernelGGL(interpBackwardKernel<true>, grid, dim3(kBwdThreads), 0, stream, G, triImage,
+                       reinterpret_cast<const float4*>(pos), attr, reinterpret_cast<const int3*>(tris), gradOut,
+                       gradPos, gradAttr, acc);
+  hipLaunchKernelGGL(fixedToFloatKernel, dim3(gridFor(nPos + nAttr)), dim3(256), 0, stream, acc, nPos, nAttr,
+                     gradPos, gradAttr);
 }
 
 }  // namespace crhip

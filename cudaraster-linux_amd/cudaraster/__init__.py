@@ -51,7 +51,7 @@ EXPORTS = [
     "cr_batch_use_graph", "cr_batch_get_info", "cr_batch_get_view_counters",
     "cr_interpolate", "cr_interpolate_backward", "cr_batch_configure", "cr_vertex_shader_passthrough",
     "cr_surface_resolve", "cr_draw_triangles_noalloc", "cr_get_profile_counters", "cr_set_shader_constants",
-    "cr_interpolate_ids",
+    "cr_interpolate_ids", "cr_interpolate_backward_deterministic",
 ]
 
 
@@ -74,7 +74,7 @@ class Stats(ctypes.Structure):
 
 class Counters(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in ("num_tris", "num_subtris", "bin_entries", "tile_entries",
-                                              "fragments", "retries", "visible_subtris")]
+                                              "fragments", "retries", "visible_subtris", "tile_list_entries")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -186,6 +186,7 @@ def load(path: str | None = None):
         "cr_get_profile_counters": [vp, ctypes.POINTER(ProfileCounters)],
         "cr_set_shader_constants": [vp, ctypes.POINTER(ShaderConstants)],
         "cr_interpolate_ids": [vp, vp, i32, i32, vp, i64, vp, i32, vp, i32, vp, vp],
+        "cr_interpolate_backward_deterministic": [vp, vp, i32, i32, vp, i64, vp, i32, vp, i32, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -270,8 +271,6 @@ class CudaSurface:
         int32 tensor [height, width] of RGBA8 words (row 0 at the bottom)."""
         import torch
         w, h = self.width, self.height
-        if self.format != FORMAT_RGBA8:
-            raise ValueError("resolve needs an RGBA8 surface")
         if out is None:
             out = torch.empty((h, w), dtype=torch.int32, device="cuda")
         elif not (hasattr(out, "is_cuda") and out.is_cuda and out.is_contiguous() and

@@ -106,10 +106,13 @@ class InterpolateViews:
 class DiffRasterizer:
     """Single-sample rasterizer + differentiable interpolation for one
     resolution.  render(pos, attr, tris) -> image [H, W, C] (row 0 at the
-    bottom); gradients flow to pos ([V, 4] clip space) and attr ([V, C])."""
+    bottom); gradients flow to pos ([V, 4] clip space) and attr ([V, C]).
+    deterministic=True makes the gradients bit-reproducible from run to run
+    (fixed-point accumulation, cr_interpolate_backward_deterministic)."""
 
-    def __init__(self, width, height):
+    def __init__(self, width, height, deterministic=False):
         self.width, self.height = int(width), int(height)
+        self.deterministic = bool(deterministic)
         self._sized = False       # capacities known from a synchronous draw
         self.cr = CudaRaster().init()
         self.color = CudaSurface((width, height), FORMAT_RGBA8, 1)
@@ -224,10 +227,10 @@ class DiffRasterizer:
         gp = torch.empty_like(pos)
         ga = torch.empty_like(attr)
         H, W = tri_image.shape
-        _check(load().cr_interpolate_backward(self.cr.ctx, tri_image.data_ptr(), W, H, pos.data_ptr(),
-                                              pos.shape[0], attr.data_ptr(), attr.shape[1], tris.data_ptr(),
-                                              tris.shape[0], grad_out.data_ptr(), gp.data_ptr(), ga.data_ptr(),
-                                              _stream()), self.cr.ctx)
+        fn = load().cr_interpolate_backward_deterministic if self.deterministic else load().cr_interpolate_backward
+        _check(fn(self.cr.ctx, tri_image.data_ptr(), W, H, pos.data_ptr(), pos.shape[0], attr.data_ptr(),
+                  attr.shape[1], tris.data_ptr(), tris.shape[0], grad_out.data_ptr(), gp.data_ptr(), ga.data_ptr(),
+                  _stream()), self.cr.ctx)
         return gp, ga
 
     def sync(self):

@@ -157,6 +157,8 @@ typedef struct cr_counters {
     int64_t retries;             /* capacity grow-and-retry rounds */
     int64_t visible_subtris;     /* subtriangles that survive setup's culls;
                                     counted only by draws with count_fragments set, else 0 */
+    int64_t tile_list_entries;   /* (subtriangle, tile) entries of the tile lists; tile_entries
+                                    is the room reserved for them (whole 32-entry segments) */
 } cr_counters;
 
 /* ProfilingMode (cudaraster/cuda/PrivateDefs.hpp:158-204, CudaRaster.cpp:296-323):
@@ -433,6 +435,17 @@ cr_status cr_interpolate_backward(cr_context* ctx, const int32_t* tri_image, int
                                   const float* pos, int64_t num_vertices, const float* attr,
                                   int32_t num_channels, const int32_t* tris, int32_t num_tris,
                                   const float* grad_out, float* grad_pos, float* grad_attr, void* stream);
+/* The same gradients, bit-reproducible from run to run: each (tile,
+ * triangle) FP64 sum is converted to a 128-bit fixed-point integer (2^-64
+ * resolution; sums must stay below 2^63 in magnitude) and accumulated with
+ * integer atomics, whose total does not depend on their order, then rounded
+ * to FP32 once.  Uses a context scratch buffer of 16 bytes per output value
+ * (grown with hipMalloc when needed). */
+cr_status cr_interpolate_backward_deterministic(cr_context* ctx, const int32_t* tri_image, int32_t width,
+                                                int32_t height, const float* pos, int64_t num_vertices,
+                                                const float* attr, int32_t num_channels, const int32_t* tris,
+                                                int32_t num_tris, const float* grad_out, float* grad_pos,
+                                                float* grad_attr, void* stream);
 
 /* Passthrough vertex shader, the step before drawTriangles: replaces
  * vertexShader_passthrough (test/shader/PassThrough.cu:16-35) as the demo

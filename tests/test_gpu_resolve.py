@@ -98,8 +98,8 @@ def test_random_frames_match_goldens(name):
 
 
 def test_resolve_rejects_wrong_outputs():
-    """resolve(out=...) checks size, dtype, layout and the surface format
-    before the library call."""
+    """resolve(out=...) checks size, dtype and layout before the library
+    call; the library refuses a surface that is not RGBA8."""
     torch = _torch()
     import cudaraster
     col = cudaraster.CudaSurface((40, 24), cudaraster.FORMAT_RGBA8, 4)
@@ -112,7 +112,7 @@ def test_resolve_rejects_wrong_outputs():
                     torch.empty((24, 40), dtype=torch.int32)):
             with pytest.raises(ValueError):
                 col.resolve(out)
-        with pytest.raises(ValueError):
+        with pytest.raises(cudaraster.CrError):
             dep.resolve()
     finally:
         col.close()

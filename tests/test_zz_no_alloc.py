@@ -69,13 +69,14 @@ def test_noalloc_matches_draw_twice_smaller_second():
 
 
 def test_noalloc_regrows_queues_inside_the_array():
-    """400 near-full-screen triangles at 1024^2 need ~6.5M tile entries,
-    more than the array's first division gives the tile lists: the draw must
-    re-divide the array by the exact demand and still match."""
+    """900 near-full-screen triangles at 1024^2 need about 9.5M tile-list
+    entries (38 MB), more than the array's first division gives the tile
+    lists (about 7.5M) and less than the whole array: the draw must re-divide
+    the array by the exact demand and still match."""
     _torch()
     from cudaraster import scenes
-    sc = scenes.skewed(num_small=2_000, num_big=400, width=1024, height=1024, samples_log2=0, seed=43)
-    print("no-alloc: skewed 2k + 400 big, 1024^2", flush=True)
+    sc = scenes.skewed(num_small=2_000, num_big=900, width=1024, height=1024, samples_log2=0, seed=43)
+    print("no-alloc: skewed 2k + 900 big, 1024^2", flush=True)
     ref, got, counters = _draw_both(sc)
     _assert_same(ref, got, "skewed")
     assert counters["retries"] >= 1
