@@ -210,7 +210,8 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
   uint32_t* rec = recAll + wave * (kChunk * RS);
   const int tile = group * W + wave;
   if (tile >= P.numTiles) return;
-  const uint32_t cnt = P.tileCount[tile];
+  // Wave-uniform (one tile per wave): held in scalar registers.
+  const uint32_t cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)P.tileCount[tile]);
   if (cnt == 0 && !P.deferredClear) {
     if (COUNT && lane == 0) P.tileFrags[tile] = 0;
     return;
@@ -255,7 +256,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
   uint64_t pHiz = 0, pWalked = 0, pEmpty = 0, pSamples = 0, pZKill = 0, pWritten = 0;
   // The tile's list: a chain of kTileSeg-entry segments, one per chunk.
   static_assert(kChunk == kTileSeg, "one list segment per chunk");
-  uint32_t seg = P.tileStart[tile];
+  uint32_t seg = (uint32_t)__builtin_amdgcn_readfirstlane((int)P.tileStart[tile]);
   const bool hiz = DEPTH && P.hiz;
 
   for (uint32_t c = 0; c < cnt; c += kChunk) {
@@ -276,7 +277,8 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
     uint32_t slot = 0;
     uint4 h = make_uint4(0, 0, 0, 0), z = make_uint4(0, 0, 0, 0);
     // The next segment, read now so that its latency overlaps the chunk.
-    const uint32_t nextSeg = (c + kChunk < cnt) ? P.tileSegNext[seg] : kNoSeg;
+    const uint32_t nextSeg =
+        (c + kChunk < cnt) ? (uint32_t)__builtin_amdgcn_readfirstlane((int)P.tileSegNext[seg]) : kNoSeg;
     if (lane < (int)n) {
       slot = tileList[((size_t)seg << kTileSegLog2) + lane];
       if (slot < (uint32_t)P.maxSubtris) {
