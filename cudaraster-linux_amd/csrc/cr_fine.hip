@@ -11,12 +11,13 @@
 //    visibility, triangle id) in registers for the whole tile, so fragments
 //    of one pixel are applied in order by construction -- no intra-wave
 //    conflict resolution and exactly one coalesced read/write of the tile;
-//  * the tile's triangle list is consumed in chunks of 32: lane j sets up
-//    triangle j (tile-relative edge thresholds per sample, depth plane per
-//    sample, hierarchical-Z test), survivors are compacted into an LDS
-//    record array, and the wave then walks the records with broadcast LDS
-//    reads -- per (triangle, tile) the lanes only do integer compares, one
-//    depth add per sample and register selects;
+//  * the tile's triangle list is consumed in chunks of 32: lane j tests
+//    entry j against the tile's hierarchical Z; the survivors, gathered over
+//    chunks into batches of up to 32, are set up one per lane (tile-relative
+//    edge thresholds per sample, depth plane per sample) into an LDS record
+//    array, and the wave then walks the records with broadcast LDS reads --
+//    per (triangle, tile) the lanes only do integer compares, one depth add
+//    per sample and register selects;
 //  * for Replace and DepthOnly blending the colour written to a sample is
 //    the shaded colour of the last fragment that passed there, so shading is
 //    deferred: the loop records (subtriangle slot, centroid) per sample and
@@ -89,9 +90,9 @@ __device__ __forceinline__ uint32_t waveMax(uint32_t v) {
   return v;
 }
 
-__device__ __forceinline__ uint64_t laneMaskLt64() {
-  const uint32_t lane = threadIdx.x & 63;
-  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+// Set bits of `mask` below this lane (v_mbcnt: no 64-bit lane mask is built).
+__device__ __forceinline__ uint32_t rankBelow(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
 // Sample order by distance to the pixel centre, lowest index first on ties:
@@ -153,6 +154,7 @@ __device__ __forceinline__ float4 varying0(const uint8_t* verts, uint32_t vi, in
 
 constexpr int kFineWaves = 4;     // tiles (waves) per workgroup
 constexpr int kChunk = 32;        // triangles set up per chunk
+constexpr int kMaxDefer = 8;      // chunks hierarchical-Z survivors may wait before their walk
 
 // LDS ordering inside one wave: every lane's record stores must land before
 // any lane reads them; a wavefront-scope fence plus the wave barrier orders
@@ -206,7 +208,8 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
 
   if (P.counters->overflow) return;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  // Wave-uniform, so the tile's coordinates and addresses are scalars.
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   uint32_t* rec = recAll + wave * (kChunk * RS);
   const int tile = group * W + wave;
   if (tile >= P.numTiles) return;
@@ -254,39 +257,71 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
   // covered samples, fragments whose covered samples all fail the depth
   // test, fragments written.
   uint64_t pHiz = 0, pWalked = 0, pEmpty = 0, pSamples = 0, pZKill = 0, pWritten = 0;
-  // The tile's list: a chain of kTileSeg-entry segments, one per chunk.
+  // The tile's list: a chain of kTileSeg-entry segments, one per chunk, read
+  // one chunk ahead -- the next chunk's entries and the link after them are
+  // in flight while the current chunk is tested (three loads in parallel
+  // instead of a chain of three per chunk).
   static_assert(kChunk == kTileSeg, "one list segment per chunk");
-  uint32_t seg = (uint32_t)__builtin_amdgcn_readfirstlane((int)P.tileStart[tile]);
+  // (The links stay in vector registers: a scalar copy would wait for each
+  // load as soon as it is issued.)
+  uint32_t cur = 0, segNext = kNoSeg;
+  if (cnt > 0) {
+    const uint32_t seg0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)P.tileStart[tile]);
+    if (lane < (int)min((uint32_t)kChunk, cnt)) cur = tileList[((size_t)seg0 << kTileSegLog2) + lane];
+    if (cnt > (uint32_t)kChunk) segNext = P.tileSegNext[seg0];
+  }
   const bool hiz = DEPTH && P.hiz;
 
+  // Hierarchical Z: a triangle whose conservative zmin is not nearer than
+  // the tile's farthest stored depth fails LESS at every sample.  The stored
+  // depths change only when records are walked, so the tile's maximum (a
+  // scalar) is recomputed only after a walk.
+  uint32_t tileZMax = 0xFFFFFFFFu;
+  bool zStale = true;
+  // Survivors of hierarchical Z are set up in batches: their slots queue up
+  // (in list order) until a batch of kChunk has gathered, the list ends or
+  // kMaxDefer chunks have passed, so that a tile whose entries are mostly
+  // culled pays for one set-up pass per batch rather than one per chunk.
+  // The depth bound a queued entry was tested against is at least the
+  // current one (depths only decrease), so the test stays conservative.
+  // Queue entry p lives in record p % kChunk, in a word that set-up never
+  // writes (Q0 for p < kChunk, Q1 above): the queue outlives the first
+  // batch's records, and up to 2 * kChunk - 1 entries fit.
+  constexpr int Q0 = VIS ? L::TRI : L::SLOT, Q1 = RS - 1;
+  static_assert(Q0 != Q1 && Q1 >= L::SH + (VIS ? 0 : 21), "two record words free for the queue");
+  uint32_t queued = 0, sinceWalk = 0;
+
   for (uint32_t c = 0; c < cnt; c += kChunk) {
-    // ---- lane j < kChunk sets up triangle j of this chunk ------------------
-    // Hierarchical Z: a triangle whose conservative zmin is not nearer than
-    // the tile's farthest stored depth fails LESS at every sample.
-    uint32_t tileZMax = 0xFFFFFFFFu;
-    if (hiz && c == 0 && P.deferredClear) {
-      tileZMax = P.clearDepth;               // every sample still holds the clear depth
-    } else if (hiz) {
-      uint32_t mx = dep[0];
+    // ---- lane j < kChunk loads and tests entry j of this chunk -------------
+    if (hiz && zStale) {
+      if (c == 0 && P.deferredClear) {
+        tileZMax = P.clearDepth;             // every sample still holds the clear depth
+      } else {
+        uint32_t mx = dep[0];
 #pragma unroll
-      for (int i = 1; i < N; i++) mx = max(mx, dep[i]);
-      tileZMax = waveMax(mx);
+        for (int i = 1; i < N; i++) mx = max(mx, dep[i]);
+        tileZMax = (uint32_t)__builtin_amdgcn_readfirstlane((int)waveMax(mx));
+      }
+      zStale = false;
     }
     const uint32_t n = min((uint32_t)kChunk, cnt - c);
-    bool keep = false;
-    uint32_t slot = 0;
-    uint4 h = make_uint4(0, 0, 0, 0), z = make_uint4(0, 0, 0, 0);
-    // The next segment, read now so that its latency overlaps the chunk.
-    const uint32_t nextSeg =
-        (c + kChunk < cnt) ? (uint32_t)__builtin_amdgcn_readfirstlane((int)P.tileSegNext[seg]) : kNoSeg;
-    if (lane < (int)n) {
-      slot = tileList[((size_t)seg << kTileSegLog2) + lane];
-      if (slot < (uint32_t)P.maxSubtris) {
-        h = hdr[slot];
-        if (DEPTH) z = tdata[(size_t)slot * 4];
-        keep = !hiz || ((h.w & 0xFFFFF000u) < tileZMax);
-      }
-    }
+    const bool lastChunk = c + kChunk >= cnt;
+    uint32_t slot = cur;
+    const bool valid = lane < (int)n && slot < (uint32_t)P.maxSubtris;
+    // The header's last word: the triangle's conservative zmin (high 20
+    // bits), issued before the next chunk's loads so that its wait does not
+    // cover them, and by every lane (slot 0 stands in for an empty one): a
+    // load under a branch would be waited for at the branch's end.
+    const uint32_t zmin = reinterpret_cast<const uint32_t*>(hdr)[(size_t)(valid ? slot : 0u) * 4 + 3];
+    // The next chunk's entries and link, loaded by every lane without a
+    // branch (so that waiting for zmin does not wait for them); after the
+    // last chunk, segment 0 (always allocated) stands in.  Entries past the
+    // list's end and a link past its last segment are never used.
+    const uint32_t segSafe = lastChunk ? 0u : segNext;
+    const uint32_t nxt = tileList[((size_t)segSafe << kTileSegLog2) + lane];
+    const uint32_t segAfter = P.tileSegNext[segSafe];
+    // (Bitwise, not short-circuit: the load's value is used on every path.)
+    const bool keep = valid & (!hiz | ((zmin & 0xFFFFF000u) < tileZMax));
     const uint64_t keepMask = __ballot(keep);
     const uint32_t m = (uint32_t)__popcll(keepMask);
     if (COUNT) {
@@ -294,219 +329,243 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
       pWalked += m;
     }
     if (keep) {
-      uint32_t* r = rec + (uint32_t)__popcll(keepMask & laneMaskLt64()) * RS;
-      int32_t ax[3], ay[3];
-      ax[0] = (int32_t)(int16_t)(h.x & 0xFFFFu) - Tx; ay[0] = (int32_t)(int16_t)(h.x >> 16) - Ty;
-      ax[1] = (int32_t)(int16_t)(h.y & 0xFFFFu) - Tx; ay[1] = (int32_t)(int16_t)(h.y >> 16) - Ty;
-      ax[2] = (int32_t)(int16_t)(h.z & 0xFFFFu) - Tx; ay[2] = (int32_t)(int16_t)(h.z >> 16) - Ty;
-      // Vertices within 2^14 subpixels of the tile origin (all but large
-      // triangles): a x d fits 31 bits, so it is two 24-bit products and
-      // needs no clamp.  Uniform over the wave's set-up lanes.
-      const uint32_t span = (uint32_t)(abs(ax[0]) | abs(ay[0]) | abs(ax[1]) | abs(ay[1]) | abs(ax[2]) | abs(ay[2]));
-      const bool near = __ballot(span >= (1u << 14)) == 0;
-      // Whole tile covered: at every sample of every pixel each edge's
-      // L = lx16*dy + ly16*(-dx) (lx16, ly16 in 0..112) is at most the
-      // sample's threshold, i.e. its largest value over the tile is at most
-      // the smallest threshold.  Exactly the per-sample test's outcome.
-      bool full = true;
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        const int kn = (k + 1) % 3;
-        const int32_t dx = ax[kn] - ax[k], dy = ay[kn] - ay[k];
-        // Edge function at the tile origin, a x d, minus 1 on exclusive
-        // edges (Util.inl:100).  |a|,|d| <= 2^16.  Clamping to +-2^30
-        // keeps the sign of a x d - s x d exact for every in-tile sample
-        // (|s x d| < 2^25), and every threshold below fits in 32 bits.
-        // With |a| < 2^14 (so |d| < 2^15), |a x d| < 2^30: the clamp never
-        // applies and 32-bit arithmetic gives the same value.
-        const int32_t excl = (dy > 0 || (dy == 0 && dx <= 0)) ? 1 : 0;
-        int32_t ck;
-        if (near) {
-          ck = __mul24(ax[k], dy) - __mul24(ay[k], dx) - excl;
-        } else {
-          int64_t cc = (int64_t)ax[k] * dy - (int64_t)ay[k] * dx - excl;
-          cc = cc > (1ll << 30) ? (1ll << 30) : (cc < -(1ll << 30) ? -(1ll << 30) : cc);
-          ck = (int32_t)cc;
-        }
-        int32_t minR = INT32_MAX;
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-          const int32_t ox = (2 * msaaPattern(S, i) + 1) << (3 - S);
-          const int32_t oy = (2 * i + 1) << (3 - S);
-          // |ox|, |oy| < 16 and |dx|, |dy| < 2^17: exact 24-bit products.
-          const int32_t rk = ck - (__mul24(ox, dy) - __mul24(oy, dx));
-          r[L::R + k * N + i] = (uint32_t)rk;
-          minR = min(minR, rk);
-        }
-        full = full && __mul24(112, max(dy, 0)) + __mul24(112, max(-dx, 0)) <= minR;
-        r[L::D + 2 * k] = (uint32_t)dy;
-        r[L::D + 2 * k + 1] = (uint32_t)(-dx);
-      }
-      if (DEPTH) {
-        r[L::Z] = (z.x & 0xFFFFu) | (z.y << 16);
-        r[L::Z + 1] = (z.x >> 16) | (z.y & 0xFFFF0000u);
-        // Depth at each sample of the tile's first pixel: one full 32-bit
-        // product per axis, then the sample offsets (< 8) as shifts and adds.
-        const uint32_t base = z.x * tileSx + z.y * tileSy + z.z;
-        sampleDepths<S>(r + L::ZS, base, z.x, z.y, std::make_integer_sequence<int, N>{});
-      }
-      // Deferred shading records the (slot << 4 | centroid code) words the
-      // walk stores per sample; at 1x the code is always the pixel centre.
-      r[L::SLOT] = VIS ? ((slot << 4) | (N == 1 ? 15u : 0u)) : slot;
-      // In-order shading: the source triangle, bit 31 set when the triangle
-      // covers the whole tile (source indices stay below 2^28).
-      if (!VIS)
-        r[L::TRI] = ((slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris]) |
-                    (full ? 0x80000000u : 0u);
-      if (TEX) {
-        // Barycentric planes and vertex indices; the shader reads the
-        // varyings, material and textures itself.
-        const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
-                    t3 = tdata[(size_t)slot * 4 + 3];
-        uint32_t* q = r + L::SH;
-        q[0] = t1.x; q[1] = t1.y; q[2] = t1.z; q[3] = t1.w; q[4] = t2.x; q[5] = t2.y; q[6] = t2.z;
-        q[7] = t2.w; q[8] = t3.x; q[9] = t3.y; q[10] = t3.z; q[11] = t3.w;
-      } else if (!VIS && gouraud) {
-        const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
-                    t3 = tdata[(size_t)slot * 4 + 3];
-        const float4 c0 = varying0(verts, t3.y, P.vertexStride), c1 = varying0(verts, t3.z, P.vertexStride),
-                     c2 = varying0(verts, t3.w, P.vertexStride);
-        uint32_t* q = r + L::SH;
-        q[0] = t1.x; q[1] = t1.y; q[2] = t1.z; q[3] = t1.w; q[4] = t2.x; q[5] = t2.y; q[6] = t2.z;
-        q[7] = t2.w; q[8] = t3.x;
-        q[9] = __float_as_uint(c0.x); q[10] = __float_as_uint(c0.y); q[11] = __float_as_uint(c0.z);
-        q[12] = __float_as_uint(c0.w); q[13] = __float_as_uint(c1.x); q[14] = __float_as_uint(c1.y);
-        q[15] = __float_as_uint(c1.z); q[16] = __float_as_uint(c1.w); q[17] = __float_as_uint(c2.x);
-        q[18] = __float_as_uint(c2.y); q[19] = __float_as_uint(c2.z); q[20] = __float_as_uint(c2.w);
-      }
+      const uint32_t p = queued + rankBelow(keepMask);
+      rec[(p & (kChunk - 1)) * RS + (p < (uint32_t)kChunk ? Q0 : Q1)] = slot;
     }
-    waveSync();
+    queued += m;
+    cur = nxt;
+    segNext = segAfter;                      // used only while c + 2 * kChunk < cnt
+    const bool walkNow = !hiz || lastChunk || queued >= (uint32_t)kChunk || ++sinceWalk >= (uint32_t)kMaxDefer;
+    if (!walkNow || queued == 0) continue;
+    const uint32_t total = queued;
+    queued = 0;
+    sinceWalk = 0;
+    zStale = true;
 
-    // ---- walk the surviving triangles in submission order ----------------
-    for (uint32_t j = 0; j < m; j++) {
-      const uint32_t* r = rec + j * RS;
-      const uint32_t* cur = r;
-      // |lx16|,|ly16| < 2^7 and |d| < 2^17: 24-bit multiplies are exact.
-      // In-order shading: a record covering the whole tile skips the
-      // per-sample edge tests (a scalar branch: the flag is uniform).
-      const uint32_t triWord = VIS ? 0u : (uint32_t)__builtin_amdgcn_readfirstlane((int)r[L::TRI]);
-      const bool fullTile = !VIS && (triWord >> 31) != 0;
-      // Per-sample coverage and write masks are kept as explicit 64-bit lane
-      // masks (scalar registers): a per-lane bool carried from one block to
-      // the next would be rebuilt from a vector register before each ballot.
-      uint64_t covM[N];
-      uint64_t anyM = 0;
-      if (fullTile) {
+    for (uint32_t b = 0; b < total; b += kChunk) {
+      // ---- set up the batch: lane j, record j, queue entry b + j -------------
+      const uint32_t nb = min((uint32_t)kChunk, total - b);
+      uint32_t* r = rec + lane * RS;
+      const bool su = lane < (int)nb;
+      waveSync();                            // the queue stores (or the last walk's reads) are done
+      if (su) {
+        slot = r[b == 0 ? Q0 : Q1];
+        // Header and depth plane, read again here (cache hits): only the
+        // set-up lanes of a batch hold them.
+        const uint4 h = hdr[slot];
+        const uint4 z = DEPTH ? tdata[(size_t)slot * 4] : make_uint4(0, 0, 0, 0);
+        int32_t ax[3], ay[3];
+        ax[0] = (int32_t)(int16_t)(h.x & 0xFFFFu) - Tx; ay[0] = (int32_t)(int16_t)(h.x >> 16) - Ty;
+        ax[1] = (int32_t)(int16_t)(h.y & 0xFFFFu) - Tx; ay[1] = (int32_t)(int16_t)(h.y >> 16) - Ty;
+        ax[2] = (int32_t)(int16_t)(h.z & 0xFFFFu) - Tx; ay[2] = (int32_t)(int16_t)(h.z >> 16) - Ty;
+        // Vertices within 2^14 subpixels of the tile origin (all but large
+        // triangles): a x d fits 31 bits, so it is two 24-bit products and
+        // needs no clamp.  Uniform over the wave's set-up lanes.
+        const uint32_t span = (uint32_t)(abs(ax[0]) | abs(ay[0]) | abs(ax[1]) | abs(ay[1]) | abs(ax[2]) | abs(ay[2]));
+        const bool near = __ballot(span >= (1u << 14)) == 0;
+        // Whole tile covered: at every sample of every pixel each edge's
+        // L = lx16*dy + ly16*(-dx) (lx16, ly16 in 0..112) is at most the
+        // sample's threshold, i.e. its largest value over the tile is at most
+        // the smallest threshold.  Exactly the per-sample test's outcome.
+        bool full = true;
 #pragma unroll
-        for (int i = 0; i < N; i++) covM[i] = ~0ull;
-        anyM = ~0ull;
-      } else {
-        int32_t Lk[3];
+        for (int k = 0; k < 3; k++) {
+          const int kn = (k + 1) % 3;
+          const int32_t dx = ax[kn] - ax[k], dy = ay[kn] - ay[k];
+          // Edge function at the tile origin, a x d, minus 1 on exclusive
+          // edges (Util.inl:100).  |a|,|d| <= 2^16.  Clamping to +-2^30
+          // keeps the sign of a x d - s x d exact for every in-tile sample
+          // (|s x d| < 2^25), and every threshold below fits in 32 bits.
+          // With |a| < 2^14 (so |d| < 2^15), |a x d| < 2^30: the clamp never
+          // applies and 32-bit arithmetic gives the same value.
+          const int32_t excl = (dy > 0 || (dy == 0 && dx <= 0)) ? 1 : 0;
+          int32_t ck;
+          if (near) {
+            ck = __mul24(ax[k], dy) - __mul24(ay[k], dx) - excl;
+          } else {
+            int64_t cc = (int64_t)ax[k] * dy - (int64_t)ay[k] * dx - excl;
+            cc = cc > (1ll << 30) ? (1ll << 30) : (cc < -(1ll << 30) ? -(1ll << 30) : cc);
+            ck = (int32_t)cc;
+          }
+          int32_t minR = INT32_MAX;
 #pragma unroll
-        for (int k = 0; k < 3; k++)
-          Lk[k] = __mul24(lx16, (int32_t)cur[L::D + 2 * k]) + __mul24(ly16, (int32_t)cur[L::D + 2 * k + 1]);
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-          covM[i] = __ballot(Lk[0] <= (int32_t)cur[L::R + i]) & __ballot(Lk[1] <= (int32_t)cur[L::R + N + i]) &
-                    __ballot(Lk[2] <= (int32_t)cur[L::R + 2 * N + i]);
-          anyM |= covM[i];
+          for (int i = 0; i < N; i++) {
+            const int32_t ox = (2 * msaaPattern(S, i) + 1) << (3 - S);
+            const int32_t oy = (2 * i + 1) << (3 - S);
+            // |ox|, |oy| < 16 and |dx|, |dy| < 2^17: exact 24-bit products.
+            const int32_t rk = ck - (__mul24(ox, dy) - __mul24(oy, dx));
+            r[L::R + k * N + i] = (uint32_t)rk;
+            minR = min(minR, rk);
+          }
+          full = full && __mul24(112, max(dy, 0)) + __mul24(112, max(-dx, 0)) <= minR;
+          r[L::D + 2 * k] = (uint32_t)dy;
+          r[L::D + 2 * k + 1] = (uint32_t)(-dx);
+        }
+        if (DEPTH) {
+          r[L::Z] = (z.x & 0xFFFFu) | (z.y << 16);
+          r[L::Z + 1] = (z.x >> 16) | (z.y & 0xFFFF0000u);
+          // Depth at each sample of the tile's first pixel: one full 32-bit
+          // product per axis, then the sample offsets (< 8) as shifts and adds.
+          const uint32_t base = z.x * tileSx + z.y * tileSy + z.z;
+          sampleDepths<S>(r + L::ZS, base, z.x, z.y, std::make_integer_sequence<int, N>{});
+        }
+        // Deferred shading records the (slot << 4 | centroid code) words the
+        // walk stores per sample; at 1x the code is always the pixel centre.
+        if (VIS) r[L::SLOT] = (slot << 4) | (N == 1 ? 15u : 0u);
+        // In-order shading: the source triangle, bit 31 set when the triangle
+        // covers the whole tile (source indices stay below 2^28).
+        if (!VIS)
+          r[L::TRI] = ((slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris]) |
+                      (full ? 0x80000000u : 0u);
+        if (TEX) {
+          // Barycentric planes and vertex indices; the shader reads the
+          // varyings, material and textures itself.
+          const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
+                      t3 = tdata[(size_t)slot * 4 + 3];
+          uint32_t* q = r + L::SH;
+          q[0] = t1.x; q[1] = t1.y; q[2] = t1.z; q[3] = t1.w; q[4] = t2.x; q[5] = t2.y; q[6] = t2.z;
+          q[7] = t2.w; q[8] = t3.x; q[9] = t3.y; q[10] = t3.z; q[11] = t3.w;
+        } else if (!VIS && gouraud) {
+          const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
+                      t3 = tdata[(size_t)slot * 4 + 3];
+          const float4 c0 = varying0(verts, t3.y, P.vertexStride), c1 = varying0(verts, t3.z, P.vertexStride),
+                       c2 = varying0(verts, t3.w, P.vertexStride);
+          uint32_t* q = r + L::SH;
+          q[0] = t1.x; q[1] = t1.y; q[2] = t1.z; q[3] = t1.w; q[4] = t2.x; q[5] = t2.y; q[6] = t2.z;
+          q[7] = t2.w; q[8] = t3.x;
+          q[9] = __float_as_uint(c0.x); q[10] = __float_as_uint(c0.y); q[11] = __float_as_uint(c0.z);
+          q[12] = __float_as_uint(c0.w); q[13] = __float_as_uint(c1.x); q[14] = __float_as_uint(c1.y);
+          q[15] = __float_as_uint(c1.z); q[16] = __float_as_uint(c1.w); q[17] = __float_as_uint(c2.x);
+          q[18] = __float_as_uint(c2.y); q[19] = __float_as_uint(c2.z); q[20] = __float_as_uint(c2.w);
         }
       }
-      if (COUNT) {
-        pEmpty += anyM == 0;
-#pragma unroll
-        for (int i = 0; i < N; i++) pSamples += (uint64_t)__popcll(covM[i]);
-      }
-      if (anyM == 0) continue;
-      if (COUNT) frags += (uint64_t)__popcll(anyM);
+      waveSync();
 
-      uint64_t wrM[N];
-      uint32_t dz[N];
-      if (DEPTH) {
-        // zx*lxN + zy*lyN mod 2^32 from the 16-bit halves of zx and zy, one
-        // dot product per half (lxN, lyN < 64: no product or sum overflows).
-        const uint32_t pl = shl16Add(dot2u(r[L::Z + 1], pixN, 0u), dot2u(r[L::Z], pixN, 0u));
-        uint64_t anyWrM = 0;
+      // ---- walk the batch's triangles in submission order --------------------
+      for (uint32_t j = 0; j < nb; j++) {
+        const uint32_t* r = rec + j * RS;
+        const uint32_t* cur = r;
+        // |lx16|,|ly16| < 2^7 and |d| < 2^17: 24-bit multiplies are exact.
+        // In-order shading: a record covering the whole tile skips the
+        // per-sample edge tests (a scalar branch: the flag is uniform).
+        const uint32_t triWord = VIS ? 0u : (uint32_t)__builtin_amdgcn_readfirstlane((int)r[L::TRI]);
+        const bool fullTile = !VIS && (triWord >> 31) != 0;
+        // Per-sample coverage and write masks are kept as explicit 64-bit lane
+        // masks (scalar registers): a per-lane bool carried from one block to
+        // the next would be rebuilt from a vector register before each ballot.
+        uint64_t covM[N];
+        uint64_t anyM = 0;
+        if (fullTile) {
 #pragma unroll
-        for (int i = 0; i < N; i++) {
-          dz[i] = r[L::ZS + i] + pl;
-          wrM[i] = covM[i] & __ballot(dz[i] < dep[i]);     // LESS (FineRaster.inl:697)
-          anyWrM |= wrM[i];
+          for (int i = 0; i < N; i++) covM[i] = ~0ull;
+          anyM = ~0ull;
+        } else {
+          int32_t Lk[3];
+#pragma unroll
+          for (int k = 0; k < 3; k++)
+            Lk[k] = __mul24(lx16, (int32_t)cur[L::D + 2 * k]) + __mul24(ly16, (int32_t)cur[L::D + 2 * k + 1]);
+#pragma unroll
+          for (int i = 0; i < N; i++) {
+            covM[i] = __ballot(Lk[0] <= (int32_t)cur[L::R + i]) & __ballot(Lk[1] <= (int32_t)cur[L::R + N + i]) &
+                      __ballot(Lk[2] <= (int32_t)cur[L::R + 2 * N + i]);
+            anyM |= covM[i];
+          }
         }
         if (COUNT) {
-          pZKill += (uint64_t)__popcll(anyM & ~anyWrM);
-          pWritten += (uint64_t)__popcll(anyWrM);
+          pEmpty += anyM == 0;
+#pragma unroll
+          for (int i = 0; i < N; i++) pSamples += (uint64_t)__popcll(covM[i]);
         }
-        if (anyWrM == 0) continue;
-      } else {
-#pragma unroll
-        for (int i = 0; i < N; i++) { wrM[i] = covM[i]; dz[i] = 0; }
-        if (COUNT) pWritten += (uint64_t)__popcll(anyM);
-      }
-      bool cov[N], wr[N];
-      bool anyWr = false;
-#pragma unroll
-      for (int i = 0; i < N; i++) {
-        cov[i] = __builtin_amdgcn_inverse_ballot_w64(covM[i]);
-        wr[i] = __builtin_amdgcn_inverse_ballot_w64(wrM[i]);
-        anyWr = anyWr | wr[i];
-      }
-      // Centroid (selectMSAACentroid): first covered sample in distance
-      // order, the pixel centre (15) when none or all are covered.
-      int code = 15;
-      if (!fullTile) {
-        bool all = true;
-#pragma unroll
-        for (int k = N - 1; k >= 0; k--) code = cov[co.order[k]] ? co.order[k] : code;
-#pragma unroll
-        for (int i = 0; i < N; i++) all = all & cov[i];
-        if (all) code = 15;
-      }
+        if (anyM == 0) continue;
+        if (COUNT) frags += (uint64_t)__popcll(anyM);
 
-      if (VIS) {
-        const uint32_t v = N == 1 ? r[L::SLOT] : (r[L::SLOT] | (uint32_t)code);
+        uint64_t wrM[N];
+        uint32_t dz[N];
+        if (DEPTH) {
+          // zx*lxN + zy*lyN mod 2^32 from the 16-bit halves of zx and zy, one
+          // dot product per half (lxN, lyN < 64: no product or sum overflows).
+          const uint32_t pl = shl16Add(dot2u(r[L::Z + 1], pixN, 0u), dot2u(r[L::Z], pixN, 0u));
+          uint64_t anyWrM = 0;
 #pragma unroll
-        for (int i = 0; i < N; i++) {
-          if (DEPTH) dep[i] = wr[i] ? dz[i] : dep[i];
-          aux[i] = wr[i] ? v : aux[i];
-        }
-      } else {
-        uint32_t src = P.constColor;
-        const uint32_t tri = triWord & 0x7FFFFFFFu;
-        bool discard = false;
-        if (TEX) {
-          if (anyWr) {
-            const uint32_t* q = r + L::SH;
-            const Planes pl = {(int32_t)q[0], (int32_t)q[1], (int32_t)q[2], (int32_t)q[3], (int32_t)q[4],
-                               (int32_t)q[5], (int32_t)q[6], (int32_t)q[7], (int32_t)q[8]};
-            discard = shadeTexPhong<S>(P, pl, q[9], q[10], q[11], tri, code, px, py, &src);
+          for (int i = 0; i < N; i++) {
+            dz[i] = r[L::ZS + i] + pl;
+            wrM[i] = covM[i] & __ballot(dz[i] < dep[i]);     // LESS (FineRaster.inl:697)
+            anyWrM |= wrM[i];
           }
-        } else if (anyWr && gouraud) {
-          const uint32_t* q = r + L::SH;
-          const uint4 t1 = make_uint4(q[0], q[1], q[2], q[3]);
-          const uint4 t2 = make_uint4(q[4], q[5], q[6], q[7]);
-          const uint4 t3 = make_uint4(q[8], 0, 0, 0);
-          const float4 c0 = make_float4(__uint_as_float(q[9]), __uint_as_float(q[10]), __uint_as_float(q[11]),
-                                        __uint_as_float(q[12]));
-          const float4 c1 = make_float4(__uint_as_float(q[13]), __uint_as_float(q[14]), __uint_as_float(q[15]),
-                                        __uint_as_float(q[16]));
-          const float4 c2 = make_float4(__uint_as_float(q[17]), __uint_as_float(q[18]), __uint_as_float(q[19]),
-                                        __uint_as_float(q[20]));
-          src = shadeGouraud<S>(lerpOn, code, px, py, t1, t2, t3, c0, c1, c2);
+          if (COUNT) {
+            pZKill += (uint64_t)__popcll(anyM & ~anyWrM);
+            pWritten += (uint64_t)__popcll(anyWrM);
+          }
+          if (anyWrM == 0) continue;
+        } else {
+#pragma unroll
+          for (int i = 0; i < N; i++) { wrM[i] = covM[i]; dz[i] = 0; }
+          if (COUNT) pWritten += (uint64_t)__popcll(anyM);
         }
-        // A discarded fragment writes neither depth nor colour.
+        bool cov[N], wr[N];
+        bool anyWr = false;
 #pragma unroll
         for (int i = 0; i < N; i++) {
-          if (wr[i] && !discard) {
-            if (DEPTH) dep[i] = dz[i];
-            col[i] = blendPixel(blend, src, col[i]);
-            aux[i] = tri;
+          cov[i] = __builtin_amdgcn_inverse_ballot_w64(covM[i]);
+          wr[i] = __builtin_amdgcn_inverse_ballot_w64(wrM[i]);
+          anyWr = anyWr | wr[i];
+        }
+        // Centroid (selectMSAACentroid): first covered sample in distance
+        // order, the pixel centre (15) when none or all are covered.
+        int code = 15;
+        if (!fullTile) {
+          bool all = true;
+#pragma unroll
+          for (int k = N - 1; k >= 0; k--) code = cov[co.order[k]] ? co.order[k] : code;
+#pragma unroll
+          for (int i = 0; i < N; i++) all = all & cov[i];
+          if (all) code = 15;
+        }
+
+        if (VIS) {
+          const uint32_t v = N == 1 ? r[L::SLOT] : (r[L::SLOT] | (uint32_t)code);
+#pragma unroll
+          for (int i = 0; i < N; i++) {
+            if (DEPTH) dep[i] = wr[i] ? dz[i] : dep[i];
+            aux[i] = wr[i] ? v : aux[i];
+          }
+        } else {
+          uint32_t src = P.constColor;
+          const uint32_t tri = triWord & 0x7FFFFFFFu;
+          bool discard = false;
+          if (TEX) {
+            if (anyWr) {
+              const uint32_t* q = r + L::SH;
+              const Planes pl = {(int32_t)q[0], (int32_t)q[1], (int32_t)q[2], (int32_t)q[3], (int32_t)q[4],
+                                 (int32_t)q[5], (int32_t)q[6], (int32_t)q[7], (int32_t)q[8]};
+              discard = shadeTexPhong<S>(P, pl, q[9], q[10], q[11], tri, code, px, py, &src);
+            }
+          } else if (anyWr && gouraud) {
+            const uint32_t* q = r + L::SH;
+            const uint4 t1 = make_uint4(q[0], q[1], q[2], q[3]);
+            const uint4 t2 = make_uint4(q[4], q[5], q[6], q[7]);
+            const uint4 t3 = make_uint4(q[8], 0, 0, 0);
+            const float4 c0 = make_float4(__uint_as_float(q[9]), __uint_as_float(q[10]), __uint_as_float(q[11]),
+                                          __uint_as_float(q[12]));
+            const float4 c1 = make_float4(__uint_as_float(q[13]), __uint_as_float(q[14]), __uint_as_float(q[15]),
+                                          __uint_as_float(q[16]));
+            const float4 c2 = make_float4(__uint_as_float(q[17]), __uint_as_float(q[18]), __uint_as_float(q[19]),
+                                          __uint_as_float(q[20]));
+            src = shadeGouraud<S>(lerpOn, code, px, py, t1, t2, t3, c0, c1, c2);
+          }
+          // A discarded fragment writes neither depth nor colour.
+#pragma unroll
+          for (int i = 0; i < N; i++) {
+            if (wr[i] && !discard) {
+              if (DEPTH) dep[i] = dz[i];
+              col[i] = blendPixel(blend, src, col[i]);
+              aux[i] = tri;
+            }
           }
         }
       }
+      waveSync();
     }
-    waveSync();
-    seg = nextSeg;
   }
 
   // ---- deferred shading: once per (pixel, distinct winning triangle) -------
