@@ -153,6 +153,47 @@ def test_render_views_matches_single_views():
         dr.close()
 
 
+def test_render_views_full_size_32_views(oracle_lib):
+    """The differentiable workload at full size (32 views, 200k triangles,
+    512x512, 8 channels) through one render_views call: every view's ids
+    equal the oracle's, its image the FP64 interpolation and its gradients
+    the FP64 reference gradients of the summed L2 loss (1e-6 of the largest,
+    as for single views)."""
+    torch = _torch()
+    from concurrent.futures import ThreadPoolExecutor
+    import diff_ref
+    from cudaraster import scenes
+    from cudaraster.diff import DiffRasterizer
+    views = scenes.diff_views(32)
+    dr = DiffRasterizer(512, 512)
+    try:
+        pos = [torch.from_numpy(v.pos).cuda().requires_grad_(True) for v in views]
+        attr = [torch.from_numpy(v.attr).cuda().requires_grad_(True) for v in views]
+        tris = [torch.from_numpy(v.tris).cuda() for v in views]
+        target = torch.stack([torch.from_numpy(v.target).cuda() for v in views])
+        out, ids = dr.render_views(pos, attr, tris)
+        ((out - target) ** 2).sum().backward()
+        torch.cuda.synchronize()
+        got = [(ids[k].cpu().numpy(), out[k].detach().cpu().numpy(), pos[k].grad.cpu().numpy(),
+                attr[k].grad.cpu().numpy()) for k in range(len(views))]
+    finally:
+        dr.close()
+    ref_ids = [oracle_ids(oracle_lib, v) for v in views]
+
+    def reference(k):
+        v, i = views[k], ref_ids[k]
+        ref_out = diff_ref.interpolate(v.pos, v.attr, v.tris, i)
+        return (ref_out,) + tuple(diff_ref.backward(v.pos, v.attr, v.tris, i, 2.0 * (ref_out - v.target)))
+
+    with ThreadPoolExecutor(8) as ex:
+        refs = list(ex.map(reference, range(len(views))))
+    for k, ((g_ids, g_out, g_pos, g_attr), (r_out, r_pos, r_attr)) in enumerate(zip(got, refs)):
+        assert np.array_equal(g_ids, ref_ids[k]), f"view {k} ids"
+        assert np.abs(g_out - r_out).max() < 1e-6, f"view {k} image"
+        for name, g, r in (("pos", g_pos, r_pos), ("attr", g_attr, r_attr)):
+            assert np.abs(g - r).max() <= 1e-6 * np.abs(r).max() + 1e-30, f"view {k} {name} gradient"
+
+
 def test_async_visibility_overflow_is_reported_then_recovers():
     """After a small sizing view, a much larger view overflows the reserved
     queues in its asynchronous draw: sync() raises instead of returning
