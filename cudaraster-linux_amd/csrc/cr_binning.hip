@@ -32,9 +32,9 @@ namespace crhip {
 // need a quarter of the LDS, so several workgroups share a CU and a block
 // barrier waits for 4 waves instead of 16.
 
-__device__ __forceinline__ uint64_t lanesBelow() {
-  const uint32_t lane = threadIdx.x & 63;
-  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+// Set bits of `mask` below this lane (v_mbcnt).
+__device__ __forceinline__ uint32_t rankBelow(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
 template <int NT>
@@ -202,7 +202,7 @@ enum CoarseCase : int { kCaseA = 0, kCaseB = 1, kCaseC = 2 };
 // CoarseCase the wave took.
 __device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const int bl,
                                             uint64_t (*emitMask)[kBinTiles]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int w = valid ? r.x1 - r.x0 + 1 : 0, h = valid ? r.y1 - r.y0 + 1 : 0;
   const bool has = valid && w > 0 && h > 0;
   const uint64_t bit = 1ull << lane;
@@ -241,9 +241,13 @@ __device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const
   // Scattered boxes: small ones by their own lane, large ones by the wave.
   const bool big = has && w * h > kCoopArea;
   if (has && !big) {
-    for (int y = r.y0; y <= r.y1; y++)
-      for (int x = r.x0; x <= r.x1; x++)
-        if (r.hit(x, y)) atomicOr((unsigned long long*)&emitMask[wave][x + (y << bl)], (unsigned long long)bit);
+    // One loop over the box (row by row): as many wave steps as the
+    // largest small box has tiles.
+    int x = r.x0, y = r.y0;
+    for (int i = 0; i < w * h; i++) {
+      if (r.hit(x, y)) atomicOr((unsigned long long*)&emitMask[wave][x + (y << bl)], (unsigned long long)bit);
+      if (++x > r.x1) { x = r.x0; y++; }
+    }
   }
   for (uint64_t b = __ballot(big); b != 0; b &= b - 1) {
     const int j = (int)__builtin_ctzll(b);
@@ -272,9 +276,10 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   __shared__ uint32_t tTotal[kBinTiles];
   __shared__ uint32_t tFirst[kBinTiles];
   __shared__ uint32_t tLast[kBinTiles];
-  __shared__ uint32_t tOld[kBinTiles];
-  __shared__ uint32_t tOldLast[kBinTiles];
-  __shared__ uint32_t tNew[kBinTiles];
+  // The round's segment map of each tile (one LDS read per lookup):
+  // segments the list held before the round, its last one then, the
+  // round's first new one.
+  __shared__ uint4 tRound[kBinTiles];
   __shared__ uint32_t sums[kCoarseWaves + 1];
   __shared__ int32_t segOf[kCoarseThreads];
   __shared__ int32_t sCarry;
@@ -288,7 +293,7 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   if (prof && threadIdx.x < 8) sProf[threadIdx.x] = 0;
   const int bl = P.binLog2, tpb = 1 << (2 * bl);   // tiles per bin side log2, tiles per bin
   const int bx = bin % P.widthBins, by = bin / P.widthBins;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint32_t* bCnt = P.binCnt + (size_t)bin * P.numChunks;
   const uint32_t* bOff = P.binOff + (size_t)bin * P.numChunks;
   const uint32_t segCap = (uint32_t)P.maxTileEntries >> kTileSegLog2;   // segments the list buffer holds
@@ -357,7 +362,7 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
       // Rank: each wave's entries of tile t follow the earlier waves' and
       // the tile's list so far; the segments the round needs past the
       // tile's last, partly filled one.
-      uint32_t need = 0;
+      uint32_t need = 0, kOld = 0, oldLast = kNoSeg;
       if (threadIdx.x < (unsigned)tpb) {
         const uint32_t old = tTotal[threadIdx.x];
         uint32_t acc = old;
@@ -366,9 +371,9 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
           emitPre[q][threadIdx.x] = acc;
           acc += (uint32_t)__popcll(emitMask[q][threadIdx.x]);
         }
-        need = ((acc + kTileSeg - 1) >> kTileSegLog2) - ((old + kTileSeg - 1) >> kTileSegLog2);
-        tOld[threadIdx.x] = old;
-        tOldLast[threadIdx.x] = tLast[threadIdx.x];
+        kOld = (old + kTileSeg - 1) >> kTileSegLog2;
+        need = ((acc + kTileSeg - 1) >> kTileSegLog2) - kOld;
+        oldLast = tLast[threadIdx.x];
         tTotal[threadIdx.x] = acc;
         if (prof && acc != old) {
           atomicAdd(&sProf[2], acc - old);
@@ -389,40 +394,50 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
         sSegBase = base >> kTileSegLog2;
       }
       __syncthreads();
-      if (threadIdx.x < (unsigned)tpb && need > 0) {
-        // Chain the round's segments (consecutive ids) to the tile's list.
-        // Segments past the buffer (an overflowing draw, re-run larger) are
-        // never written.
-        const uint32_t first = sSegBase + segEx, last = first + need - 1;
-        const uint32_t prev = tOldLast[threadIdx.x];
-        if (prev == kNoSeg) tFirst[threadIdx.x] = first;
-        else if (prev < segCap) P.tileSegNext[prev] = first;
-        for (uint32_t j = first; j < last; j++)
-          if (j < segCap) P.tileSegNext[j] = j + 1;
-        if (last < segCap) P.tileSegNext[last] = kNoSeg;
-        tNew[threadIdx.x] = first;
-        tLast[threadIdx.x] = last;
+      if (threadIdx.x < (unsigned)tpb) {
+        const uint32_t first = sSegBase + segEx;
+        tRound[threadIdx.x] = make_uint4(kOld, oldLast, first, 0u);
+        if (need > 0) {
+          // Chain the round's segments (consecutive ids) to the tile's
+          // list.  Segments past the buffer (an overflowing draw, re-run
+          // larger) are never written.
+          const uint32_t last = first + need - 1;
+          if (oldLast == kNoSeg) tFirst[threadIdx.x] = first;
+          else if (oldLast < segCap) P.tileSegNext[oldLast] = first;
+          for (uint32_t j = first; j < last; j++)
+            if (j < segCap) P.tileSegNext[j] = j + 1;
+          if (last < segCap) P.tileSegNext[last] = kNoSeg;
+          tLast[threadIdx.x] = last;
+        }
       }
       __syncthreads();
       // Write the entries: list position e of tile ti is entry e & 31 of
       // the tile's (e >> 5)-th segment -- the old last segment or one of the
-      // round's.  Large boxes are walked by the whole wave, one entry at a
-      // time.
-      auto put = [&](int ti, uint32_t e, uint32_t sl) {
-        const uint32_t k = e >> kTileSegLog2, kOld = (tOld[ti] + kTileSeg - 1) >> kTileSegLog2;
-        const uint32_t sg = k < kOld ? tOldLast[ti] : tNew[ti] + (k - kOld);
+      // round's.  A tile's mask, rank base and segment map are read
+      // together, before the test that uses them, so that each tile costs
+      // one LDS round trip.  Large boxes are walked by the whole wave, one
+      // entry at a time.
+      auto put = [&](const uint4& q, uint32_t e, uint32_t sl) {
+        const uint32_t k = e >> kTileSegLog2;
+        const uint32_t sg = k < q.x ? q.y : q.z + (k - q.x);
         if (sg < segCap) P.tileList[((size_t)sg << kTileSegLog2) + (e & (kTileSeg - 1))] = sl;
       };
       const bool hasBox = valid && r.x0 <= r.x1 && r.y0 <= r.y1;
-      const bool bigBox = hasBox && (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1) > kCoopArea;
+      const int boxArea = hasBox ? (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1) : 0;
+      const bool bigBox = boxArea > kCoopArea;
       if (hasBox && !bigBox) {
-        const uint64_t bit = 1ull << lane, lt = lanesBelow();
-        for (int y = r.y0; y <= r.y1; y++)
-          for (int x = r.x0; x <= r.x1; x++) {
-            const int ti = x + (y << bl);
-            const uint64_t m = emitMask[wave][ti];
-            if (m & bit) put(ti, emitPre[wave][ti] + (uint32_t)__popcll(m & lt), slot);
-          }
+        const uint64_t bit = 1ull << lane;
+        // One loop over the box (row by row): the wave runs as many steps
+        // as its largest box has tiles.
+        int x = r.x0, y = r.y0;
+        for (int i = 0; i < boxArea; i++) {
+          const int ti = x + (y << bl);
+          const uint64_t m = emitMask[wave][ti];
+          const uint32_t pre = emitPre[wave][ti];
+          const uint4 q = tRound[ti];
+          if (m & bit) put(q, pre + rankBelow(m), slot);
+          if (++x > r.x1) { x = r.x0; y++; }
+        }
       }
       for (uint64_t b = __ballot(bigBox); b != 0; b &= b - 1) {
         const int j = (int)__builtin_ctzll(b);
@@ -436,7 +451,9 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
           if (x <= qx1 && y <= qy1) {
             const int ti = x + (y << bl);
             const uint64_t m = emitMask[wave][ti];
-            if (m & bj) put(ti, emitPre[wave][ti] + (uint32_t)__popcll(m & ltj), sj);
+            const uint32_t pre = emitPre[wave][ti];
+            const uint4 q = tRound[ti];
+            if (m & bj) put(q, pre + (uint32_t)__popcll(m & ltj), sj);
           }
         }
       }
