@@ -197,11 +197,42 @@ __device__ __forceinline__ void forEachHitCoop(const TileTest& q, F&& f) {
 // by the whole wave.
 enum CoarseCase : int { kCaseA = 0, kCaseB = 1, kCaseC = 2 };
 
+// A large box queued for the workgroup's waves to share: its tile test,
+// bin entry slot and (wave << 6 | lane) of the entry it came from.
+struct BigEntry {
+  uint4 a, b, c, d;
+};
+
+__device__ __forceinline__ void putBig(BigEntry* e, const TileTest& r, uint32_t slot, uint32_t id) {
+  e->a = make_uint4(r.x0, r.y0, r.x1, r.y1);
+  e->b = make_uint4(r.G[0], r.G[1], r.G[2], r.sx[0]);
+  e->c = make_uint4(r.sx[1], r.sx[2], r.sy[0], r.sy[1]);
+  e->d = make_uint4(r.sy[2], slot, id, 0u);
+}
+
+__device__ __forceinline__ TileTest getBig(const BigEntry* e, uint32_t& slot, uint32_t& id) {
+  const uint4 a = e->a, b = e->b, c = e->c, d = e->d;
+  TileTest q;
+  q.x0 = (int32_t)a.x; q.y0 = (int32_t)a.y; q.x1 = (int32_t)a.z; q.y1 = (int32_t)a.w;
+  q.G[0] = (int32_t)b.x; q.G[1] = (int32_t)b.y; q.G[2] = (int32_t)b.z;
+  q.sx[0] = (int32_t)b.w; q.sx[1] = (int32_t)c.x; q.sx[2] = (int32_t)c.y;
+  q.sy[0] = (int32_t)c.z; q.sy[1] = (int32_t)c.w; q.sy[2] = (int32_t)d.x;
+  slot = d.y;
+  id = d.z;
+  return q;
+}
+
 // One round of up to one bin entry per lane: set bit `lane` of
-// emitMask[wave][tile] for every tile the entry's edges reach.  Returns the
+// emitMask[wave][tile] for every tile the entry's edges reach.  Large boxes
+// of a scattered wave go to the workgroup's queue `big` (while it has room;
+// *queued tells the lane) and are walked after the next barrier by
+// whichever waves are free, spreading them over the workgroup.  Returns the
 // CoarseCase the wave took.
 __device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const int bl,
-                                            uint64_t (*emitMask)[kBinTiles]) {
+                                            uint64_t (*emitMask)[kBinTiles], BigEntry* big,
+                                            uint32_t* bigCount, uint32_t bigCap, uint32_t slot,
+                                            bool& queued) {
+  queued = false;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int w = valid ? r.x1 - r.x0 + 1 : 0, h = valid ? r.y1 - r.y0 + 1 : 0;
   const bool has = valid && w > 0 && h > 0;
@@ -238,9 +269,21 @@ __device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const
       }
     return kCaseB;
   }
-  // Scattered boxes: small ones by their own lane, large ones by the wave.
-  const bool big = has && w * h > kCoopArea;
-  if (has && !big) {
+  // Scattered boxes: small ones by their own lane, large ones queued (or,
+  // past the queue's room, by the wave).
+  const bool isBig = has && w * h > kCoopArea;
+  const uint64_t bigM = __ballot(isBig);
+  if (bigM != 0) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(bigCount, (uint32_t)__popcll(bigM));
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+    const uint32_t idx = base + rankBelow(bigM);
+    if (isBig && idx < bigCap) {
+      putBig(&big[idx], r, slot, ((uint32_t)wave << 6) | (uint32_t)lane);
+      queued = true;
+    }
+  }
+  if (has && !isBig) {
     // One loop over the box (row by row): as many wave steps as the
     // largest small box has tiles.
     int x = r.x0, y = r.y0;
@@ -249,7 +292,7 @@ __device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const
       if (++x > r.x1) { x = r.x0; y++; }
     }
   }
-  for (uint64_t b = __ballot(big); b != 0; b &= b - 1) {
+  for (uint64_t b = __ballot(isBig && !queued); b != 0; b &= b - 1) {
     const int j = (int)__builtin_ctzll(b);
     const TileTest q = readTileTest(r, j);
     const uint64_t bj = 1ull << j;
@@ -285,6 +328,11 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   __shared__ int32_t sCarry;
   __shared__ uint32_t sSegBase;
   __shared__ uint32_t sWritten;
+  // Large boxes of the round, shared out over the waves (a quarter of the
+  // round's entries fit; more are walked by their own wave).
+  constexpr uint32_t kBigCap = CT / 4;
+  __shared__ BigEntry bigQ[kBigCap];
+  __shared__ uint32_t sBigCount;
   // Profiling draws: entries, rounds, emits, tiles with emits, entries per case.
   __shared__ uint32_t sProf[8];
 
@@ -329,6 +377,7 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
       segOf[threadIdx.x] = -1;
       for (int i = threadIdx.x; i < kCoarseWaves * kBinTiles; i += kCoarseThreads)
         emitMask[i / kBinTiles][i % kBinTiles] = 0ull;
+      if (threadIdx.x == 0) sBigCount = 0;
       __syncthreads();
       const int carry = (r0 == 0) ? -1 : sCarry;
       if (n0 > 0 && s0 >= r0 && s0 - r0 < (uint32_t)kCoarseThreads) segOf[s0 - r0] = i0;
@@ -349,7 +398,8 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
         valid = slot < (uint32_t)P.maxSubtris;
         if (valid) r = tileTestOf(P, make_uint4(e.y, e.z, e.w, 0u), bx, by);
       }
-      const int wcase = coarseRecord(r, valid, bl, emitMask);
+      bool queued;
+      const int wcase = coarseRecord(r, valid, bl, emitMask, bigQ, &sBigCount, kBigCap, slot, queued);
       if (prof) {
         const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
         if (lane == 0 && nv) {
@@ -359,6 +409,21 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
         if (threadIdx.x == 0) sProf[1]++;
       }
       __syncthreads();
+      // The queued large boxes, one per wave at a time: a bit per tile hit
+      // in the mask of the wave the entry came from.
+      const uint32_t nBig = min(sBigCount, kBigCap);
+      if (nBig > 0) {
+        for (uint32_t i = (uint32_t)wave; i < nBig; i += kCoarseWaves) {
+          uint32_t sj, id;
+          const TileTest q = getBig(&bigQ[i], sj, id);
+          const uint64_t bj = 1ull << (id & 63);
+          uint64_t* mrow = emitMask[id >> 6];
+          forEachHitCoop(q, [&](int x, int y) {
+            atomicOr((unsigned long long*)&mrow[x + (y << bl)], (unsigned long long)bj);
+          });
+        }
+        __syncthreads();
+      }
       // Rank: each wave's entries of tile t follow the earlier waves' and
       // the tile's list so far; the segments the round needs past the
       // tile's last, partly filled one.
@@ -439,7 +504,7 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
           if (++x > r.x1) { x = r.x0; y++; }
         }
       }
-      for (uint64_t b = __ballot(bigBox); b != 0; b &= b - 1) {
+      for (uint64_t b = __ballot(bigBox && !queued); b != 0; b &= b - 1) {
         const int j = (int)__builtin_ctzll(b);
         const uint64_t bj = 1ull << j, ltj = bj - 1;
         const uint32_t sj = (uint32_t)__builtin_amdgcn_readlane((int)slot, j);
@@ -452,6 +517,24 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
             const int ti = x + (y << bl);
             const uint64_t m = emitMask[wave][ti];
             const uint32_t pre = emitPre[wave][ti];
+            const uint4 q = tRound[ti];
+            if (m & bj) put(q, pre + (uint32_t)__popcll(m & ltj), sj);
+          }
+        }
+      }
+      for (uint32_t i = (uint32_t)wave; i < nBig; i += kCoarseWaves) {
+        uint32_t sj, id;
+        const TileTest qt = getBig(&bigQ[i], sj, id);
+        const uint64_t bj = 1ull << (id & 63), ltj = bj - 1;
+        const uint64_t* mrow = emitMask[id >> 6];
+        const uint32_t* prow = emitPre[id >> 6];
+        const int x = qt.x0 + (lane & 15);
+        for (int y0 = qt.y0; y0 <= qt.y1; y0 += 4) {
+          const int y = y0 + (lane >> 4);
+          if (x <= qt.x1 && y <= qt.y1) {
+            const int ti = x + (y << bl);
+            const uint64_t m = mrow[ti];
+            const uint32_t pre = prow[ti];
             const uint4 q = tRound[ti];
             if (m & bj) put(q, pre + (uint32_t)__popcll(m & ltj), sj);
           }
