@@ -61,8 +61,21 @@ __device__ __forceinline__ uint32_t selectFlips(int32_t dx, int32_t dy) {
   return f;
 }
 
+// The draw state the clip path reads, passed to it by value: a reference
+// to the kernel's Params would make every wave copy the whole argument
+// block to scratch memory.
+struct ClipCtx {
+  int32_t viewportWidth, viewportHeight, samplesLog2, flags, shader, numTris, maxSubtris;
+  uint8_t* triSubtris;
+  Header* triHeader;
+  TriData* triData;
+  uint32_t* subtriTri;
+  Counters* counters;
+};
+
 // snapTriangle (TriangleSetup.inl:19-31)
-__device__ __forceinline__ void snap(const Params& P, const float4& v0, const float4& v1, const float4& v2,
+template <class PP>
+__device__ __forceinline__ void snap(const PP& P, const float4& v0, const float4& v1, const float4& v2,
                                      I2& p0, I2& p1, I2& p2, float rcpW[3], I2& lo, I2& hi) {
   float sx = (float)(P.viewportWidth << (kSubpixelLog2 - 1));
   float sy = (float)(P.viewportHeight << (kSubpixelLog2 - 1));
@@ -78,7 +91,8 @@ __device__ __forceinline__ void snap(const Params& P, const float4& v0, const fl
 
 // prepareTriangle (TriangleSetup.inl:38-100): 0 visible, 1 backfacing,
 // 2 between samples.
-__device__ __forceinline__ int prepare(const Params& P, I2 p0, I2 p1, I2 p2, I2 lo, I2 hi,
+template <class PP>
+__device__ __forceinline__ int prepare(const PP& P, I2 p0, I2 p1, I2 p2, I2 lo, I2 hi,
                                        I2& d1, I2& d2, int32_t& area) {
   int s = P.samplesLog2;
   d1.x = wsub(p1.x, p0.x); d1.y = wsub(p1.y, p0.y);
@@ -118,7 +132,8 @@ __device__ __forceinline__ int prepare(const Params& P, I2 p0, I2 p1, I2 p2, I2 
 }
 
 // setupTriangle (TriangleSetup.inl:104-215)
-__device__ __forceinline__ void setupTriangle(const Params& P, Header* th, TriData* td, int3 vidx,
+template <class PP>
+__device__ __forceinline__ void setupTriangle(const PP& P, Header* th, TriData* td, int3 vidx,
                                               const float4& v0, const float4& v1, const float4& v2,
                                               float2 b0, float2 b1, float2 b2, I2 p0, I2 p1, I2 p2,
                                               const float rcpW[3], I2 d1, I2 d2, int32_t area) {
@@ -233,7 +248,7 @@ __device__ __forceinline__ float4 lerpClip(const float4& o, const float4& d1, co
 // Slow path, kept out of line: frustum clip and fan (TriangleSetup.inl:326-412).
 // Returns the number of subtriangles written (0 when the slot allocation
 // overflowed) and their first slot in *baseOut.
-__device__ __noinline__ int setupClipped(const Params P, int t, int3 vidx, float4 v0, float4 v1, float4 v2,
+__device__ __noinline__ int setupClipped(const ClipCtx P, int t, int3 vidx, float4 v0, float4 v1, float4 v2,
                                          int* baseOut) {
   float4 od1 = make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, v1.w - v0.w);
   float4 od2 = make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, v2.w - v0.w);
@@ -350,7 +365,9 @@ __device__ __forceinline__ int setupOne(const Params& P, int t, int* base, uint4
     }
   }
   *why = kSetupClipPath;
-  return setupClipped(P, t, vidx, v0, v1, v2, base);
+  const ClipCtx C = {P.viewportWidth, P.viewportHeight, P.samplesLog2, (int32_t)P.flags, P.shader,
+                     P.numTris, P.maxSubtris, P.triSubtris, P.triHeader, P.triData, P.subtriTri, P.counters};
+  return setupClipped(C, t, vidx, v0, v1, v2, base);
 }
 
 // ---------------------------------------------------------------------------
