@@ -5,7 +5,8 @@
 A step is one CudaRaster::drawTriangles() of the workload -- deferred clear,
 setup, bin, coarse, fine -- with the reference's synchronous semantics (the
 call returns when the frame is complete and the overflow counters have been
-read back).  Inputs are resident in HBM before timing starts.  Prints one
+read back).  Inputs are resident in HBM before timing starts; the draws
+are timed by a native loop over the C ABI.  Prints one
 JSON line: Mtris/s as `value`, Gfragments/s beside it, the roofline of the
 dominant kernel (timed with HIP events on the draw stream) and the CPU
 oracle's throughput on the same workload as `cpu_baseline`.
@@ -198,10 +199,7 @@ def multi_view_ratio(reps=5, triid=True):
     # The same sequential draws from a native loop over the C ABI (no
     # interpreter between the calls): the plain leg of the ratio.
     import ctypes
-    bench_lib = ctypes.CDLL(os.path.join(REPO, "cudaraster-linux_amd", "lib", "libcrbench.so"))
-    bench_lib.crbench_plain_draws.restype = ctypes.c_double
-    bench_lib.crbench_plain_draws.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
-                                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    bench_lib = _bench_lib()
     cviews = (cudaraster.View * len(views))()
     for k, sc in enumerate(views):
         col, dep, tid = plain_surf[k]
@@ -383,6 +381,41 @@ def cpu_baseline(scene, runs=5, single_runs=3, max_threads=16):
     return out
 
 
+def _bench_lib():
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(REPO, "cudaraster-linux_amd", "lib", "libcrbench.so"))
+    lib.crbench_plain_draws.restype = ctypes.c_double
+    lib.crbench_plain_draws.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    return lib
+
+
+def native_draw_seconds(r, steps):
+    """Seconds per synchronous drawTriangles() of a SceneRenderer's frame,
+    timed by a native loop over the C ABI (bench_native/plain_draws.cpp:
+    the calls a C++ caller makes per frame -- bind surfaces, pipe and
+    buffers, deferredClear, drawTriangles -- with no interpreter between
+    them)."""
+    import ctypes
+    import cudaraster
+    sc = r.scene
+    views = (cudaraster.View * 1)()
+    v = views[0]
+    v.color, v.depth = r.color.handle.value, r.depth.handle.value
+    v.tri_id = r.triid.handle.value if r.triid is not None else None
+    v.vertices, v.num_vertices = r.vbuf.data_ptr(), sc.vertices.shape[0]
+    v.indices, v.num_tris = r.ibuf.data_ptr(), sc.num_tris
+    v.deferred_clear = 1 if sc.deferred_clear else 0
+    v.pipe = cudaraster.scene_pipe(sc)
+    v.clear_color[:] = [float(x) for x in sc.clear_color]
+    v.clear_depth = float(sc.clear_depth)
+    st, bad = ctypes.c_int32(), ctypes.c_int32()
+    t = _bench_lib().crbench_plain_draws(r.cr.ctx, views, 1, steps, ctypes.byref(st), ctypes.byref(bad))
+    if t < 0:
+        raise RuntimeError(f"native draw loop failed: status {st.value}")
+    return t
+
+
 def plain_workload_leg(name, steps=10, warmup=2, triid=True, cpu=True):
     """One plain-draw workload beside the headline one (the skewed 4M +
     1024-triangle 8x src-over frame): Mtris/s and Gfragments/s over `steps`
@@ -408,8 +441,9 @@ def plain_workload_leg(name, steps=10, warmup=2, triid=True, cpu=True):
         for k in stage:
             stage[k] += st[k]
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    elapsed_py = time.perf_counter() - t0
     counters = r.cr.getCounters()
+    elapsed = native_draw_seconds(r, steps) * steps
     r.close()
     avg = {k: v / steps for k, v in stage.items()}
     kernels = {}
@@ -421,6 +455,8 @@ def plain_workload_leg(name, steps=10, warmup=2, triid=True, cpu=True):
         kernels[kname]["frac"] = kernels[kname]["GBps"] / HBM_PEAK_GBS if kernels[kname]["GBps"] else None
     ms = elapsed / steps * 1e3
     out = {"workload": desc, "triangles": scene.num_tris, "steps": steps, "ms_per_step": ms,
+           "timing": "native C loop over the C ABI; stage_ms from the library's stage events",
+           "ms_per_step_python": elapsed_py / steps * 1e3,
            "mtris_per_s": scene.num_tris * steps / elapsed / 1e6,
            "gfragments_per_s": fragments * steps / elapsed / 1e9, "fragments_per_frame": fragments,
            "stage_ms": {k: v * 1e3 for k, v in avg.items()}, "kernels": kernels,
@@ -504,8 +540,12 @@ def main():
         for k in stage:
             stage[k] += st[k]
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    elapsed_py = time.perf_counter() - t0
     counters = r.cr.getCounters()
+    # `value`: the same draws timed by a native loop over the C ABI (the
+    # Python loop above, which also reads the stage times, is reported as
+    # ms_per_step_python).
+    elapsed = native_draw_seconds(r, args.steps) * args.steps
     ms = elapsed / args.steps * 1e3
 
     avg = {k: v / args.steps for k, v in stage.items()}
@@ -534,6 +574,9 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms,
+        "timing": "native C loop over the C ABI (cudaraster-linux_amd/bench_native/plain_draws.cpp); "
+                  "stage_ms from the library's stage events",
+        "ms_per_step_python": elapsed_py / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
