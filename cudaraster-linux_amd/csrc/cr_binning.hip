@@ -78,36 +78,6 @@ __global__ void __launch_bounds__(64) resetCountersKernel(Counters* c, uint32_t 
   if (threadIdx.x == 0) c->numSubtris = numTris;
 }
 
-// Inclusive prefix maximum over the NT threads of the block (in thread order).
-template <int NT>
-__device__ __forceinline__ int blockMaxScanC(int v, uint32_t* sums) {
-  constexpr int NW = NT / 64;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o, 64);
-    if (lane >= o) x = max(x, y);
-  }
-  if (lane == 63) sums[wave] = (uint32_t)x;
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    int sc = (threadIdx.x < NW) ? (int)sums[threadIdx.x] : -1;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(sc, o, 64);
-      if ((int)threadIdx.x >= o) sc = max(sc, y);
-    }
-    int ex = __shfl_up(sc, 1, 64);
-    if (threadIdx.x == 0) ex = -1;
-    if (threadIdx.x < NW) sums[threadIdx.x] = (uint32_t)ex;
-  }
-  __syncthreads();
-  const int r = max(x, (int)sums[wave]);
-  __syncthreads();
-  return r;
-}
-
 // Per-lane tile test data of one triangle inside bin (bx, by).
 struct TileTest {
   int32_t x0, y0, x1, y1;   // bin-local tile box (inclusive); empty when x0 > x1
@@ -324,8 +294,6 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   // round's first new one.
   __shared__ uint4 tRound[kBinTiles];
   __shared__ uint32_t sums[kCoarseWaves + 1];
-  __shared__ int32_t segOf[kCoarseThreads];
-  __shared__ int32_t sCarry;
   __shared__ uint32_t sSegBase;
   __shared__ uint32_t sWritten;
   // Large boxes of the round, shared out over the waves (a quarter of the
@@ -369,37 +337,44 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
     if (threadIdx.x == 0) segStart[nb] = segTotal;
     __syncthreads();
 
+    // Entry g of the batch lies in the last segment that starts at or before
+    // it (an empty segment starts where its successor does, so that one is
+    // never the last): a binary search of the segment table, which stays in
+    // LDS for the whole batch.  The entry carries the slot and the snapped
+    // vertices: one contiguous 16-byte read, no dependent header fetch.
+    auto entryAt = [&](uint32_t g) -> uint4 {
+      int lo = 0, hi = nb - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segStart[mid] <= g) lo = mid;
+        else hi = mid - 1;
+      }
+      return P.binList[segAddr[lo] + (g - segStart[lo])];
+    };
+    // Each round's entry is loaded one round ahead, so that its latency
+    // overlaps the previous round's ranking and writes.
+    uint4 eNext = make_uint4(0u, 0u, 0u, 0u);
+    if (threadIdx.x < segTotal) eNext = entryAt(threadIdx.x);
+
     for (uint32_t r0 = 0; r0 < segTotal; r0 += kCoarseThreads) {
-      // Segment of entry r0 + t: every non-empty segment starting in this
-      // round marks its first entry; an inclusive max-scan then gives each
-      // entry the last segment started at or before it (the carry: the
-      // segment of entry r0 - 1).
-      segOf[threadIdx.x] = -1;
       for (int i = threadIdx.x; i < kCoarseWaves * kBinTiles; i += kCoarseThreads)
         emitMask[i / kBinTiles][i % kBinTiles] = 0ull;
       if (threadIdx.x == 0) sBigCount = 0;
       __syncthreads();
-      const int carry = (r0 == 0) ? -1 : sCarry;
-      if (n0 > 0 && s0 >= r0 && s0 - r0 < (uint32_t)kCoarseThreads) segOf[s0 - r0] = i0;
-      if (n1 > 0 && s0 + n0 >= r0 && s0 + n0 - r0 < (uint32_t)kCoarseThreads) segOf[s0 + n0 - r0] = i1;
-      __syncthreads();
-      const int seg = max(blockMaxScanC<kCoarseThreads>(segOf[threadIdx.x], sums), carry);
-      if (threadIdx.x == kCoarseThreads - 1) sCarry = seg;
       const uint32_t g = r0 + threadIdx.x;
       bool valid = g < segTotal;
+      const uint4 e = eNext;
       uint32_t slot = 0;
       TileTest r;
       r.x0 = 1; r.x1 = 0; r.y0 = 1; r.y1 = 0;
       if (valid) {
-        // The entry carries the slot and the snapped vertices: one
-        // contiguous 16-byte read, no dependent header fetch.
-        const uint4 e = P.binList[segAddr[seg] + (g - segStart[seg])];
         slot = e.x;
         valid = slot < (uint32_t)P.maxSubtris;
         if (valid) r = tileTestOf(P, make_uint4(e.y, e.z, e.w, 0u), bx, by);
       }
       bool queued;
       const int wcase = coarseRecord(r, valid, bl, emitMask, bigQ, &sBigCount, kBigCap, slot, queued);
+      if (g + kCoarseThreads < segTotal) eNext = entryAt(g + kCoarseThreads);
       if (prof) {
         const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
         if (lane == 0 && nv) {
