@@ -21,8 +21,10 @@
 //
 // The bin's entries are the concatenation, in chunk order, of the segments
 // the fused setup/bin stage wrote per chunk of triangles; a segment table is
-// scanned into LDS and every thread finds its entry by a max-scan over the
-// segment starts.  Binning never changes the image; the fine stage is exact.
+// scanned into LDS and every thread finds its entry by a binary search of
+// the segment starts, one round ahead.  Large boxes are shared out over the
+// workgroup's waves through an LDS queue.  Binning never changes the image;
+// the fine stage is exact.
 #include "cr_device.h"
 
 namespace crhip {
