@@ -422,10 +422,21 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
           atomicAdd(&sProf[3], 1u);
         }
       }
-      // Extend: one reservation for the whole round (its own barriers order
-      // the stores above).
-      uint32_t segsRound;
-      const uint32_t segEx = blockScanC<kCoarseThreads>(need, sums, segsRound);
+      // Extend: one reservation for the whole round.  The tiles (at most
+      // 256: the first four waves) scan their segment needs -- a wave scan,
+      // then the four wave totals -- so the reservation costs two barriers.
+      static_assert(kBinTiles <= 256 && kCoarseWaves >= 4, "tiles in the first four waves");
+      uint32_t x = need;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+      }
+      if (wave < 4 && lane == 63) sums[wave] = x;
+      __syncthreads();
+      const uint32_t w0 = sums[0], w1 = sums[1], w2 = sums[2], w3 = sums[3];
+      const uint32_t segsRound = w0 + w1 + w2 + w3;
+      const uint32_t segEx = x - need + (wave >= 1 ? w0 : 0u) + (wave >= 2 ? w1 : 0u) + (wave >= 3 ? w2 : 0u);
       if (threadIdx.x == 0) {
         uint32_t base = 0;
         if (segsRound) {
