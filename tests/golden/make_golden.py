@@ -10,7 +10,12 @@ cube_256x192_4x_resolved.png: the demo cube (seed 0, demo camera) at
 (float32 Mat4f * Vec4f), drawn by the oracle and box-resolved.
 random_96x64_{1,2,4,8}x_resolved.png: 1500 seeded random triangles (areas
 0.5-64 px^2, depth test, Gouraud with perspective-correct interpolation) at
-each sample count, drawn by the oracle and box-resolved."""
+each sample count, drawn by the oracle and box-resolved.
+cube_1024x768_1x_surfaces.npz: the C0 frame -- the demo cube at 1024x768,
+1x, clip positions from the passthrough vertex shader -- as drawn by the
+oracle: the colour, depth and triangle-id surfaces (uint32 texels in the
+reference's CudaSurface layout, row 0 the bottom row; numpy's compressed
+archive, arrays `color`, `depth`, `triid`)."""
 import os
 import sys
 
@@ -64,9 +69,27 @@ def random_golden(name):
     return oracle.resolve(r["color"], scene.width, scene.height, scene.samples_log2)
 
 
+CUBE_FRAME = "cube_1024x768_1x_surfaces.npz"
+
+
+def cube_frame():
+    """The C0 frame's scene and the oracle's surfaces for it."""
+    import oracle
+    scene, _ = cube_scene(1024, 768, 0)
+    r = oracle.render(scene)
+    return scene, {k: r[k] for k in ("color", "depth", "triid")}
+
+
+def load_cube_frame():
+    with np.load(os.path.join(HERE, CUBE_FRAME)) as z:
+        return {k: z[k] for k in ("color", "depth", "triid")}
+
+
 def main():
     import cudaraster
     cudaraster.save_png(os.path.join(HERE, "cube_256x192_4x_resolved.png"), cube_golden())
+    _, surf = cube_frame()
+    np.savez_compressed(os.path.join(HERE, CUBE_FRAME), **surf)
     for name in RANDOM_GOLDENS:
         cudaraster.save_png(os.path.join(HERE, name), random_golden(name))
 

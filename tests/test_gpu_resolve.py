@@ -43,6 +43,31 @@ def test_resolve_rejects_non_colour_surfaces():
         surf.close()
 
 
+def test_cube_c0_frame_matches_stored_surfaces():
+    """The C0 frame (the demo cube at 1024x768, 1x, clip positions from the
+    GPU vertex shader) drawn on the GPU equals the stored colour, depth and
+    triangle-id surfaces."""
+    torch = _torch()
+    import sys
+    import cudaraster
+    from cudaraster import scenes
+    sys.path.insert(0, GOLDEN)
+    import make_golden
+    scene, m = make_golden.cube_scene(1024, 768, 0)
+    gold = make_golden.load_cube_frame()
+    base = scenes.cube(scene.width, scene.height, samples_log2=0)
+    r = cudaraster.SceneRenderer(base)
+    try:
+        dp = torch.from_numpy(scenes.CUBE_POSITIONS.astype(np.float32)).cuda()
+        r.cr.vertexShaderPassthrough(dp, m, r.vbuf, out_stride=32)
+        r.draw()
+        got = r.read()
+    finally:
+        r.close()
+    for k in ("color", "depth", "triid"):
+        assert np.array_equal(got[k], gold[k]), k
+
+
 def test_cube_frame_matches_golden():
     torch = _torch()
     import sys

@@ -85,3 +85,19 @@ def test_oracle_reproduces_random_goldens(name):
     sys.path.insert(0, GOLDEN)
     import make_golden
     assert np.array_equal(make_golden.random_golden(name), cudaraster.load_png(os.path.join(GOLDEN, name)))
+
+
+def test_cube_frame_surfaces_golden(oracle_lib):
+    """The C0 frame (the demo cube at 1024x768, 1x) as stored: the oracle
+    draws exactly the stored colour, depth and triangle-id surfaces; from
+    the demo camera two faces (four of the cube's 12 triangles) are
+    visible."""
+    import sys
+    sys.path.insert(0, GOLDEN)
+    import make_golden
+    _, surf = make_golden.cube_frame()
+    gold = make_golden.load_cube_frame()
+    for k in ("color", "depth", "triid"):
+        assert gold[k].shape == (768, 1024) and np.array_equal(surf[k], gold[k]), k
+    ids = set(np.unique(gold["triid"]).tolist())
+    assert ids <= set(range(12)) | {0xFFFFFFFF} and len(ids - {0xFFFFFFFF}) == 4
