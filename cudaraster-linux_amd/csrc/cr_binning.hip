@@ -285,9 +285,7 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   __shared__ uint32_t segAddr[kSegBatch];
   __shared__ uint64_t emitMask[kCoarseWaves][kBinTiles];
   __shared__ uint32_t emitPre[kCoarseWaves][kBinTiles];
-  // Per tile of the bin: list length so far, first and last segment, and
-  // for the round being written its length before the round, its last
-  // segment before the round and the first segment the round added.
+  // Per tile of the bin: list length so far, first and last segment.
   __shared__ uint32_t tTotal[kBinTiles];
   __shared__ uint32_t tFirst[kBinTiles];
   __shared__ uint32_t tLast[kBinTiles];
