@@ -240,25 +240,44 @@ __device__ __forceinline__ uint32_t toABGR(float r, float g, float b, float a) {
 }
 
 // blendABGR / blendABGRClamp with the stock factors (Util.inl:42-60,
-// PixelPipe.inl:73-85).
+// PixelPipe.inl:73-85): per channel x = s*a + d*(255-a) (src-over) or
+// x = min(255*(s+d), 255*255) (additive), then (x * 0x010101 + 0x800000) >> 24.
+// Both x fit 16 bits, so two channels share one packed 16-bit operation:
+//  - round(x / 255) = (t + (t >> 8)) >> 8 with t = x + 128 (no intermediate
+//    above 65407) equals (x * 0x010101 + 0x800000) >> 24 for every x in
+//    [0, 255*255] (checked exhaustively);
+//  - for additive x = 255*k with k = min(s+d, 255) and that rounding gives k
+//    back, so the channel is the saturating sum min(s+d, 255).
+typedef unsigned short crU16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ crU16x2 bytes02(uint32_t v) {   // bytes 0 and 2, one per half
+  return __builtin_bit_cast(crU16x2, v & 0x00FF00FFu);
+}
+__device__ __forceinline__ crU16x2 bytes13(uint32_t v) {   // bytes 1 and 3 (selector 0x0C: zero)
+  return __builtin_bit_cast(crU16x2, __builtin_amdgcn_perm(0u, v, 0x0C030C01u));
+}
+__device__ __forceinline__ uint32_t packBytes(crU16x2 lo, crU16x2 hi) {
+  return __builtin_bit_cast(uint32_t, lo) | (__builtin_bit_cast(uint32_t, hi) << 8);
+}
 __device__ __forceinline__ uint32_t blendPixel(int32_t mode, uint32_t src, uint32_t dst) {
   if (mode == kBlendReplace) return src;
   if (mode == kBlendDepthOnly) return dst;
-  uint32_t out = 0;
-  uint32_t a = src >> 24;
-#pragma unroll
-  for (int ch = 0; ch < 4; ch++) {
-    uint32_t s = (src >> (8 * ch)) & 0xFFu;
-    uint32_t d = (dst >> (8 * ch)) & 0xFFu;
-    uint32_t x;
-    if (mode == kBlendSrcOver) {
-      x = s * a + d * (255u - a);
-    } else {
-      x = min(s * 255u + d * 255u, 255u * 255u);
-    }
-    out |= ((x * 0x010101u + 0x800000u) >> 24) << (8 * ch);
+  const crU16x2 s02 = bytes02(src), s13 = bytes13(src), d02 = bytes02(dst), d13 = bytes13(dst);
+  if (mode == kBlendSrcOver) {
+    const unsigned short a = (unsigned short)(src >> 24), ia = (unsigned short)(255u - a);
+    const crU16x2 va = {a, a}, via = {ia, ia};
+    // The bias is kept opaque so that it folds into the first multiply-add
+    // instead of costing an add of its own.
+    uint32_t bias = 0x00800080u;
+    __asm__("" : "+s"(bias));
+    const crU16x2 half = __builtin_bit_cast(crU16x2, bias);
+    crU16x2 x02 = s02 * va + (d02 * via + half);
+    crU16x2 x13 = s13 * va + (d13 * via + half);
+    x02 = (x02 + (x02 >> (unsigned short)8)) >> (unsigned short)8;
+    x13 = (x13 + (x13 >> (unsigned short)8)) >> (unsigned short)8;
+    return packBytes(x02, x13);
   }
-  return out;
+  const crU16x2 cap = {(unsigned short)255, (unsigned short)255};
+  return packBytes(__builtin_elementwise_min(s02 + d02, cap), __builtin_elementwise_min(s13 + d13, cap));
 }
 
 // c_msaaPatterns (Util.hpp:27-36), packed 4 bits per sample.
