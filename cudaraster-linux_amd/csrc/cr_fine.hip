@@ -206,15 +206,24 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
   uint32_t* __restrict__ triid = P.triid;
   const int32_t blend = BL >= 0 ? BL : P.blend;
 
-  if (P.counters->overflow) return;
   const int lane = threadIdx.x & 63;
   // Wave-uniform, so the tile's coordinates and addresses are scalars.
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   uint32_t* rec = recAll + wave * (kChunk * RS);
   const int tile = group * W + wave;
   if (tile >= P.numTiles) return;
+  // The overflow flag, the tile's entry count and its first segment do not
+  // depend on each other: issued together, the tile's start waits for one
+  // load latency instead of a chain of three.
+  const uint32_t overflowed = P.counters->overflow;
+  const uint32_t cntLoaded = P.tileCount[tile];
+  const uint32_t startLoaded = P.tileStart[tile];
+  // (An empty asm that takes all three keeps the compiler from sinking the
+  // later loads below the branches that follow.)
+  __asm__ volatile("" ::"s"(overflowed), "s"(cntLoaded), "s"(startLoaded));
+  if (overflowed) return;
   // Wave-uniform (one tile per wave): held in scalar registers.
-  const uint32_t cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)P.tileCount[tile]);
+  const uint32_t cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)cntLoaded);
   if (cnt == 0 && !P.deferredClear) {
     if (COUNT && lane == 0) P.tileFrags[tile] = 0;
     return;
@@ -264,12 +273,13 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
   static_assert(kChunk == kTileSeg, "one list segment per chunk");
   // (The links stay in vector registers: a scalar copy would wait for each
   // load as soon as it is issued.)
-  uint32_t cur = 0, segNext = kNoSeg;
-  if (cnt > 0) {
-    const uint32_t seg0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)P.tileStart[tile]);
-    if (lane < (int)min((uint32_t)kChunk, cnt)) cur = tileList[((size_t)seg0 << kTileSegLog2) + lane];
-    if (cnt > (uint32_t)kChunk) segNext = P.tileSegNext[seg0];
-  }
+  // The first segment's entries and link are loaded without a branch:
+  // segment 0 (always allocated) stands in for an empty list, whose start
+  // word was not written by this draw; entries past the count and a link
+  // past the last segment are never used.
+  const uint32_t seg0 = cnt > 0 ? (uint32_t)__builtin_amdgcn_readfirstlane((int)startLoaded) : 0u;
+  uint32_t cur = tileList[((size_t)seg0 << kTileSegLog2) + lane];
+  uint32_t segNext = P.tileSegNext[seg0];
   const bool hiz = DEPTH && P.hiz;
 
   // Hierarchical Z: a triangle whose conservative zmin is not nearer than
