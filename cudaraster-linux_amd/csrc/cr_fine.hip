@@ -193,6 +193,7 @@ constexpr int recWords() { return kChunk * (VIS ? Rec<S>::SIZE_VIS : Rec<S>::SIZ
 template <int S, bool VIS, bool COUNT, int W, bool DEPTH, int BL, bool TEX = false>
 __device__ __forceinline__ void fineBody(const Params& P, const int group, uint32_t* __restrict__ recAll) {
   static_assert(!(TEX && VIS), "texPhong shades in order");
+  static_assert(!(COUNT && VIS), "counting draws shade in order (their statistics are per record)");
   constexpr int N = 1 << S;
   using L = Rec<S>;
   constexpr int RS = VIS ? L::SIZE_VIS : L::SIZE_FULL;
@@ -257,6 +258,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
   const int32_t lx16 = lx << kSubpixelLog2, ly16 = ly << kSubpixelLog2;
   const uint32_t lxN = (uint32_t)(lx * N), lyN = (uint32_t)(ly * N);
   const uint32_t pixN = lxN | (lyN << 16);
+  const uint32_t quadShift = (uint32_t)((((ly >> 2) << 1) | (lx >> 2)) * 8);   // byte of this lane's quadrant
   const uint32_t tileSx = (uint32_t)(tx * 8 * N), tileSy = (uint32_t)(ty * 8 * N);
   constexpr CentroidOrder<S> co;
 
@@ -357,6 +359,9 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
       const uint32_t nb = min((uint32_t)kChunk, total - b);
       uint32_t* r = rec + lane * RS;
       const bool su = lane < (int)nb;
+      // Deferred shading: the 4x4-pixel quadrants of the tile this record
+      // may cover (bit (ly >= 4) * 2 + (lx >= 4)).
+      uint32_t quads = 0;
       waveSync();                            // the queue stores (or the last walk's reads) are done
       if (su) {
         slot = r[b == 0 ? Q0 : Q1];
@@ -378,6 +383,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
         // sample's threshold, i.e. its largest value over the tile is at most
         // the smallest threshold.  Exactly the per-sample test's outcome.
         bool full = true;
+        if (VIS) quads = 15u;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           const int kn = (k + 1) % 3;
@@ -397,7 +403,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
             cc = cc > (1ll << 30) ? (1ll << 30) : (cc < -(1ll << 30) ? -(1ll << 30) : cc);
             ck = (int32_t)cc;
           }
-          int32_t minR = INT32_MAX;
+          int32_t minR = INT32_MAX, maxR = INT32_MIN;
 #pragma unroll
           for (int i = 0; i < N; i++) {
             const int32_t ox = (2 * msaaPattern(S, i) + 1) << (3 - S);
@@ -406,8 +412,20 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
             const int32_t rk = ck - (__mul24(ox, dy) - __mul24(oy, dx));
             r[L::R + k * N + i] = (uint32_t)rk;
             minR = min(minR, rk);
+            maxR = max(maxR, rk);
           }
           full = full && __mul24(112, max(dy, 0)) + __mul24(112, max(-dx, 0)) <= minR;
+          if (VIS) {
+            // A quadrant none of whose samples can pass this edge: the
+            // smallest L over its pixels (lx16 in 64*qx + {0..48}, ly16 in
+            // 64*qy + {0..48}) exceeds the largest threshold.  |d| < 2^17:
+            // every term below fits 25 bits.
+            const int32_t ndx = -dx;
+            const int32_t m00 = (dy < 0 ? __mul24(48, dy) : 0) + (ndx < 0 ? __mul24(48, ndx) : 0);
+            const int32_t ex = dy * 64, ey = ndx * 64;
+            quads &= (m00 <= maxR ? 1u : 0u) | (m00 + ex <= maxR ? 2u : 0u) | (m00 + ey <= maxR ? 4u : 0u) |
+                     (m00 + ex + ey <= maxR ? 8u : 0u);
+          }
           r[L::D + 2 * k] = (uint32_t)dy;
           r[L::D + 2 * k + 1] = (uint32_t)(-dx);
         }
@@ -448,12 +466,44 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
           q[15] = __float_as_uint(c1.z); q[16] = __float_as_uint(c1.w); q[17] = __float_as_uint(c2.x);
           q[18] = __float_as_uint(c2.y); q[19] = __float_as_uint(c2.z); q[20] = __float_as_uint(c2.w);
         }
+        // Quadrant queue slots start empty (this record's queue entry was
+        // read above).
+        if (VIS) r[Q0] = 0xFFFFFFFFu;
       }
       waveSync();
 
+      // Deferred shading: per 4x4 quadrant, the batch's records that may
+      // cover it, in submission order -- byte q of record i's word Q0 is the
+      // index of quadrant q's i-th record (0xFF past its end).  Step i of the
+      // walk takes the i-th record of every quadrant, so the quadrants walk
+      // their own records side by side; their pixels are disjoint, so every
+      // pixel still sees its triangles in submission order, and a step
+      // serves up to four small triangles.
+      uint32_t steps = nb;
+      if (VIS) {
+        steps = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const uint64_t m = __ballot((quads >> q) & 1u);
+          steps = max(steps, (uint32_t)__popcll(m));
+          if ((quads >> q) & 1u)
+            reinterpret_cast<uint8_t*>(rec + rankBelow(m) * RS + Q0)[q] = (uint8_t)lane;
+        }
+        waveSync();
+      }
+
       // ---- walk the batch's triangles in submission order --------------------
-      for (uint32_t j = 0; j < nb; j++) {
+      for (uint32_t j = 0; j < steps; j++) {
+        // In-order shading: record j for the whole wave (uniform address).
+        // Deferred shading: this lane's quadrant's j-th record; lanes whose
+        // quadrant has no j-th record read record 31 and take no part.
+        uint64_t validM = ~0ull;
         const uint32_t* r = rec + j * RS;
+        if (VIS) {
+          const uint32_t t = (r[Q0] >> quadShift) & 0xFFu;
+          validM = __ballot(t < (uint32_t)kChunk);
+          r = rec + min(t, (uint32_t)kChunk - 1) * RS;
+        }
         const uint32_t* cur = r;
         // |lx16|,|ly16| < 2^7 and |d| < 2^17: 24-bit multiplies are exact.
         // In-order shading: a record covering the whole tile skips the
@@ -477,7 +527,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
 #pragma unroll
           for (int i = 0; i < N; i++) {
             covM[i] = __ballot(Lk[0] <= (int32_t)cur[L::R + i]) & __ballot(Lk[1] <= (int32_t)cur[L::R + N + i]) &
-                      __ballot(Lk[2] <= (int32_t)cur[L::R + 2 * N + i]);
+                      __ballot(Lk[2] <= (int32_t)cur[L::R + 2 * N + i]) & validM;
             anyM |= covM[i];
           }
         }
@@ -578,6 +628,12 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
     }
   }
 
+  // The pixel's coordinates and surface offset, derived again from the lane
+  // index (v_mbcnt) instead of being held in registers through the walk.
+  const int laneE = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  const int pxE = (tx << kTileLog2) + (laneE & 7), pyE = (ty << kTileLog2) + (laneE >> 3);
+  const size_t texelE = (size_t)pyE * pitch + (size_t)tx * (8 * N) + (laneE & 7);
+
   // ---- deferred shading: once per (pixel, distinct winning triangle) -------
   uint32_t tid[N];
   if (VIS) {
@@ -585,7 +641,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
 #pragma unroll
     for (int i = 0; i < N; i++) {
       tid[i] = 0xFFFFFFFFu;
-      if (!P.deferredClear && idOn && aux[i] == 0xFFFFFFFFu) tid[i] = triid[texel + i * 8];
+      if (!P.deferredClear && idOn && aux[i] == 0xFFFFFFFFu) tid[i] = triid[texelE + i * 8];
     }
     // Each pass shades the winner of this lane's first unresolved sample and
     // resolves every sample it won (aux -> ~0); a lane leaves when none is
@@ -601,7 +657,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
       if (gouraud) {
         const uint4* td = tdata + (size_t)slot * 4;
         const uint4 t1 = td[1], t2 = td[2], t3 = td[3];
-        src = shadeGouraud<S>(lerpOn, (int)(v & 15u), px, py, t1, t2, t3, varying0(verts, t3.y, P.vertexStride),
+        src = shadeGouraud<S>(lerpOn, (int)(v & 15u), pxE, pyE, t1, t2, t3, varying0(verts, t3.y, P.vertexStride),
                               varying0(verts, t3.z, P.vertexStride), varying0(verts, t3.w, P.vertexStride));
       }
 #pragma unroll
@@ -620,9 +676,9 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
   const bool storeColor = blend != kBlendDepthOnly || P.deferredClear;
 #pragma unroll
   for (int i = 0; i < N; i++) {
-    if (storeColor) color[texel + i * 8] = col[i];
-    depth[texel + i * 8] = dep[i];
-    if (idOn) triid[texel + i * 8] = tid[i];
+    if (storeColor) color[texelE + i * 8] = col[i];
+    depth[texelE + i * 8] = dep[i];
+    if (idOn) triid[texelE + i * 8] = tid[i];
   }
   // Fragment statistics only on counting draws: one plain store per tile
   // (a device-wide atomic per tile serialises across the XCDs).
