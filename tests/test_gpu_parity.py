@@ -138,6 +138,59 @@ def test_pathological_vertices(oracle_lib):
     assert got["counters"]["fragments"] == ref["stats"]["fragments"]
 
 
+def quadrant_scene(samples_log2, flags, blend=0, seed=0):
+    """Deferred-shading draws whose triangles sit on the tiles' 4x4-pixel
+    quadrant boundaries: tiny triangles centred on pixel coordinates that are
+    multiples of 4 (jittered by 0, +-1/16 or +-1/2 pixel), right triangles
+    with both legs exactly on such lines, and larger triangles spanning
+    several quadrants and tiles, in random order.  Without the depth test the
+    last triangle of a sample wins, so every pixel's submission order is
+    visible in the image."""
+    from cudaraster import scenes
+    rng = np.random.default_rng(seed)
+    width, height, n = 96, 64, 3000
+    kind = rng.integers(0, 3, n)
+    cx = rng.integers(0, width // 4 + 1, n) * 4.0 + rng.choice([0.0, 1 / 16, -1 / 16, 0.5, -0.5], n)
+    cy = rng.integers(0, height // 4 + 1, n) * 4.0 + rng.choice([0.0, 1 / 16, -1 / 16, 0.5, -0.5], n)
+    area = np.where(kind == 0, 0.3 + 5.7 * rng.random(n), 20.0 + 180.0 * rng.random(n))
+    sx, sy = scenes._triangle_shapes(rng, n, area)
+    px, py = cx[:, None] + sx, cy[:, None] + sy
+    # Right triangles: the corner on a boundary crossing, legs along x = 4i and y = 4j.
+    legs = rng.choice([-1.0, 1.0], (n, 2)) * rng.integers(1, 9, (n, 2))
+    r = kind == 1
+    px[r] = np.stack([cx[r], cx[r] + legs[r, 0], cx[r]], axis=1).round()
+    py[r] = np.stack([cy[r], cy[r], cy[r] + legs[r, 1]], axis=1).round()
+    w = np.repeat(1.0 + 9.0 * rng.random((n, 1)), 3, axis=1)
+    clip = scenes._to_clip(px, py, w, width, height)
+    col = rng.random((n, 3, 4))
+    verts = np.concatenate([clip, col], axis=-1).reshape(n * 3, 8).astype(np.float32)
+    idx = np.arange(n * 3, dtype=np.int32).reshape(n, 3)       # about half face away (random winding)
+    return scenes.Scene(f"quadrants s={samples_log2} f={flags} b={blend}", width, height, samples_log2, flags,
+                        blend, scenes.SHADER_GOURAUD, np.ascontiguousarray(verts), idx)
+
+
+@pytest.mark.parametrize("samples_log2", [0, 1, 2, 3])
+@pytest.mark.parametrize("depth", [True, False])
+def test_quadrant_boundaries(oracle_lib, samples_log2, depth):
+    """The deferred-shading walk gives each 4x4 quadrant its own record queue;
+    a record reaching a quadrant only across its boundary, or an edge on the
+    boundary, must still be walked there, in submission order."""
+    _torch()
+    from cudaraster import scenes
+    flags = scenes.FLAG_LERP | (scenes.FLAG_DEPTH if depth else 0)
+    s = quadrant_scene(samples_log2, flags, seed=40 + samples_log2 + (4 if depth else 0))
+    ref = oracle_lib.render(s)
+    assert ref["stats"]["numSubtris"] > 1000
+    assert_parity(render_gpu(s), ref, s.name)
+
+
+def test_quadrant_boundaries_depth_only(oracle_lib):
+    _torch()
+    from cudaraster import scenes
+    s = quadrant_scene(2, scenes.FLAG_DEPTH | scenes.FLAG_LERP, blend=scenes.BLEND_DEPTHONLY, seed=9)
+    assert_parity(render_gpu(s), oracle_lib.render(s), s.name)
+
+
 def test_zero_triangles(oracle_lib):
     """A draw without triangles still applies the deferred clear."""
     _torch()
