@@ -141,7 +141,7 @@ cr_status ensureBuffers(cr_context* ctx, const Geometry& g, int32_t numChunks) {
   CR_HIP(ctx, ctx->tileStart.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileCount.ensure((size_t)g.nt * 4));
   CR_HIP(ctx, ctx->tileList.ensure((size_t)ctx->maxTileEntries * 4));
-  CR_HIP(ctx, ctx->tileSegNext.ensure((size_t)std::max<int64_t>(ctx->maxTileEntries >> kTileSegLog2, 1) * 4));
+  CR_HIP(ctx, ctx->tileSegNext.ensure((size_t)std::max<int64_t>(ctx->maxTileEntries >> kTileSegLog2, 1) * 8));
   if (countingDraw(ctx)) {
     CR_HIP(ctx, ctx->tileFrags.ensure((size_t)g.nt * 4));
     CR_HIP(ctx, ctx->chunkSubtris.ensure((size_t)std::max(numChunks, 1) * 4));

@@ -124,7 +124,7 @@ cr_status layout(cr_batch* b) {
   CR_HIP(ctx, b->tileStart.ensure(nTile * 4));
   CR_HIP(ctx, b->tileCount.ensure(nTile * 4));
   CR_HIP(ctx, b->tileList.ensure(nTileList * 4));
-  CR_HIP(ctx, b->tileSegNext.ensure(std::max<size_t>(nTileList >> kTileSegLog2, 1) * 4));
+  CR_HIP(ctx, b->tileSegNext.ensure(std::max<size_t>(nTileList >> kTileSegLog2, 1) * 8));
   CR_HIP(ctx, b->counters.ensure(b->views.size() * sizeof(Counters)));
   b->reserved = b->triSubtris.bytes + b->triHeader.bytes + b->triData.bytes + b->subtriTri.bytes +
                 b->chunkBase.bytes + b->binCnt.bytes + b->binOff.bytes + b->binList.bytes + b->tileStart.bytes +
@@ -179,7 +179,7 @@ Params viewParams(const cr_batch* b, int i) {
   P.tileCount = b->tileCount.as<uint32_t>() + v.oTile;
   P.maxTileEntries = (int32_t)v.maxTile;
   P.tileList = b->tileList.as<uint32_t>() + v.oTileList;
-  P.tileSegNext = b->tileSegNext.as<uint32_t>() + (v.oTileList >> kTileSegLog2);   // oTileList: a multiple of 64
+  P.tileSegNext = b->tileSegNext.as<uint32_t>() + 2 * (v.oTileList >> kTileSegLog2);   // oTileList: a multiple of 64
   P.color = d.color->dev;
   P.depth = d.depth->dev;
   P.triid = d.tri_id ? d.tri_id->dev : nullptr;

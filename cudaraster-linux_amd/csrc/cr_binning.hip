@@ -285,10 +285,12 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   __shared__ uint32_t segAddr[kSegBatch];
   __shared__ uint64_t emitMask[kCoarseWaves][kBinTiles];
   __shared__ uint32_t emitPre[kCoarseWaves][kBinTiles];
-  // Per tile of the bin: list length so far, first and last segment.
+  // Per tile of the bin: list length so far, first, last and second-to-last
+  // segment (the last two wait for their link to the segment after next).
   __shared__ uint32_t tTotal[kBinTiles];
   __shared__ uint32_t tFirst[kBinTiles];
   __shared__ uint32_t tLast[kBinTiles];
+  __shared__ uint32_t tPrev[kBinTiles];
   // The round's segment map of each tile (one LDS read per lookup):
   // segments the list held before the round, its last one then, the
   // round's first new one.
@@ -318,6 +320,7 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
     tTotal[i] = 0;
     tFirst[i] = kNoSeg;
     tLast[i] = kNoSeg;
+    tPrev[i] = kNoSeg;
   }
   if (threadIdx.x == 0) sWritten = 0;
   __syncthreads();
@@ -450,14 +453,27 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
         tRound[threadIdx.x] = make_uint4(kOld, oldLast, first, 0u);
         if (need > 0) {
           // Chain the round's segments (consecutive ids) to the tile's
-          // list.  Segments past the buffer (an overflowing draw, re-run
-          // larger) are never written.
-          const uint32_t last = first + need - 1;
-          if (oldLast == kNoSeg) tFirst[threadIdx.x] = first;
-          else if (oldLast < segCap) P.tileSegNext[oldLast] = first;
-          for (uint32_t j = first; j < last; j++)
-            if (j < segCap) P.tileSegNext[j] = j + 1;
-          if (last < segCap) P.tileSegNext[last] = kNoSeg;
+          // list: each segment's next one and the one after that.  The
+          // list's last two segments learn the latter from a later round
+          // (or never need it).  Segments past the buffer (an overflowing
+          // draw, re-run larger) are never written.
+          uint32_t* links = P.tileSegNext;
+          const uint32_t last = first + need - 1, prev = tPrev[threadIdx.x];
+          if (oldLast == kNoSeg) {
+            tFirst[threadIdx.x] = first;
+          } else if (oldLast < segCap) {
+            links[2 * oldLast] = first;
+            if (need > 1) links[2 * oldLast + 1] = first + 1;
+          }
+          if (prev != kNoSeg && prev < segCap) links[2 * prev + 1] = first;
+          for (uint32_t j = first; j < last; j++) {
+            if (j < segCap) {
+              links[2 * j] = j + 1;
+              if (j + 1 < last) links[2 * j + 1] = j + 2;
+            }
+          }
+          if (last < segCap) links[2 * last] = kNoSeg;
+          tPrev[threadIdx.x] = need > 1 ? last - 1 : oldLast;
           tLast[threadIdx.x] = last;
         }
       }

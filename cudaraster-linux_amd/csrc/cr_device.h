@@ -173,13 +173,15 @@ struct Params {
   int32_t maxBinEntries;
   uint4* binList;                  // per entry: slot, then the header's three snapped vertices
   // Tile lists in submission order, each a chain of kTileSeg-entry
-  // segments: segment s holds tileList[s * kTileSeg ...], tileSegNext[s] is
-  // the next segment of its tile (kNoSeg after the last).
+  // segments: segment s holds tileList[s * kTileSeg ...]; tileSegNext[2s] is
+  // the next segment of its tile (kNoSeg after the last) and
+  // tileSegNext[2s + 1] the one after that (written once it exists; never
+  // read past the list's end).
   uint32_t* tileStart;             // [numTiles] first segment (kNoSeg: empty list)
   uint32_t* tileCount;             // [numTiles] entries in the list
   int32_t maxTileEntries;          // tileList capacity; maxTileEntries / kTileSeg segments
   uint32_t* tileList;
-  uint32_t* tileSegNext;           // [maxTileEntries / kTileSeg]
+  uint32_t* tileSegNext;           // [2 * maxTileEntries / kTileSeg]
   // Surfaces: (widthPixels * N) x heightPixels texels, row 0 at the bottom.
   uint32_t* color;
   uint32_t* depth;

@@ -268,20 +268,28 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
   // covered samples, fragments whose covered samples all fail the depth
   // test, fragments written.
   uint64_t pHiz = 0, pWalked = 0, pEmpty = 0, pSamples = 0, pZKill = 0, pWritten = 0;
-  // The tile's list: a chain of kTileSeg-entry segments, one per chunk, read
-  // one chunk ahead -- the next chunk's entries and the link after them are
-  // in flight while the current chunk is tested (three loads in parallel
-  // instead of a chain of three per chunk).
-  static_assert(kChunk == kTileSeg, "one list segment per chunk");
+  // The tile's list: a chain of kTileSeg-entry segments, read one step
+  // ahead -- the next step's entries and the link after them are in flight
+  // while this step's are tested, so a step waits for one load latency, not
+  // a chain.  The in-order copies (deep, mostly culled lists) take two
+  // segments per step, 64 entries, one per lane: each half of the wave
+  // follows every other segment -- lanes 0-31 the even ones, lanes 32-63
+  // the odd ones -- through the links to the segment after next.  The
+  // deferred-shading copies take one segment per step (their lists are
+  // short, and the second half's state would not fit their registers).
+  constexpr uint32_t kHalves = VIS ? 1u : 2u;
+  static_assert(kChunk == kTileSeg, "one list segment per half step");
   // (The links stay in vector registers: a scalar copy would wait for each
-  // load as soon as it is issued.)
-  // The first segment's entries and link are loaded without a branch:
-  // segment 0 (always allocated) stands in for an empty list, whose start
-  // word was not written by this draw; entries past the count and a link
-  // past the last segment are never used.
+  // load as soon as it is issued.)  The first segments are loaded without a
+  // branch: segment 0 (always allocated) stands in for a missing one (an
+  // empty list's start word was not written by this draw); entries past the
+  // count and links past the last segment are never used.
+  const uint32_t half = kHalves == 2 ? (uint32_t)lane >> 5 : 0u, hl = (uint32_t)lane & 31u;
   const uint32_t seg0 = cnt > 0 ? (uint32_t)__builtin_amdgcn_readfirstlane((int)startLoaded) : 0u;
-  uint32_t cur = tileList[((size_t)seg0 << kTileSegLog2) + lane];
-  uint32_t segNext = P.tileSegNext[seg0];
+  const uint2 link0 = *reinterpret_cast<const uint2*>(P.tileSegNext + 2 * (size_t)seg0);
+  const uint32_t mySeg = half ? (cnt > (uint32_t)kChunk ? link0.x : 0u) : seg0;
+  uint32_t cur = tileList[((size_t)mySeg << kTileSegLog2) + hl];
+  uint32_t myNext = kHalves == 1 ? link0.x : (half ? P.tileSegNext[2 * (size_t)mySeg + 1] : link0.y);
   const bool hiz = DEPTH && P.hiz;
 
   // Hierarchical Z: a triangle whose conservative zmin is not nearer than
@@ -303,328 +311,335 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
   static_assert(Q0 != Q1 && Q1 >= L::SH + (VIS ? 0 : 21), "two record words free for the queue");
   uint32_t queued = 0, sinceWalk = 0;
 
-  for (uint32_t c = 0; c < cnt; c += kChunk) {
-    // ---- lane j < kChunk loads and tests entry j of this chunk -------------
-    if (hiz && zStale) {
-      if (c == 0 && P.deferredClear) {
-        tileZMax = P.clearDepth;             // every sample still holds the clear depth
-      } else {
-        uint32_t mx = dep[0];
-#pragma unroll
-        for (int i = 1; i < N; i++) mx = max(mx, dep[i]);
-        tileZMax = (uint32_t)__builtin_amdgcn_readfirstlane((int)waveMax(mx));
-      }
-      zStale = false;
-    }
-    const uint32_t n = min((uint32_t)kChunk, cnt - c);
-    const bool lastChunk = c + kChunk >= cnt;
-    uint32_t slot = cur;
-    const bool valid = lane < (int)n && slot < (uint32_t)P.maxSubtris;
+  for (uint32_t c = 0; c < cnt; c += kHalves * kChunk) {
+    // ---- lane j loads entry c + j of the list ------------------------------
+    const uint32_t n = min(kHalves * kChunk, cnt - c);
+    const bool lastStep = c + kHalves * kChunk >= cnt;
+    const uint32_t entry = cur;
+    const bool valid = lane < (int)n && entry < (uint32_t)P.maxSubtris;
     // The header's last word: the triangle's conservative zmin (high 20
-    // bits), issued before the next chunk's loads so that its wait does not
+    // bits), issued before the next step's loads so that its wait does not
     // cover them, and by every lane (slot 0 stands in for an empty one): a
     // load under a branch would be waited for at the branch's end.
-    const uint32_t zmin = reinterpret_cast<const uint32_t*>(hdr)[(size_t)(valid ? slot : 0u) * 4 + 3];
-    // The next chunk's entries and link, loaded by every lane without a
-    // branch (so that waiting for zmin does not wait for them); after the
-    // last chunk, segment 0 (always allocated) stands in.  Entries past the
-    // list's end and a link past its last segment are never used.
-    const uint32_t segSafe = lastChunk ? 0u : segNext;
-    const uint32_t nxt = tileList[((size_t)segSafe << kTileSegLog2) + lane];
-    const uint32_t segAfter = P.tileSegNext[segSafe];
-    // (Bitwise, not short-circuit: the load's value is used on every path.)
-    const bool keep = valid & (!hiz | ((zmin & 0xFFFFF000u) < tileZMax));
-    const uint64_t keepMask = __ballot(keep);
-    const uint32_t m = (uint32_t)__popcll(keepMask);
-    if (COUNT) {
-      pHiz += n - m;
-      pWalked += m;
-    }
-    if (keep) {
-      const uint32_t p = queued + rankBelow(keepMask);
-      rec[(p & (kChunk - 1)) * RS + (p < (uint32_t)kChunk ? Q0 : Q1)] = slot;
-    }
-    queued += m;
+    const uint32_t zmin = reinterpret_cast<const uint32_t*>(hdr)[(size_t)(valid ? entry : 0u) * 4 + 3];
+    // This half's next segment (entries c + kHalves * 32 + 32 * half on) and
+    // the link after it, loaded by every lane without a branch (so that
+    // waiting for zmin does not wait for them); segment 0 stands in past the
+    // list.
+    const uint32_t segSafe = c + kHalves * kChunk + kChunk * half < cnt ? myNext : 0u;
+    const uint32_t nxt = tileList[((size_t)segSafe << kTileSegLog2) + hl];
+    const uint32_t after = P.tileSegNext[2 * (size_t)segSafe + (kHalves - 1)];
     cur = nxt;
-    segNext = segAfter;                      // used only while c + 2 * kChunk < cnt
-    const bool walkNow = !hiz || lastChunk || queued >= (uint32_t)kChunk || ++sinceWalk >= (uint32_t)kMaxDefer;
-    if (!walkNow || queued == 0) continue;
-    const uint32_t total = queued;
-    queued = 0;
-    sinceWalk = 0;
-    zStale = true;
-
-    for (uint32_t b = 0; b < total; b += kChunk) {
-      // ---- set up the batch: lane j, record j, queue entry b + j -------------
-      const uint32_t nb = min((uint32_t)kChunk, total - b);
-      uint32_t* r = rec + lane * RS;
-      const bool su = lane < (int)nb;
-      // Deferred shading: the 4x4-pixel quadrants of the tile this record
-      // may cover (bit (ly >= 4) * 2 + (lx >= 4)).
-      uint32_t quads = 0;
-      waveSync();                            // the queue stores (or the last walk's reads) are done
-      if (su) {
-        slot = r[b == 0 ? Q0 : Q1];
-        // Header and depth plane, read again here (cache hits): only the
-        // set-up lanes of a batch hold them.
-        const uint4 h = hdr[slot];
-        const uint4 z = DEPTH ? tdata[(size_t)slot * 4] : make_uint4(0, 0, 0, 0);
-        int32_t ax[3], ay[3];
-        ax[0] = (int32_t)(int16_t)(h.x & 0xFFFFu) - Tx; ay[0] = (int32_t)(int16_t)(h.x >> 16) - Ty;
-        ax[1] = (int32_t)(int16_t)(h.y & 0xFFFFu) - Tx; ay[1] = (int32_t)(int16_t)(h.y >> 16) - Ty;
-        ax[2] = (int32_t)(int16_t)(h.z & 0xFFFFu) - Tx; ay[2] = (int32_t)(int16_t)(h.z >> 16) - Ty;
-        // Vertices within 2^14 subpixels of the tile origin (all but large
-        // triangles): a x d fits 31 bits, so it is two 24-bit products and
-        // needs no clamp.  Uniform over the wave's set-up lanes.
-        const uint32_t span = (uint32_t)(abs(ax[0]) | abs(ay[0]) | abs(ax[1]) | abs(ay[1]) | abs(ax[2]) | abs(ay[2]));
-        const bool near = __ballot(span >= (1u << 14)) == 0;
-        // Whole tile covered: at every sample of every pixel each edge's
-        // L = lx16*dy + ly16*(-dx) (lx16, ly16 in 0..112) is at most the
-        // sample's threshold, i.e. its largest value over the tile is at most
-        // the smallest threshold.  Exactly the per-sample test's outcome.
-        bool full = true;
-        if (VIS) quads = 15u;
+    myNext = after;
+    // The two halves are tested and queued in list order, each followed by a
+    // walk when it is due (the queue holds at most 2 * kChunk - 1 entries),
+    // so the second half is tested against the depths the first one left.
+    const int halves = kHalves == 2 && n > (uint32_t)kChunk ? 2 : 1;
+    for (int h = 0; h < halves; h++) {
+      if (hiz && zStale) {
+        if (c == 0 && h == 0 && P.deferredClear) {
+          tileZMax = P.clearDepth;           // every sample still holds the clear depth
+        } else {
+          uint32_t mx = dep[0];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-          const int kn = (k + 1) % 3;
-          const int32_t dx = ax[kn] - ax[k], dy = ay[kn] - ay[k];
-          // Edge function at the tile origin, a x d, minus 1 on exclusive
-          // edges (Util.inl:100).  |a|,|d| <= 2^16.  Clamping to +-2^30
-          // keeps the sign of a x d - s x d exact for every in-tile sample
-          // (|s x d| < 2^25), and every threshold below fits in 32 bits.
-          // With |a| < 2^14 (so |d| < 2^15), |a x d| < 2^30: the clamp never
-          // applies and 32-bit arithmetic gives the same value.
-          const int32_t excl = (dy > 0 || (dy == 0 && dx <= 0)) ? 1 : 0;
-          int32_t ck;
-          if (near) {
-            ck = __mul24(ax[k], dy) - __mul24(ay[k], dx) - excl;
-          } else {
-            int64_t cc = (int64_t)ax[k] * dy - (int64_t)ay[k] * dx - excl;
-            cc = cc > (1ll << 30) ? (1ll << 30) : (cc < -(1ll << 30) ? -(1ll << 30) : cc);
-            ck = (int32_t)cc;
-          }
-          int32_t minR = INT32_MAX, maxR = INT32_MIN;
-#pragma unroll
-          for (int i = 0; i < N; i++) {
-            const int32_t ox = (2 * msaaPattern(S, i) + 1) << (3 - S);
-            const int32_t oy = (2 * i + 1) << (3 - S);
-            // |ox|, |oy| < 16 and |dx|, |dy| < 2^17: exact 24-bit products.
-            const int32_t rk = ck - (__mul24(ox, dy) - __mul24(oy, dx));
-            r[L::R + k * N + i] = (uint32_t)rk;
-            minR = min(minR, rk);
-            maxR = max(maxR, rk);
-          }
-          full = full && __mul24(112, max(dy, 0)) + __mul24(112, max(-dx, 0)) <= minR;
-          if (VIS) {
-            // A quadrant none of whose samples can pass this edge: the
-            // smallest L over its pixels (lx16 in 64*qx + {0..48}, ly16 in
-            // 64*qy + {0..48}) exceeds the largest threshold.  |d| < 2^17:
-            // every term below fits 25 bits.
-            const int32_t ndx = -dx;
-            const int32_t m00 = (dy < 0 ? __mul24(48, dy) : 0) + (ndx < 0 ? __mul24(48, ndx) : 0);
-            const int32_t ex = dy * 64, ey = ndx * 64;
-            quads &= (m00 <= maxR ? 1u : 0u) | (m00 + ex <= maxR ? 2u : 0u) | (m00 + ey <= maxR ? 4u : 0u) |
-                     (m00 + ex + ey <= maxR ? 8u : 0u);
-          }
-          r[L::D + 2 * k] = (uint32_t)dy;
-          r[L::D + 2 * k + 1] = (uint32_t)(-dx);
+          for (int i = 1; i < N; i++) mx = max(mx, dep[i]);
+          tileZMax = (uint32_t)__builtin_amdgcn_readfirstlane((int)waveMax(mx));
         }
-        if (DEPTH) {
-          r[L::Z] = (z.x & 0xFFFFu) | (z.y << 16);
-          r[L::Z + 1] = (z.x >> 16) | (z.y & 0xFFFF0000u);
-          // Depth at each sample of the tile's first pixel: one full 32-bit
-          // product per axis, then the sample offsets (< 8) as shifts and adds.
-          const uint32_t base = z.x * tileSx + z.y * tileSy + z.z;
-          sampleDepths<S>(r + L::ZS, base, z.x, z.y, std::make_integer_sequence<int, N>{});
-        }
-        // Deferred shading records the (slot << 4 | centroid code) words the
-        // walk stores per sample; at 1x the code is always the pixel centre.
-        if (VIS) r[L::SLOT] = (slot << 4) | (N == 1 ? 15u : 0u);
-        // In-order shading: the source triangle, bit 31 set when the triangle
-        // covers the whole tile (source indices stay below 2^28).
-        if (!VIS)
-          r[L::TRI] = ((slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris]) |
-                      (full ? 0x80000000u : 0u);
-        if (TEX) {
-          // Barycentric planes and vertex indices; the shader reads the
-          // varyings, material and textures itself.
-          const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
-                      t3 = tdata[(size_t)slot * 4 + 3];
-          uint32_t* q = r + L::SH;
-          q[0] = t1.x; q[1] = t1.y; q[2] = t1.z; q[3] = t1.w; q[4] = t2.x; q[5] = t2.y; q[6] = t2.z;
-          q[7] = t2.w; q[8] = t3.x; q[9] = t3.y; q[10] = t3.z; q[11] = t3.w;
-        } else if (!VIS && gouraud) {
-          const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
-                      t3 = tdata[(size_t)slot * 4 + 3];
-          const float4 c0 = varying0(verts, t3.y, P.vertexStride), c1 = varying0(verts, t3.z, P.vertexStride),
-                       c2 = varying0(verts, t3.w, P.vertexStride);
-          uint32_t* q = r + L::SH;
-          q[0] = t1.x; q[1] = t1.y; q[2] = t1.z; q[3] = t1.w; q[4] = t2.x; q[5] = t2.y; q[6] = t2.z;
-          q[7] = t2.w; q[8] = t3.x;
-          q[9] = __float_as_uint(c0.x); q[10] = __float_as_uint(c0.y); q[11] = __float_as_uint(c0.z);
-          q[12] = __float_as_uint(c0.w); q[13] = __float_as_uint(c1.x); q[14] = __float_as_uint(c1.y);
-          q[15] = __float_as_uint(c1.z); q[16] = __float_as_uint(c1.w); q[17] = __float_as_uint(c2.x);
-          q[18] = __float_as_uint(c2.y); q[19] = __float_as_uint(c2.z); q[20] = __float_as_uint(c2.w);
-        }
-        // Quadrant queue slots start empty (this record's queue entry was
-        // read above).
-        if (VIS) r[Q0] = 0xFFFFFFFFu;
+        zStale = false;
       }
-      waveSync();
+      // (Bitwise, not short-circuit: the load's value is used on every path.)
+      const bool keep = valid & ((int)half == h) & (!hiz | ((zmin & 0xFFFFF000u) < tileZMax));
+      const uint64_t km = __ballot(keep);
+      const uint32_t m = (uint32_t)__popcll(km);
+      if (COUNT) {
+        pHiz += min((uint32_t)kChunk, n - (uint32_t)h * kChunk) - m;
+        pWalked += m;
+      }
+      if (keep) {
+        const uint32_t p = queued + rankBelow(km);
+        rec[(p & (kChunk - 1)) * RS + (p < (uint32_t)kChunk ? Q0 : Q1)] = entry;
+      }
+      queued += m;
+      const bool lastChunk = lastStep && h == halves - 1;
+      const bool walkNow = !hiz || lastChunk || queued >= (uint32_t)kChunk || ++sinceWalk >= (uint32_t)kMaxDefer;
+      if (!walkNow || queued == 0) continue;
+      const uint32_t total = queued;
+      queued = 0;
+      sinceWalk = 0;
+      zStale = true;
 
-      // Deferred shading: per 4x4 quadrant, the batch's records that may
-      // cover it, in submission order -- byte q of record i's word Q0 is the
-      // index of quadrant q's i-th record (0xFF past its end).  Step i of the
-      // walk takes the i-th record of every quadrant, so the quadrants walk
-      // their own records side by side; their pixels are disjoint, so every
-      // pixel still sees its triangles in submission order, and a step
-      // serves up to four small triangles.
-      uint32_t steps = nb;
-      if (VIS) {
-        steps = 0;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const uint64_t m = __ballot((quads >> q) & 1u);
-          steps = max(steps, (uint32_t)__popcll(m));
-          if ((quads >> q) & 1u)
-            reinterpret_cast<uint8_t*>(rec + rankBelow(m) * RS + Q0)[q] = (uint8_t)lane;
+      for (uint32_t b = 0; b < total; b += kChunk) {
+        // ---- set up the batch: lane j, record j, queue entry b + j -------------
+        const uint32_t nb = min((uint32_t)kChunk, total - b);
+        uint32_t* r = rec + lane * RS;
+        const bool su = lane < (int)nb;
+        // Deferred shading: the 4x4-pixel quadrants of the tile this record
+        // may cover (bit (ly >= 4) * 2 + (lx >= 4)).
+        uint32_t quads = 0;
+        waveSync();                            // the queue stores (or the last walk's reads) are done
+        if (su) {
+          const uint32_t slot = r[b == 0 ? Q0 : Q1];
+          // Header and depth plane, read again here (cache hits): only the
+          // set-up lanes of a batch hold them.
+          const uint4 h = hdr[slot];
+          const uint4 z = DEPTH ? tdata[(size_t)slot * 4] : make_uint4(0, 0, 0, 0);
+          int32_t ax[3], ay[3];
+          ax[0] = (int32_t)(int16_t)(h.x & 0xFFFFu) - Tx; ay[0] = (int32_t)(int16_t)(h.x >> 16) - Ty;
+          ax[1] = (int32_t)(int16_t)(h.y & 0xFFFFu) - Tx; ay[1] = (int32_t)(int16_t)(h.y >> 16) - Ty;
+          ax[2] = (int32_t)(int16_t)(h.z & 0xFFFFu) - Tx; ay[2] = (int32_t)(int16_t)(h.z >> 16) - Ty;
+          // Vertices within 2^14 subpixels of the tile origin (all but large
+          // triangles): a x d fits 31 bits, so it is two 24-bit products and
+          // needs no clamp.  Uniform over the wave's set-up lanes.
+          const uint32_t span = (uint32_t)(abs(ax[0]) | abs(ay[0]) | abs(ax[1]) | abs(ay[1]) | abs(ax[2]) | abs(ay[2]));
+          const bool near = __ballot(span >= (1u << 14)) == 0;
+          // Whole tile covered: at every sample of every pixel each edge's
+          // L = lx16*dy + ly16*(-dx) (lx16, ly16 in 0..112) is at most the
+          // sample's threshold, i.e. its largest value over the tile is at most
+          // the smallest threshold.  Exactly the per-sample test's outcome.
+          bool full = true;
+          if (VIS) quads = 15u;
+  #pragma unroll
+          for (int k = 0; k < 3; k++) {
+            const int kn = (k + 1) % 3;
+            const int32_t dx = ax[kn] - ax[k], dy = ay[kn] - ay[k];
+            // Edge function at the tile origin, a x d, minus 1 on exclusive
+            // edges (Util.inl:100).  |a|,|d| <= 2^16.  Clamping to +-2^30
+            // keeps the sign of a x d - s x d exact for every in-tile sample
+            // (|s x d| < 2^25), and every threshold below fits in 32 bits.
+            // With |a| < 2^14 (so |d| < 2^15), |a x d| < 2^30: the clamp never
+            // applies and 32-bit arithmetic gives the same value.
+            const int32_t excl = (dy > 0 || (dy == 0 && dx <= 0)) ? 1 : 0;
+            int32_t ck;
+            if (near) {
+              ck = __mul24(ax[k], dy) - __mul24(ay[k], dx) - excl;
+            } else {
+              int64_t cc = (int64_t)ax[k] * dy - (int64_t)ay[k] * dx - excl;
+              cc = cc > (1ll << 30) ? (1ll << 30) : (cc < -(1ll << 30) ? -(1ll << 30) : cc);
+              ck = (int32_t)cc;
+            }
+            int32_t minR = INT32_MAX, maxR = INT32_MIN;
+  #pragma unroll
+            for (int i = 0; i < N; i++) {
+              const int32_t ox = (2 * msaaPattern(S, i) + 1) << (3 - S);
+              const int32_t oy = (2 * i + 1) << (3 - S);
+              // |ox|, |oy| < 16 and |dx|, |dy| < 2^17: exact 24-bit products.
+              const int32_t rk = ck - (__mul24(ox, dy) - __mul24(oy, dx));
+              r[L::R + k * N + i] = (uint32_t)rk;
+              minR = min(minR, rk);
+              maxR = max(maxR, rk);
+            }
+            full = full && __mul24(112, max(dy, 0)) + __mul24(112, max(-dx, 0)) <= minR;
+            if (VIS) {
+              // A quadrant none of whose samples can pass this edge: the
+              // smallest L over its pixels (lx16 in 64*qx + {0..48}, ly16 in
+              // 64*qy + {0..48}) exceeds the largest threshold.  |d| < 2^17:
+              // every term below fits 25 bits.
+              const int32_t ndx = -dx;
+              const int32_t m00 = (dy < 0 ? __mul24(48, dy) : 0) + (ndx < 0 ? __mul24(48, ndx) : 0);
+              const int32_t ex = dy * 64, ey = ndx * 64;
+              quads &= (m00 <= maxR ? 1u : 0u) | (m00 + ex <= maxR ? 2u : 0u) | (m00 + ey <= maxR ? 4u : 0u) |
+                       (m00 + ex + ey <= maxR ? 8u : 0u);
+            }
+            r[L::D + 2 * k] = (uint32_t)dy;
+            r[L::D + 2 * k + 1] = (uint32_t)(-dx);
+          }
+          if (DEPTH) {
+            r[L::Z] = (z.x & 0xFFFFu) | (z.y << 16);
+            r[L::Z + 1] = (z.x >> 16) | (z.y & 0xFFFF0000u);
+            // Depth at each sample of the tile's first pixel: one full 32-bit
+            // product per axis, then the sample offsets (< 8) as shifts and adds.
+            const uint32_t base = z.x * tileSx + z.y * tileSy + z.z;
+            sampleDepths<S>(r + L::ZS, base, z.x, z.y, std::make_integer_sequence<int, N>{});
+          }
+          // Deferred shading records the (slot << 4 | centroid code) words the
+          // walk stores per sample; at 1x the code is always the pixel centre.
+          if (VIS) r[L::SLOT] = (slot << 4) | (N == 1 ? 15u : 0u);
+          // In-order shading: the source triangle, bit 31 set when the triangle
+          // covers the whole tile (source indices stay below 2^28).
+          if (!VIS)
+            r[L::TRI] = ((slot < (uint32_t)P.numTris) ? slot : subtriTri[slot - (uint32_t)P.numTris]) |
+                        (full ? 0x80000000u : 0u);
+          if (TEX) {
+            // Barycentric planes and vertex indices; the shader reads the
+            // varyings, material and textures itself.
+            const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
+                        t3 = tdata[(size_t)slot * 4 + 3];
+            uint32_t* q = r + L::SH;
+            q[0] = t1.x; q[1] = t1.y; q[2] = t1.z; q[3] = t1.w; q[4] = t2.x; q[5] = t2.y; q[6] = t2.z;
+            q[7] = t2.w; q[8] = t3.x; q[9] = t3.y; q[10] = t3.z; q[11] = t3.w;
+          } else if (!VIS && gouraud) {
+            const uint4 t1 = tdata[(size_t)slot * 4 + 1], t2 = tdata[(size_t)slot * 4 + 2],
+                        t3 = tdata[(size_t)slot * 4 + 3];
+            const float4 c0 = varying0(verts, t3.y, P.vertexStride), c1 = varying0(verts, t3.z, P.vertexStride),
+                         c2 = varying0(verts, t3.w, P.vertexStride);
+            uint32_t* q = r + L::SH;
+            q[0] = t1.x; q[1] = t1.y; q[2] = t1.z; q[3] = t1.w; q[4] = t2.x; q[5] = t2.y; q[6] = t2.z;
+            q[7] = t2.w; q[8] = t3.x;
+            q[9] = __float_as_uint(c0.x); q[10] = __float_as_uint(c0.y); q[11] = __float_as_uint(c0.z);
+            q[12] = __float_as_uint(c0.w); q[13] = __float_as_uint(c1.x); q[14] = __float_as_uint(c1.y);
+            q[15] = __float_as_uint(c1.z); q[16] = __float_as_uint(c1.w); q[17] = __float_as_uint(c2.x);
+            q[18] = __float_as_uint(c2.y); q[19] = __float_as_uint(c2.z); q[20] = __float_as_uint(c2.w);
+          }
+          // Quadrant queue slots start empty (this record's queue entry was
+          // read above).
+          if (VIS) r[Q0] = 0xFFFFFFFFu;
+        }
+        waveSync();
+
+        // Deferred shading: per 4x4 quadrant, the batch's records that may
+        // cover it, in submission order -- byte q of record i's word Q0 is the
+        // index of quadrant q's i-th record (0xFF past its end).  Step i of the
+        // walk takes the i-th record of every quadrant, so the quadrants walk
+        // their own records side by side; their pixels are disjoint, so every
+        // pixel still sees its triangles in submission order, and a step
+        // serves up to four small triangles.
+        uint32_t steps = nb;
+        if (VIS) {
+          steps = 0;
+  #pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const uint64_t m = __ballot((quads >> q) & 1u);
+            steps = max(steps, (uint32_t)__popcll(m));
+            if ((quads >> q) & 1u)
+              reinterpret_cast<uint8_t*>(rec + rankBelow(m) * RS + Q0)[q] = (uint8_t)lane;
+          }
+          waveSync();
+        }
+
+        // ---- walk the batch's triangles in submission order --------------------
+        for (uint32_t j = 0; j < steps; j++) {
+          // In-order shading: record j for the whole wave (uniform address).
+          // Deferred shading: this lane's quadrant's j-th record; lanes whose
+          // quadrant has no j-th record read record 31 and take no part.
+          uint64_t validM = ~0ull;
+          const uint32_t* r = rec + j * RS;
+          if (VIS) {
+            const uint32_t t = (r[Q0] >> quadShift) & 0xFFu;
+            validM = __ballot(t < (uint32_t)kChunk);
+            r = rec + min(t, (uint32_t)kChunk - 1) * RS;
+          }
+          const uint32_t* cur = r;
+          // |lx16|,|ly16| < 2^7 and |d| < 2^17: 24-bit multiplies are exact.
+          // In-order shading: a record covering the whole tile skips the
+          // per-sample edge tests (a scalar branch: the flag is uniform).
+          const uint32_t triWord = VIS ? 0u : (uint32_t)__builtin_amdgcn_readfirstlane((int)r[L::TRI]);
+          const bool fullTile = !VIS && (triWord >> 31) != 0;
+          // Per-sample coverage and write masks are kept as explicit 64-bit lane
+          // masks (scalar registers): a per-lane bool carried from one block to
+          // the next would be rebuilt from a vector register before each ballot.
+          uint64_t covM[N];
+          uint64_t anyM = 0;
+          if (fullTile) {
+  #pragma unroll
+            for (int i = 0; i < N; i++) covM[i] = ~0ull;
+            anyM = ~0ull;
+          } else {
+            int32_t Lk[3];
+  #pragma unroll
+            for (int k = 0; k < 3; k++)
+              Lk[k] = __mul24(lx16, (int32_t)cur[L::D + 2 * k]) + __mul24(ly16, (int32_t)cur[L::D + 2 * k + 1]);
+  #pragma unroll
+            for (int i = 0; i < N; i++) {
+              covM[i] = __ballot(Lk[0] <= (int32_t)cur[L::R + i]) & __ballot(Lk[1] <= (int32_t)cur[L::R + N + i]) &
+                        __ballot(Lk[2] <= (int32_t)cur[L::R + 2 * N + i]) & validM;
+              anyM |= covM[i];
+            }
+          }
+          if (COUNT) {
+            pEmpty += anyM == 0;
+  #pragma unroll
+            for (int i = 0; i < N; i++) pSamples += (uint64_t)__popcll(covM[i]);
+          }
+          if (anyM == 0) continue;
+          if (COUNT) frags += (uint64_t)__popcll(anyM);
+
+          uint64_t wrM[N];
+          uint32_t dz[N];
+          if (DEPTH) {
+            // zx*lxN + zy*lyN mod 2^32 from the 16-bit halves of zx and zy, one
+            // dot product per half (lxN, lyN < 64: no product or sum overflows).
+            const uint32_t pl = shl16Add(dot2u(r[L::Z + 1], pixN, 0u), dot2u(r[L::Z], pixN, 0u));
+            uint64_t anyWrM = 0;
+  #pragma unroll
+            for (int i = 0; i < N; i++) {
+              dz[i] = r[L::ZS + i] + pl;
+              wrM[i] = covM[i] & __ballot(dz[i] < dep[i]);     // LESS (FineRaster.inl:697)
+              anyWrM |= wrM[i];
+            }
+            if (COUNT) {
+              pZKill += (uint64_t)__popcll(anyM & ~anyWrM);
+              pWritten += (uint64_t)__popcll(anyWrM);
+            }
+            if (anyWrM == 0) continue;
+          } else {
+  #pragma unroll
+            for (int i = 0; i < N; i++) { wrM[i] = covM[i]; dz[i] = 0; }
+            if (COUNT) pWritten += (uint64_t)__popcll(anyM);
+          }
+          bool cov[N], wr[N];
+          bool anyWr = false;
+  #pragma unroll
+          for (int i = 0; i < N; i++) {
+            cov[i] = __builtin_amdgcn_inverse_ballot_w64(covM[i]);
+            wr[i] = __builtin_amdgcn_inverse_ballot_w64(wrM[i]);
+            anyWr = anyWr | wr[i];
+          }
+          // Centroid (selectMSAACentroid): first covered sample in distance
+          // order, the pixel centre (15) when none or all are covered.
+          int code = 15;
+          if (!fullTile) {
+            bool all = true;
+  #pragma unroll
+            for (int k = N - 1; k >= 0; k--) code = cov[co.order[k]] ? co.order[k] : code;
+  #pragma unroll
+            for (int i = 0; i < N; i++) all = all & cov[i];
+            if (all) code = 15;
+          }
+
+          if (VIS) {
+            const uint32_t v = N == 1 ? r[L::SLOT] : (r[L::SLOT] | (uint32_t)code);
+  #pragma unroll
+            for (int i = 0; i < N; i++) {
+              if (DEPTH) dep[i] = wr[i] ? dz[i] : dep[i];
+              aux[i] = wr[i] ? v : aux[i];
+            }
+          } else {
+            uint32_t src = P.constColor;
+            const uint32_t tri = triWord & 0x7FFFFFFFu;
+            bool discard = false;
+            if (TEX) {
+              if (anyWr) {
+                const uint32_t* q = r + L::SH;
+                const Planes pl = {(int32_t)q[0], (int32_t)q[1], (int32_t)q[2], (int32_t)q[3], (int32_t)q[4],
+                                   (int32_t)q[5], (int32_t)q[6], (int32_t)q[7], (int32_t)q[8]};
+                discard = shadeTexPhong<S>(P, pl, q[9], q[10], q[11], tri, code, px, py, &src);
+              }
+            } else if (anyWr && gouraud) {
+              const uint32_t* q = r + L::SH;
+              const uint4 t1 = make_uint4(q[0], q[1], q[2], q[3]);
+              const uint4 t2 = make_uint4(q[4], q[5], q[6], q[7]);
+              const uint4 t3 = make_uint4(q[8], 0, 0, 0);
+              const float4 c0 = make_float4(__uint_as_float(q[9]), __uint_as_float(q[10]), __uint_as_float(q[11]),
+                                            __uint_as_float(q[12]));
+              const float4 c1 = make_float4(__uint_as_float(q[13]), __uint_as_float(q[14]), __uint_as_float(q[15]),
+                                            __uint_as_float(q[16]));
+              const float4 c2 = make_float4(__uint_as_float(q[17]), __uint_as_float(q[18]), __uint_as_float(q[19]),
+                                            __uint_as_float(q[20]));
+              src = shadeGouraud<S>(lerpOn, code, px, py, t1, t2, t3, c0, c1, c2);
+            }
+            // A discarded fragment writes neither depth nor colour.
+  #pragma unroll
+            for (int i = 0; i < N; i++) {
+              if (wr[i] && !discard) {
+                if (DEPTH) dep[i] = dz[i];
+                col[i] = blendPixel(blend, src, col[i]);
+                aux[i] = tri;
+              }
+            }
+          }
         }
         waveSync();
       }
-
-      // ---- walk the batch's triangles in submission order --------------------
-      for (uint32_t j = 0; j < steps; j++) {
-        // In-order shading: record j for the whole wave (uniform address).
-        // Deferred shading: this lane's quadrant's j-th record; lanes whose
-        // quadrant has no j-th record read record 31 and take no part.
-        uint64_t validM = ~0ull;
-        const uint32_t* r = rec + j * RS;
-        if (VIS) {
-          const uint32_t t = (r[Q0] >> quadShift) & 0xFFu;
-          validM = __ballot(t < (uint32_t)kChunk);
-          r = rec + min(t, (uint32_t)kChunk - 1) * RS;
-        }
-        const uint32_t* cur = r;
-        // |lx16|,|ly16| < 2^7 and |d| < 2^17: 24-bit multiplies are exact.
-        // In-order shading: a record covering the whole tile skips the
-        // per-sample edge tests (a scalar branch: the flag is uniform).
-        const uint32_t triWord = VIS ? 0u : (uint32_t)__builtin_amdgcn_readfirstlane((int)r[L::TRI]);
-        const bool fullTile = !VIS && (triWord >> 31) != 0;
-        // Per-sample coverage and write masks are kept as explicit 64-bit lane
-        // masks (scalar registers): a per-lane bool carried from one block to
-        // the next would be rebuilt from a vector register before each ballot.
-        uint64_t covM[N];
-        uint64_t anyM = 0;
-        if (fullTile) {
-#pragma unroll
-          for (int i = 0; i < N; i++) covM[i] = ~0ull;
-          anyM = ~0ull;
-        } else {
-          int32_t Lk[3];
-#pragma unroll
-          for (int k = 0; k < 3; k++)
-            Lk[k] = __mul24(lx16, (int32_t)cur[L::D + 2 * k]) + __mul24(ly16, (int32_t)cur[L::D + 2 * k + 1]);
-#pragma unroll
-          for (int i = 0; i < N; i++) {
-            covM[i] = __ballot(Lk[0] <= (int32_t)cur[L::R + i]) & __ballot(Lk[1] <= (int32_t)cur[L::R + N + i]) &
-                      __ballot(Lk[2] <= (int32_t)cur[L::R + 2 * N + i]) & validM;
-            anyM |= covM[i];
-          }
-        }
-        if (COUNT) {
-          pEmpty += anyM == 0;
-#pragma unroll
-          for (int i = 0; i < N; i++) pSamples += (uint64_t)__popcll(covM[i]);
-        }
-        if (anyM == 0) continue;
-        if (COUNT) frags += (uint64_t)__popcll(anyM);
-
-        uint64_t wrM[N];
-        uint32_t dz[N];
-        if (DEPTH) {
-          // zx*lxN + zy*lyN mod 2^32 from the 16-bit halves of zx and zy, one
-          // dot product per half (lxN, lyN < 64: no product or sum overflows).
-          const uint32_t pl = shl16Add(dot2u(r[L::Z + 1], pixN, 0u), dot2u(r[L::Z], pixN, 0u));
-          uint64_t anyWrM = 0;
-#pragma unroll
-          for (int i = 0; i < N; i++) {
-            dz[i] = r[L::ZS + i] + pl;
-            wrM[i] = covM[i] & __ballot(dz[i] < dep[i]);     // LESS (FineRaster.inl:697)
-            anyWrM |= wrM[i];
-          }
-          if (COUNT) {
-            pZKill += (uint64_t)__popcll(anyM & ~anyWrM);
-            pWritten += (uint64_t)__popcll(anyWrM);
-          }
-          if (anyWrM == 0) continue;
-        } else {
-#pragma unroll
-          for (int i = 0; i < N; i++) { wrM[i] = covM[i]; dz[i] = 0; }
-          if (COUNT) pWritten += (uint64_t)__popcll(anyM);
-        }
-        bool cov[N], wr[N];
-        bool anyWr = false;
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-          cov[i] = __builtin_amdgcn_inverse_ballot_w64(covM[i]);
-          wr[i] = __builtin_amdgcn_inverse_ballot_w64(wrM[i]);
-          anyWr = anyWr | wr[i];
-        }
-        // Centroid (selectMSAACentroid): first covered sample in distance
-        // order, the pixel centre (15) when none or all are covered.
-        int code = 15;
-        if (!fullTile) {
-          bool all = true;
-#pragma unroll
-          for (int k = N - 1; k >= 0; k--) code = cov[co.order[k]] ? co.order[k] : code;
-#pragma unroll
-          for (int i = 0; i < N; i++) all = all & cov[i];
-          if (all) code = 15;
-        }
-
-        if (VIS) {
-          const uint32_t v = N == 1 ? r[L::SLOT] : (r[L::SLOT] | (uint32_t)code);
-#pragma unroll
-          for (int i = 0; i < N; i++) {
-            if (DEPTH) dep[i] = wr[i] ? dz[i] : dep[i];
-            aux[i] = wr[i] ? v : aux[i];
-          }
-        } else {
-          uint32_t src = P.constColor;
-          const uint32_t tri = triWord & 0x7FFFFFFFu;
-          bool discard = false;
-          if (TEX) {
-            if (anyWr) {
-              const uint32_t* q = r + L::SH;
-              const Planes pl = {(int32_t)q[0], (int32_t)q[1], (int32_t)q[2], (int32_t)q[3], (int32_t)q[4],
-                                 (int32_t)q[5], (int32_t)q[6], (int32_t)q[7], (int32_t)q[8]};
-              discard = shadeTexPhong<S>(P, pl, q[9], q[10], q[11], tri, code, px, py, &src);
-            }
-          } else if (anyWr && gouraud) {
-            const uint32_t* q = r + L::SH;
-            const uint4 t1 = make_uint4(q[0], q[1], q[2], q[3]);
-            const uint4 t2 = make_uint4(q[4], q[5], q[6], q[7]);
-            const uint4 t3 = make_uint4(q[8], 0, 0, 0);
-            const float4 c0 = make_float4(__uint_as_float(q[9]), __uint_as_float(q[10]), __uint_as_float(q[11]),
-                                          __uint_as_float(q[12]));
-            const float4 c1 = make_float4(__uint_as_float(q[13]), __uint_as_float(q[14]), __uint_as_float(q[15]),
-                                          __uint_as_float(q[16]));
-            const float4 c2 = make_float4(__uint_as_float(q[17]), __uint_as_float(q[18]), __uint_as_float(q[19]),
-                                          __uint_as_float(q[20]));
-            src = shadeGouraud<S>(lerpOn, code, px, py, t1, t2, t3, c0, c1, c2);
-          }
-          // A discarded fragment writes neither depth nor colour.
-#pragma unroll
-          for (int i = 0; i < N; i++) {
-            if (wr[i] && !discard) {
-              if (DEPTH) dep[i] = dz[i];
-              col[i] = blendPixel(blend, src, col[i]);
-              aux[i] = tri;
-            }
-          }
-        }
-      }
-      waveSync();
     }
   }
 

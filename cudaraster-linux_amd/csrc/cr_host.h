@@ -116,7 +116,7 @@ struct DrawBuffers {
   uint32_t* tileStart = nullptr;
   uint32_t* tileCount = nullptr;
   uint32_t* tileList = nullptr;
-  uint32_t* tileSegNext = nullptr;    // maxTileEntries / 32 segment links
+  uint32_t* tileSegNext = nullptr;    // maxTileEntries / 32 segments: next and next-but-one links
   uint32_t* tileFrags = nullptr;      // counting draws only
   uint32_t* chunkSubtris = nullptr;   // counting draws only
   unsigned long long* prof = nullptr; // profiling draws only: 2 * kProfCount counters

@@ -67,7 +67,7 @@ size_t fixedBytes(int64_t numTris, const crhost::Geometry& g, int32_t numChunks,
 size_t queueBytes(int64_t numTris, int64_t subtris, int64_t bin, int64_t tile) {
   return alignUp((size_t)subtris * sizeof(Header)) + alignUp((size_t)subtris * sizeof(TriData)) +
          alignUp((size_t)std::max<int64_t>(subtris - numTris, 1) * 4) + alignUp((size_t)bin * 16) +
-         alignUp((size_t)tile * 4) + alignUp((size_t)std::max<int64_t>(tile >> kTileSegLog2, 1) * 4);
+         alignUp((size_t)tile * 4) + alignUp((size_t)std::max<int64_t>(tile >> kTileSegLog2, 1) * 8);
 }
 
 // Lays the draw out in the array at `base`.
@@ -101,7 +101,7 @@ crhost::DrawBuffers layout(uint8_t* base, int64_t numTris, const crhost::Geometr
   B.subtriTri = reinterpret_cast<uint32_t*>(take((size_t)std::max<int64_t>(subtris - numTris, 1) * 4));
   B.binList = reinterpret_cast<uint4*>(take((size_t)bin * 16));
   B.tileList = reinterpret_cast<uint32_t*>(take((size_t)tile * 4));
-  B.tileSegNext = reinterpret_cast<uint32_t*>(take((size_t)std::max<int64_t>(tile >> kTileSegLog2, 1) * 4));
+  B.tileSegNext = reinterpret_cast<uint32_t*>(take((size_t)std::max<int64_t>(tile >> kTileSegLog2, 1) * 8));
   return B;
 }
 
