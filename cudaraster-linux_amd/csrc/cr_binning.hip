@@ -198,15 +198,41 @@ __device__ __forceinline__ TileTest getBig(const BigEntry* e, uint32_t& slot, ui
 // emitMask[wave][tile] for every tile the entry's edges reach.  Large boxes
 // of a scattered wave go to the workgroup's queue `big` (while it has room;
 // *queued tells the lane) and are walked after the next barrier by
-// whichever waves are free, spreading them over the workgroup.  Returns the
+// whichever waves are free, spreading them over the workgroup.  A large box
+// that reaches every tile of a complete bin (*whole) sets no mask bits: its
+// wave's word fullBits[wave] holds it for every tile, and it is listed in
+// `whole` (its (wave << 6 | lane) and slot) for the writes.  Returns the
 // CoarseCase the wave took.
 __device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const int bl,
                                             uint64_t (*emitMask)[kBinTiles], BigEntry* big,
                                             uint32_t* bigCount, uint32_t bigCap, uint32_t slot,
-                                            bool& queued) {
+                                            bool& queued, bool binComplete, uint64_t* fullBits,
+                                            uint2* wholeList, uint32_t* wholeCount, bool& whole) {
   queued = false;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int w = valid ? r.x1 - r.x0 + 1 : 0, h = valid ? r.y1 - r.y0 + 1 : 0;
+  int w = valid ? r.x1 - r.x0 + 1 : 0, h = valid ? r.y1 - r.y0 + 1 : 0;
+  // Every tile of the bin hit: the box is the bin and the tile test's left
+  // side, linear in the tile offsets, stays within each threshold at the
+  // box's far corners (|15 * s| < 2^28).
+  const int side = 1 << bl;
+  whole = binComplete && valid && w * h > kCoopArea && r.x0 == 0 && r.y0 == 0 && w == side && h == side;
+  uint64_t wholeM = __ballot(whole);
+  if (wholeM != 0) {                     // (the edge tests only where some box is the bin)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      whole = whole && max(0, (side - 1) * r.sx[k]) + max(0, (side - 1) * r.sy[k]) <= r.G[k];
+    wholeM = __ballot(whole);
+  }
+  if (wholeM != 0) {
+    uint32_t base = 0;
+    if (lane == 0) {
+      fullBits[wave] = wholeM;
+      base = atomicAdd(wholeCount, (uint32_t)__popcll(wholeM));
+    }
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+    if (whole) wholeList[base + rankBelow(wholeM)] = make_uint2(((uint32_t)wave << 6) | (uint32_t)lane, slot);
+    if (whole) w = h = 0;
+  }
   const bool has = valid && w > 0 && h > 0;
   const uint64_t bit = 1ull << lane;
   const bool allSmall = __ballot(has && (w > 2 || h > 2)) == 0;
@@ -303,6 +329,12 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   constexpr uint32_t kBigCap = CT / 4;
   __shared__ BigEntry bigQ[kBigCap];
   __shared__ uint32_t sBigCount;
+  // Entries of the round that reach every tile of the bin: per wave, their
+  // lanes (or-ed into every tile's mask once per round); the list of their
+  // (wave << 6 | lane) and slots for the writes.
+  __shared__ uint64_t sFull[kCoarseWaves];
+  __shared__ uint2 wholeList[CT];
+  __shared__ uint32_t sWholeCount;
   // Profiling draws: entries, rounds, emits, tiles with emits, entries per case.
   __shared__ uint32_t sProf[8];
 
@@ -311,6 +343,8 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   if (prof && threadIdx.x < 8) sProf[threadIdx.x] = 0;
   const int bl = P.binLog2, tpb = 1 << (2 * bl);   // tiles per bin side log2, tiles per bin
   const int bx = bin % P.widthBins, by = bin / P.widthBins;
+  // Every tile of the bin's square lies in the viewport.
+  const bool binComplete = ((bx + 1) << bl) <= P.widthTiles && ((by + 1) << bl) <= P.heightTiles;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint32_t* bCnt = P.binCnt + (size_t)bin * P.numChunks;
   const uint32_t* bOff = P.binOff + (size_t)bin * P.numChunks;
@@ -362,7 +396,11 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
     for (uint32_t r0 = 0; r0 < segTotal; r0 += kCoarseThreads) {
       for (int i = threadIdx.x; i < kCoarseWaves * kBinTiles; i += kCoarseThreads)
         emitMask[i / kBinTiles][i % kBinTiles] = 0ull;
-      if (threadIdx.x == 0) sBigCount = 0;
+      if (threadIdx.x == 0) {
+        sBigCount = 0;
+        sWholeCount = 0;
+      }
+      if (threadIdx.x < (unsigned)kCoarseWaves) sFull[threadIdx.x] = 0ull;
       __syncthreads();
       const uint32_t g = r0 + threadIdx.x;
       bool valid = g < segTotal;
@@ -375,8 +413,9 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
         valid = slot < (uint32_t)P.maxSubtris;
         if (valid) r = tileTestOf(P, make_uint4(e.y, e.z, e.w, 0u), bx, by);
       }
-      bool queued;
-      const int wcase = coarseRecord(r, valid, bl, emitMask, bigQ, &sBigCount, kBigCap, slot, queued);
+      bool queued, whole;
+      const int wcase = coarseRecord(r, valid, bl, emitMask, bigQ, &sBigCount, kBigCap, slot, queued, binComplete,
+                                     sFull, wholeList, &sWholeCount, whole);
       if (g + kCoarseThreads < segTotal) eNext = entryAt(g + kCoarseThreads);
       if (prof) {
         const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
@@ -390,6 +429,17 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
       // The queued large boxes, one per wave at a time: a bit per tile hit
       // in the mask of the wave the entry came from.
       const uint32_t nBig = min(sBigCount, kBigCap);
+      const uint32_t nWhole = sWholeCount;
+      // Entries that reach every tile: their wave's bits into every tile's
+      // mask, one word per thread at a time (atomically: the large boxes
+      // below may set bits in the same words).
+      if (nWhole > 0) {
+        for (int i = threadIdx.x; i < kCoarseWaves * tpb; i += kCoarseThreads) {
+          const int q = i >> (2 * bl), t = i & (tpb - 1);
+          const uint64_t f = sFull[q];
+          if (f) atomicOr((unsigned long long*)&emitMask[q][t], (unsigned long long)f);
+        }
+      }
       if (nBig > 0) {
         for (uint32_t i = (uint32_t)wave; i < nBig; i += kCoarseWaves) {
           uint32_t sj, id;
@@ -400,8 +450,8 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
             atomicOr((unsigned long long*)&mrow[x + (y << bl)], (unsigned long long)bj);
           });
         }
-        __syncthreads();
       }
+      if (nBig > 0 || nWhole > 0) __syncthreads();
       // Rank: each wave's entries of tile t follow the earlier waves' and
       // the tile's list so far; the segments the round needs past the
       // tile's last, partly filled one.
@@ -489,7 +539,7 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
         const uint32_t sg = k < q.x ? q.y : q.z + (k - q.x);
         if (sg < segCap) P.tileList[((size_t)sg << kTileSegLog2) + (e & (kTileSeg - 1))] = sl;
       };
-      const bool hasBox = valid && r.x0 <= r.x1 && r.y0 <= r.y1;
+      const bool hasBox = valid && !whole && r.x0 <= r.x1 && r.y0 <= r.y1;
       const int boxArea = hasBox ? (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1) : 0;
       const bool bigBox = boxArea > kCoopArea;
       if (hasBox && !bigBox) {
@@ -540,6 +590,20 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
             const uint4 q = tRound[ti];
             if (m & bj) put(q, pre + (uint32_t)__popcll(m & ltj), sj);
           }
+        }
+      }
+      // Entries that reach every tile: one wave step per entry and 64 tiles
+      // (a complete bin of at least 8 x 8 tiles), shared out over the waves.
+      if (nWhole > 0) {
+        const int stepsLog2 = 2 * bl - 6;
+        for (uint32_t i = (uint32_t)wave; i < (nWhole << stepsLog2); i += kCoarseWaves) {
+          const uint2 we = wholeList[i >> stepsLog2];
+          const int ti = (int)((i & ((1u << stepsLog2) - 1)) << 6) + lane;
+          const uint32_t q = we.x >> 6;
+          const uint64_t ltj = (1ull << (we.x & 63)) - 1;
+          const uint64_t m = emitMask[q][ti];
+          const uint32_t pre = emitPre[q][ti];
+          put(tRound[ti], pre + (uint32_t)__popcll(m & ltj), we.y);
         }
       }
       __syncthreads();
