@@ -200,14 +200,14 @@ __device__ __forceinline__ TileTest getBig(const BigEntry* e, uint32_t& slot, ui
 // *queued tells the lane) and are walked after the next barrier by
 // whichever waves are free, spreading them over the workgroup.  A large box
 // that reaches every tile of a complete bin (*whole) sets no mask bits: its
-// wave's word fullBits[wave] holds it for every tile, and it is listed in
-// `whole` (its (wave << 6 | lane) and slot) for the writes.  Returns the
-// CoarseCase the wave took.
+// wave's word fullBits[wave] holds it for every tile, and its slot goes to
+// wholeSlot[wave * 64 + lane] for the writes.  Returns the CoarseCase the
+// wave took.
 __device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const int bl,
                                             uint64_t (*emitMask)[kBinTiles], BigEntry* big,
                                             uint32_t* bigCount, uint32_t bigCap, uint32_t slot,
                                             bool& queued, bool binComplete, uint64_t* fullBits,
-                                            uint2* wholeList, uint32_t* wholeCount, bool& whole) {
+                                            uint32_t* wholeSlot, uint32_t* wholeCount, bool& whole) {
   queued = false;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   int w = valid ? r.x1 - r.x0 + 1 : 0, h = valid ? r.y1 - r.y0 + 1 : 0;
@@ -224,14 +224,14 @@ __device__ __forceinline__ int coarseRecord(const TileTest& r, bool valid, const
     wholeM = __ballot(whole);
   }
   if (wholeM != 0) {
-    uint32_t base = 0;
     if (lane == 0) {
       fullBits[wave] = wholeM;
-      base = atomicAdd(wholeCount, (uint32_t)__popcll(wholeM));
+      atomicAdd(wholeCount, (uint32_t)__popcll(wholeM));
     }
-    base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
-    if (whole) wholeList[base + rankBelow(wholeM)] = make_uint2(((uint32_t)wave << 6) | (uint32_t)lane, slot);
-    if (whole) w = h = 0;
+    if (whole) {
+      wholeSlot[(wave << 6) + lane] = slot;
+      w = h = 0;
+    }
   }
   const bool has = valid && w > 0 && h > 0;
   const uint64_t bit = 1ull << lane;
@@ -330,10 +330,10 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
   __shared__ BigEntry bigQ[kBigCap];
   __shared__ uint32_t sBigCount;
   // Entries of the round that reach every tile of the bin: per wave, their
-  // lanes (or-ed into every tile's mask once per round); the list of their
-  // (wave << 6 | lane) and slots for the writes.
+  // lanes (or-ed into every tile's mask once per round) and, per lane, their
+  // slots for the writes.
   __shared__ uint64_t sFull[kCoarseWaves];
-  __shared__ uint2 wholeList[CT];
+  __shared__ uint32_t wholeSlot[CT];
   __shared__ uint32_t sWholeCount;
   // Profiling draws: entries, rounds, emits, tiles with emits, entries per case.
   __shared__ uint32_t sProf[8];
@@ -415,7 +415,7 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
       }
       bool queued, whole;
       const int wcase = coarseRecord(r, valid, bl, emitMask, bigQ, &sBigCount, kBigCap, slot, queued, binComplete,
-                                     sFull, wholeList, &sWholeCount, whole);
+                                     sFull, wholeSlot, &sWholeCount, whole);
       if (g + kCoarseThreads < segTotal) eNext = entryAt(g + kCoarseThreads);
       if (prof) {
         const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
@@ -592,18 +592,24 @@ __device__ __forceinline__ void coarseBody(const Params& P, const int bin) {
           }
         }
       }
-      // Entries that reach every tile: one wave step per entry and 64 tiles
-      // (a complete bin of at least 8 x 8 tiles), shared out over the waves.
+      // Entries that reach every tile: per (source wave, 64 tiles of a
+      // complete bin of at least 8 x 8 tiles), one lane per tile reads the
+      // tile's mask, rank base and segment map once and writes every such
+      // entry of that wave; the pairs are shared out over the waves.
       if (nWhole > 0) {
-        const int stepsLog2 = 2 * bl - 6;
-        for (uint32_t i = (uint32_t)wave; i < (nWhole << stepsLog2); i += kCoarseWaves) {
-          const uint2 we = wholeList[i >> stepsLog2];
-          const int ti = (int)((i & ((1u << stepsLog2) - 1)) << 6) + lane;
-          const uint32_t q = we.x >> 6;
-          const uint64_t ltj = (1ull << (we.x & 63)) - 1;
+        const int chunksLog2 = 2 * bl - 6;
+        for (int i = wave; i < (kCoarseWaves << chunksLog2); i += kCoarseWaves) {
+          const int q = i >> chunksLog2;
+          const uint64_t f = sFull[q];
+          if (f == 0) continue;
+          const int ti = ((i & ((1 << chunksLog2) - 1)) << 6) + lane;
           const uint64_t m = emitMask[q][ti];
           const uint32_t pre = emitPre[q][ti];
-          put(tRound[ti], pre + (uint32_t)__popcll(m & ltj), we.y);
+          const uint4 tq = tRound[ti];
+          for (uint64_t b = f; b != 0; b &= b - 1) {
+            const int j = (int)__builtin_ctzll(b);
+            put(tq, pre + (uint32_t)__popcll(m & ((1ull << j) - 1)), wholeSlot[(q << 6) + j]);
+          }
         }
       }
       __syncthreads();
