@@ -398,7 +398,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
           // the smallest threshold.  Exactly the per-sample test's outcome.
           bool full = true;
           if (VIS) quads = 15u;
-  #pragma unroll
+#pragma unroll
           for (int k = 0; k < 3; k++) {
             const int kn = (k + 1) % 3;
             const int32_t dx = ax[kn] - ax[k], dy = ay[kn] - ay[k];
@@ -418,7 +418,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
               ck = (int32_t)cc;
             }
             int32_t minR = INT32_MAX, maxR = INT32_MIN;
-  #pragma unroll
+#pragma unroll
             for (int i = 0; i < N; i++) {
               const int32_t ox = (2 * msaaPattern(S, i) + 1) << (3 - S);
               const int32_t oy = (2 * i + 1) << (3 - S);
@@ -496,7 +496,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
         uint32_t steps = nb;
         if (VIS) {
           steps = 0;
-  #pragma unroll
+#pragma unroll
           for (int q = 0; q < 4; q++) {
             const uint64_t m = __ballot((quads >> q) & 1u);
             steps = max(steps, (uint32_t)__popcll(m));
@@ -530,15 +530,15 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
           uint64_t covM[N];
           uint64_t anyM = 0;
           if (fullTile) {
-  #pragma unroll
+#pragma unroll
             for (int i = 0; i < N; i++) covM[i] = ~0ull;
             anyM = ~0ull;
           } else {
             int32_t Lk[3];
-  #pragma unroll
+#pragma unroll
             for (int k = 0; k < 3; k++)
               Lk[k] = __mul24(lx16, (int32_t)cur[L::D + 2 * k]) + __mul24(ly16, (int32_t)cur[L::D + 2 * k + 1]);
-  #pragma unroll
+#pragma unroll
             for (int i = 0; i < N; i++) {
               covM[i] = __ballot(Lk[0] <= (int32_t)cur[L::R + i]) & __ballot(Lk[1] <= (int32_t)cur[L::R + N + i]) &
                         __ballot(Lk[2] <= (int32_t)cur[L::R + 2 * N + i]) & validM;
@@ -547,7 +547,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
           }
           if (COUNT) {
             pEmpty += anyM == 0;
-  #pragma unroll
+#pragma unroll
             for (int i = 0; i < N; i++) pSamples += (uint64_t)__popcll(covM[i]);
           }
           if (anyM == 0) continue;
@@ -560,7 +560,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
             // dot product per half (lxN, lyN < 64: no product or sum overflows).
             const uint32_t pl = shl16Add(dot2u(r[L::Z + 1], pixN, 0u), dot2u(r[L::Z], pixN, 0u));
             uint64_t anyWrM = 0;
-  #pragma unroll
+#pragma unroll
             for (int i = 0; i < N; i++) {
               dz[i] = r[L::ZS + i] + pl;
               wrM[i] = covM[i] & __ballot(dz[i] < dep[i]);     // LESS (FineRaster.inl:697)
@@ -572,13 +572,13 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
             }
             if (anyWrM == 0) continue;
           } else {
-  #pragma unroll
+#pragma unroll
             for (int i = 0; i < N; i++) { wrM[i] = covM[i]; dz[i] = 0; }
             if (COUNT) pWritten += (uint64_t)__popcll(anyM);
           }
           bool cov[N], wr[N];
           bool anyWr = false;
-  #pragma unroll
+#pragma unroll
           for (int i = 0; i < N; i++) {
             cov[i] = __builtin_amdgcn_inverse_ballot_w64(covM[i]);
             wr[i] = __builtin_amdgcn_inverse_ballot_w64(wrM[i]);
@@ -589,16 +589,16 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
           int code = 15;
           if (!fullTile) {
             bool all = true;
-  #pragma unroll
+#pragma unroll
             for (int k = N - 1; k >= 0; k--) code = cov[co.order[k]] ? co.order[k] : code;
-  #pragma unroll
+#pragma unroll
             for (int i = 0; i < N; i++) all = all & cov[i];
             if (all) code = 15;
           }
 
           if (VIS) {
             const uint32_t v = N == 1 ? r[L::SLOT] : (r[L::SLOT] | (uint32_t)code);
-  #pragma unroll
+#pragma unroll
             for (int i = 0; i < N; i++) {
               if (DEPTH) dep[i] = wr[i] ? dz[i] : dep[i];
               aux[i] = wr[i] ? v : aux[i];
@@ -628,7 +628,7 @@ __device__ __forceinline__ void fineBody(const Params& P, const int group, uint3
               src = shadeGouraud<S>(lerpOn, code, px, py, t1, t2, t3, c0, c1, c2);
             }
             // A discarded fragment writes neither depth nor colour.
-  #pragma unroll
+#pragma unroll
             for (int i = 0; i < N; i++) {
               if (wr[i] && !discard) {
                 if (DEPTH) dep[i] = dz[i];
