@@ -191,6 +191,43 @@ def test_quadrant_boundaries_depth_only(oracle_lib):
     assert_parity(render_gpu(s), oracle_lib.render(s), s.name)
 
 
+def deep_list_scene(samples_log2, flags, blend, seed=0):
+    """Tiny triangles packed into a 32-tile region (4 x 8 tiles) of a 256 x 256
+    view: every tile's list grows by about one 32-entry segment per coarse
+    round (1024 bin entries), so its lists are long chains whose links to the
+    segment after next are filled in a round or two later."""
+    from cudaraster import scenes
+    rng = np.random.default_rng(seed)
+    width = height = 256
+    n = 20000
+    cx = 96.0 + 32.0 * rng.random(n)
+    cy = 64.0 + 64.0 * rng.random(n)
+    sx, sy = scenes._triangle_shapes(rng, n, 0.4 + 2.6 * rng.random(n))
+    w = np.repeat(1.0 + 9.0 * rng.random((n, 1)), 3, axis=1)
+    clip = scenes._to_clip(cx[:, None] + sx, cy[:, None] + sy, w, width, height)
+    col = rng.random((n, 3, 4))
+    col[..., 3] = 0.2 + 0.6 * rng.random((n, 3))
+    verts = np.concatenate([clip, col], axis=-1).reshape(n * 3, 8).astype(np.float32)
+    idx = np.arange(n * 3, dtype=np.int32).reshape(n, 3)
+    return scenes.Scene(f"deep lists s={samples_log2} f={flags} b={blend}", width, height, samples_log2, flags,
+                        blend, scenes.SHADER_GOURAUD, np.ascontiguousarray(verts), idx)
+
+
+@pytest.mark.parametrize("samples_log2,depth,blend", [(3, False, 1), (2, True, 1), (0, False, 2), (2, False, 0)])
+def test_deep_tile_lists(oracle_lib, samples_log2, depth, blend):
+    """Long tile lists built one segment per round: the in-order copies
+    (src-over, additive) read them two segments per step through the links
+    to the segment after next; the deferred-shading copy (replace) one."""
+    _torch()
+    from cudaraster import scenes
+    flags = scenes.FLAG_LERP | (scenes.FLAG_DEPTH if depth else 0)
+    s = deep_list_scene(samples_log2, flags, blend, seed=60 + samples_log2)
+    ref = oracle_lib.render(s)
+    got = render_gpu(s)
+    assert got["counters"]["tile_list_entries"] > 8 * 32 * 32      # > 8 segments per tile on average
+    assert_parity(got, ref, s.name)
+
+
 def test_zero_triangles(oracle_lib):
     """A draw without triangles still applies the deferred clear."""
     _torch()
