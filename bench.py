@@ -244,10 +244,11 @@ def multi_view_ratio(reps=5, triid=True):
 
 def differentiable_views(reps=3):
     """Differentiable workload: 32 views at 512x512, 200k triangles each, 8
-    FP32 channels; per view one exact visibility draw and the interpolation
-    forward, an L2 loss against the views' targets and the backward pass to
-    every view's positions and attributes (one torch autograd call over the
-    HIP kernels of all views: DiffRasterizer.render_views)."""
+    FP32 channels; the exact visibility of all views in one batched draw,
+    per view the interpolation forward, an L2 loss against the views'
+    targets and the backward pass to every view's positions and attributes
+    (one torch autograd call over the HIP kernels of all views:
+    DiffRasterizer.render_views)."""
     import torch
     from cudaraster import scenes
     from cudaraster.diff import DiffRasterizer
@@ -274,7 +275,8 @@ def differentiable_views(reps=3):
     dr.close()
     return {"views": len(views), "triangles_per_view": int(views[0].tris.shape[0]), "resolution": 512,
             "channels": int(views[0].attr.shape[1]), "ms_per_pass": t * 1e3, "views_per_s": len(views) / t,
-            "steps": "per view: visibility draw + interpolate forward; L2 loss; backward to positions and attributes",
+            "steps": "batched visibility draw of the views; per view interpolate forward; L2 loss; "
+                     "backward to positions and attributes",
             "api": "DiffRasterizer.render_views (one autograd call for the 32 views)"}
 
 

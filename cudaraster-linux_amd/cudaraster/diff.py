@@ -13,8 +13,8 @@ from __future__ import annotations
 
 import ctypes
 
-from . import (CR_ERR_OVERFLOW, CR_OK, CrError, CudaRaster, CudaSurface, FORMAT_DEPTH32, FORMAT_RGBA8,
-               FORMAT_TRIANGLE_ID, _check, load, pipe_desc, SHADER_CONSTANT, BLEND_REPLACE, RENDER_DEPTH)
+from . import (CR_ERR_OVERFLOW, CR_OK, Batch, CrError, CudaRaster, CudaSurface, FORMAT_DEPTH32, FORMAT_RGBA8,
+               FORMAT_TRIANGLE_ID, View, _check, load, pipe_desc, SHADER_CONSTANT, BLEND_REPLACE, RENDER_DEPTH)
 
 
 def _stream():
@@ -78,9 +78,12 @@ class InterpolateViews:
                 H, W, C = raster.height, raster.width, attr[0].shape[1]
                 out = torch.empty((n, H, W, C), dtype=torch.float32, device=pos[0].device)
                 ids = torch.empty((n, H, W), dtype=torch.int32, device=pos[0].device)
+                # Visibility of every view in one batched draw, then each
+                # view's interpolation from its own triangle-id surface.
+                surfaces = raster._draw_views(pos, tris)
                 for i in range(n):
-                    raster._draw(pos[i], tris[i])
-                    raster._interpolate(pos[i].detach(), attr[i].detach(), tris[i], out[i], ids[i])
+                    raster._interpolate(pos[i].detach(), attr[i].detach(), tris[i], out[i], ids[i],
+                                        surface=surfaces[i])
                 ctx.save_for_backward(ids, *tensors)
                 ctx.raster, ctx.n = raster, n
                 ctx.mark_non_differentiable(ids)
@@ -122,6 +125,66 @@ class DiffRasterizer:
         self.cr.setTriangleIdSurface(self.ids)
         # Positions only (16-byte vertices), depth test on, constant colour.
         self.cr.setPixelPipe(pipe_desc(16, SHADER_CONSTANT, BLEND_REPLACE, 0, RENDER_DEPTH))
+        # render_views: the batched visibility draw of the last view sizes
+        # (signature, cr_batch, per-view surfaces, staging buffers).
+        self._views = None
+
+    def _draw_views(self, pos, tris):
+        """Exact visibility of several views in one batched draw (cr_batch:
+        one launch per stage over all views, the same images as drawing them
+        in turn); returns each view's triangle-id surface.  The batch, its
+        per-view surfaces and staging buffers for the positions and indices
+        are kept for the next call with the same view sizes."""
+        import torch
+        for p, t in zip(pos, tris):
+            if not (p.is_cuda and p.dtype == torch.float32 and p.dim() == 2 and p.shape[1] == 4):
+                raise CrError(1, "pos must be a float32 [V, 4] device tensor")
+            if not (t.is_cuda and t.dtype == torch.int32 and t.dim() == 2 and t.shape[1] == 3):
+                raise CrError(1, "tris must be an int32 [T, 3] device tensor")
+        key = tuple((int(p.shape[0]), int(t.shape[0]), p.device.index) for p, t in zip(pos, tris))
+        if self._views is None or self._views[0] != key:
+            self._close_views()
+            vbuf = [torch.empty((k[0], 4), dtype=torch.float32, device=p.device) for k, p in zip(key, pos)]
+            ibuf = [torch.empty((k[1], 3), dtype=torch.int32, device=p.device) for k, p in zip(key, pos)]
+            surfaces, views = [], []
+            pipe = pipe_desc(16, SHADER_CONSTANT, BLEND_REPLACE, 0, RENDER_DEPTH)
+            try:
+                for vb, ib in zip(vbuf, ibuf):
+                    trio = (CudaSurface((self.width, self.height), FORMAT_RGBA8, 1),
+                            CudaSurface((self.width, self.height), FORMAT_DEPTH32, 1),
+                            CudaSurface((self.width, self.height), FORMAT_TRIANGLE_ID, 1))
+                    surfaces.append(trio)
+                    v = View()
+                    v.color, v.depth, v.tri_id = (s.handle.value for s in trio)
+                    v.vertices, v.num_vertices = (vb.data_ptr() if vb.numel() else None), vb.shape[0]
+                    v.indices, v.num_tris = (ib.data_ptr() if ib.numel() else None), ib.shape[0]
+                    v.deferred_clear = 1
+                    v.pipe = pipe
+                    v.clear_color[:] = [0.0, 0.0, 0.0, 0.0]
+                    v.clear_depth = 1.0
+                    views.append(v)
+                batch = Batch(self.cr, views)
+            except Exception:
+                for trio in surfaces:
+                    for sf in trio:
+                        sf.close()
+                raise
+            self._views = (key, batch, surfaces, vbuf, ibuf)
+        _, batch, surfaces, vbuf, ibuf = self._views
+        for vb, ib, p, t in zip(vbuf, ibuf, pos, tris):
+            vb.copy_(p.detach())
+            ib.copy_(t)
+        batch.draw(torch.cuda.current_stream())
+        return [trio[2] for trio in surfaces]
+
+    def _close_views(self):
+        if self._views is not None:
+            _, batch, surfaces, _, _ = self._views
+            self._views = None
+            batch.close()
+            for trio in surfaces:
+                for sf in trio:
+                    sf.close()
 
     def _draw(self, pos, tris):
         """Exact visibility of pos/tris into the triangle-id surface."""
@@ -199,13 +262,16 @@ class DiffRasterizer:
                 tuple(tri_image.shape) == (self.height, self.width)):
             raise CrError(1, "tri_image must be a contiguous int32 [H, W] device tensor at the rasterizer's resolution")
 
-    def _interpolate(self, pos, attr, tris, out, tri_image, channels=None):
+    def _interpolate(self, pos, attr, tris, out, tri_image, channels=None, surface=None):
+        """Interpolates from the triangle ids of `surface` (default: this
+        rasterizer's own, from the last _draw) and copies them to tri_image."""
         C = attr.shape[1] if channels is None else channels
         self._check_inputs(pos, attr, tris, tri_image, C)
         hw = self.height * self.width
         if C > 0 and (out.numel() != hw * C or not out.is_contiguous() or not out.is_cuda):
             raise CrError(1, "interpolation output buffers do not match the rasterizer's resolution")
-        _check(load().cr_interpolate(self.cr.ctx, self.ids.handle, pos.data_ptr(), pos.shape[0],
+        ids = self.ids if surface is None else surface
+        _check(load().cr_interpolate(self.cr.ctx, ids.handle, pos.data_ptr(), pos.shape[0],
                                      attr.data_ptr() if attr is not None else None, C, tris.data_ptr(),
                                      tris.shape[0], out.data_ptr(), tri_image.data_ptr(), _stream()), self.cr.ctx)
 
@@ -250,6 +316,7 @@ class DiffRasterizer:
         try:
             self.sync()
         finally:
+            self._close_views()
             self.cr.close()
             for s in (self.color, self.depth, self.ids):
                 s.close()
