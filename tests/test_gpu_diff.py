@@ -194,6 +194,35 @@ def test_render_views_full_size_32_views(oracle_lib):
             assert np.abs(g - r).max() <= 1e-6 * np.abs(r).max() + 1e-30, f"view {k} {name} gradient"
 
 
+def test_render_views_mixed_with_single_views(oracle_lib):
+    """render_views draws its views' visibility as one batch kept for the
+    next call with the same view sizes: single-view draws between such calls
+    keep their own state, a call with other view sizes rebuilds the batch,
+    and every view's ids equal the oracle's throughout."""
+    torch = _torch()
+    from cudaraster import scenes
+    from cudaraster.diff import DiffRasterizer
+    views_a = [scenes.diff_view(num_tris=3000, width=96, height=80, channels=3, seed=70 + k) for k in range(3)]
+    views_b = [scenes.diff_view(num_tris=1500 + 700 * k, width=96, height=80, channels=3, seed=80 + k)
+               for k in range(2)]
+    ref = {id(v): oracle_ids(oracle_lib, v) for v in views_a + views_b}
+    dr = DiffRasterizer(96, 80)
+    try:
+        def tensors(vs):
+            return ([torch.from_numpy(v.pos).cuda() for v in vs], [torch.from_numpy(v.attr).cuda() for v in vs],
+                    [torch.from_numpy(v.tris).cuda() for v in vs])
+        for vs in (views_a, views_b, views_a, views_a):
+            pos, attr, tris = tensors(vs)
+            _, ids = dr.render_views(pos, attr, tris)
+            for k, v in enumerate(vs):
+                assert np.array_equal(ids[k].cpu().numpy(), ref[id(v)]), "render_views ids differ from the oracle"
+            # A single view drawn between batched calls.
+            single = dr.rasterize(pos[-1], tris[-1])
+            assert np.array_equal(single.cpu().numpy(), ref[id(vs[-1])]), "single view after render_views"
+    finally:
+        dr.close()
+
+
 def test_async_visibility_overflow_is_reported_then_recovers():
     """After a small sizing view, a much larger view overflows the reserved
     queues in its asynchronous draw: sync() raises instead of returning
