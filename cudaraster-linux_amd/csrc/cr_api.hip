@@ -724,7 +724,9 @@ cr_status cr_interpolate(cr_context* ctx, const cr_surface* triId, const float* 
   if (!ctx || !triId || !out) return fail(ctx, CR_ERR_INVALID, "cr_interpolate: NULL argument");
   if (triId->format != CR_FORMAT_TRIANGLE_ID || triId->samples != 1)
     return fail(ctx, CR_ERR_INVALID, "cr_interpolate: needs a single-sample CR_FORMAT_TRIANGLE_ID surface");
-  if (numChannels < 0 || numVertices < 0 || numTris < 0 || (numChannels > 0 && !attr) ||
+  // (Without vertices the kernels read no position or attribute: an empty
+  // view may pass NULL for them.)
+  if (numChannels < 0 || numVertices < 0 || numTris < 0 || (numChannels > 0 && numVertices > 0 && !attr) ||
       (numTris > 0 && (!pos || !tris)) || ((uintptr_t)pos & 15) || ((uintptr_t)tris & 3))
     return fail(ctx, CR_ERR_INVALID, "cr_interpolate: bad buffers or counts");
   InterpGeom G{triId->width, triId->height, triId->textureW, numChannels, numTris, numVertices};
@@ -739,8 +741,9 @@ cr_status cr_interpolate_ids(cr_context* ctx, const int32_t* triImage, int32_t w
                              const int32_t* tris, int32_t numTris, float* out, void* stream) {
   if (!ctx || !triImage || !out) return fail(ctx, CR_ERR_INVALID, "cr_interpolate_ids: NULL argument");
   if (width <= 0 || height <= 0 || width > kMaxViewport || height > kMaxViewport || numChannels < 0 ||
-      numVertices < 0 || numTris < 0 || (numChannels > 0 && !attr) || (numTris > 0 && (!pos || !tris)) ||
-      ((uintptr_t)pos & 15) || ((uintptr_t)tris & 3) || ((uintptr_t)triImage & 3))
+      numVertices < 0 || numTris < 0 || (numChannels > 0 && numVertices > 0 && !attr) ||
+      (numTris > 0 && (!pos || !tris)) || ((uintptr_t)pos & 15) || ((uintptr_t)tris & 3) ||
+      ((uintptr_t)triImage & 3))
     return fail(ctx, CR_ERR_INVALID, "cr_interpolate_ids: bad sizes or buffers");
   // An int32 id image is a single-sample id surface with a pitch of `width`
   // texels (-1 reads as 0xFFFFFFFF: no triangle).
@@ -758,7 +761,8 @@ cr_status interpolateBackward(cr_context* ctx, const char* name, bool determinis
                               int32_t width, int32_t height, const float* pos, int64_t numVertices,
                               const float* attr, int32_t numChannels, const int32_t* tris, int32_t numTris,
                               const float* gradOut, float* gradPos, float* gradAttr, void* stream) {
-  if (!ctx || !triImage || !gradOut || !gradPos || (numChannels > 0 && (!attr || !gradAttr)))
+  // (Without vertices there is no gradient to write and no attribute to read.)
+  if (!ctx || !triImage || !gradOut || (numVertices > 0 && (!gradPos || (numChannels > 0 && (!attr || !gradAttr)))))
     return fail(ctx, CR_ERR_INVALID, "%s: NULL argument", name);
   if (width <= 0 || height <= 0 || width > kMaxViewport || height > kMaxViewport || numChannels < 0 ||
       numVertices < 0 || numTris < 0 || (numTris > 0 && (!pos || !tris)) || ((uintptr_t)pos & 15) ||

@@ -411,6 +411,8 @@ cr_status cr_batch_get_view_counters(const cr_batch* batch, int32_t view, cr_cou
  *
  * pos:  float4 clip position per vertex (16-byte aligned); attr: C floats per
  * vertex; tris: int3 per triangle (the index buffer the frame was drawn with).
+ * With num_vertices == 0 (an empty view) pos, attr, grad_pos and grad_attr
+ * may be NULL: nothing is read or written through them.
  *
  * Forward: out[y][x][c] = sum_k b_k attr[tris[t][k]][c] for the triangle t
  * in the single-sample id surface, 0 where no triangle is; tri_image[y][x]

@@ -223,6 +223,31 @@ def test_render_views_mixed_with_single_views(oracle_lib):
         dr.close()
 
 
+def test_render_views_with_an_empty_view(oracle_lib):
+    """A view without triangles in a render_views call: empty ids, a zero
+    image and zero gradients for it; the other views as drawn alone."""
+    torch = _torch()
+    from cudaraster import scenes
+    from cudaraster.diff import DiffRasterizer
+    v = scenes.diff_view(num_tris=2000, width=64, height=48, channels=2, seed=90)
+    dr = DiffRasterizer(64, 48)
+    try:
+        pos = [torch.from_numpy(v.pos).cuda().requires_grad_(True), torch.zeros((0, 4), device="cuda",
+                                                                                 requires_grad=True)]
+        attr = [torch.from_numpy(v.attr).cuda().requires_grad_(True), torch.zeros((0, 2), device="cuda",
+                                                                                   requires_grad=True)]
+        tris = [torch.from_numpy(v.tris).cuda(), torch.zeros((0, 3), dtype=torch.int32, device="cuda")]
+        out, ids = dr.render_views(pos, attr, tris)
+        out.sum().backward()
+        torch.cuda.synchronize()
+        assert np.array_equal(ids[0].cpu().numpy(), oracle_ids(oracle_lib, v))
+        assert bool((ids[1] == -1).all()) and float(out[1].detach().abs().max()) == 0.0
+        assert pos[1].grad.shape == (0, 4) and attr[1].grad.shape == (0, 2)
+        assert float(pos[0].grad.abs().sum()) > 0.0
+    finally:
+        dr.close()
+
+
 def test_async_visibility_overflow_is_reported_then_recovers():
     """After a small sizing view, a much larger view overflows the reserved
     queues in its asynchronous draw: sync() raises instead of returning
